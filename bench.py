@@ -1,0 +1,213 @@
+"""Benchmark: wall time of one THC G0W0 iteration on one MI355X.
+
+Workload (default): N_k = 64 (4x4x4), N_orb = 52, N_mu = 800, beta = 1000, complex128
+throughout, on the shipped IR table with 72 tau points (the nearest to 80; lambda = 1e3,
+eps = 1e-15).  One step = chi0 -> tau_to_w -> Dyson W (+ RPA Ec from the same LU factors)
+-> w_to_tau -> Sigma (+ hermitize), with G(tau,k), X and V resident in HBM.
+
+Prints one JSON line.  Also reported: the complex-FP64 GEMM kernel (the dominant kernel)
+measured with HIP events on its launch stream during the timed steps, and the CPU oracle
+timed on a bounded sample of the same workload and extrapolated to one iteration.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    # name: kgrid, N_orb, N_mu, IR table
+    "cfg2": ((4, 4, 4), 52, 800, "1e3.1e-15"),
+    "cfg1": ((2, 2, 2), 26, 200, "1e2.1e-15"),
+    "cfg3": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(s, ia, G_host_pair, budget_note):
+    """Oracle (C++/OpenMP FP64 restatement of the reference) on a bounded sample:
+    one tau slice of chi0, one tau pair of Sigma, two Dyson and two Ec matrices, one q
+    block of each imaginary-axis transform; extrapolated to a full iteration."""
+    from oracle import oracle
+
+    tab = oracle.load_ir(ia.beta, None, table=ia.name)
+    nk, nmu = s.nk, s.nmu
+    M = nmu * nmu
+    nth, nwh = ia.nt_half, ia.nw_half
+    rng = np.random.default_rng(0)
+    t = {}
+    t0 = time.perf_counter()
+    Pi1 = oracle.pi_rspace(G_host_pair, s.X, s.kgrid)  # nt = 2 -> one tau slice
+    t["pi_slice"] = time.perf_counter() - t0
+    Wqt = np.ascontiguousarray(np.swapaxes(Pi1, 0, 1))  # (nk, 1, nmu, nmu): any W of the right shape
+    t0 = time.perf_counter()
+    oracle.sigma_rspace(G_host_pair, s.X, Wqt, s.kgrid)  # both passes of one tau pair
+    t["sigma_pair"] = time.perf_counter() - t0
+    B = rng.standard_normal((1, 2, nmu, nmu)) + 1j * rng.standard_normal((1, 2, nmu, nmu))
+    Pi = -(B @ np.conj(np.swapaxes(B, -1, -2))) / nmu * 0.05
+    V2 = s.V_host2
+    t0 = time.perf_counter()
+    oracle.dyson(Pi, V2)
+    t["dyson_2"] = time.perf_counter() - t0
+    tab2 = dict(tab)
+    t0 = time.perf_counter()
+    oracle.rpa_ec(tab2, Pi, V2)
+    t["ec_2"] = time.perf_counter() - t0
+    Xt = rng.standard_normal((nth, 1, M)) + 0j
+    t0 = time.perf_counter()
+    Xw = oracle.tau_to_w_phsym(tab, Xt)
+    oracle.w_to_tau_phsym(tab, Xw)
+    t["ft_q"] = time.perf_counter() - t0
+    total = (t["pi_slice"] * nth + (t["dyson_2"] + t["ec_2"]) / 2.0 * nwh * nk + t["ft_q"] * nk
+             + t["sigma_pair"] * nth)
+    sample_s = sum(t.values())
+    return {
+        "value": total,
+        "unit": "s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": ("oracle on one tau slice of chi0, one tau pair of Sigma, 2 Dyson + 2 Ec (q, iW) matrices "
+                   "and one q block of tau<->iW, timed %.1f s and extrapolated to %d tau slices, %d (q, iW) "
+                   "matrices, %d q blocks; %s" % (sample_s, nth, nwh * nk, nk, budget_note)),
+        "sample_times_s": {k: round(v, 3) for k, v in t.items()},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU sharding); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from coqui_amd import IAFT, ThcGW, make_system
+
+    kgrid, no, nmu, table = CONFIGS[args.config]
+    beta = 1000.0
+    log("bench: config %s kgrid=%s no=%d nmu=%d table=%s" % (args.config, kgrid, no, nmu, table))
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    s = make_system(kgrid, no, nmu, system_id=0, with_v=False)
+    ia = IAFT.from_table(beta, table)
+    ctx = ThcGW(kgrid, no, nmu, ia)
+    Zd = torch.from_numpy(s.Z).cuda()
+    Vd = torch.matmul(Zd, Zd.conj().transpose(1, 2)) / nmu  # input preparation, V = Z Z^H / N_mu
+    del Zd
+    Xd = torch.from_numpy(s.X).cuda()
+    Gd = ctx.update_G(s.C, s.E, s.mu)
+    s.V_host2 = Vd[:2].cpu().numpy()
+    G_pair = Gd[[0, ia.nt - 1]].cpu().numpy()
+    nth, nwh = ia.nt_half, ia.nw_half
+    Sig = torch.empty((ia.nt, 1, s.nk, no, no), dtype=torch.complex128, device="cuda")
+    Wqt = torch.empty((s.nk, nth, nmu, nmu), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    log("bench: inputs ready in %.1f s" % (time.perf_counter() - t0))
+
+    stream = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
+        log("bench: warmup %d done, Ec = %.12f" % (i, ec))
+    ctx.profile_read(-1)
+    ctx.set_flag(4, 1)  # GEMM launch events during the timed steps
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for i in range(args.steps):
+        _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
+        log("bench: step %d done" % i)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    ctx.set_flag(4, 0)
+    g_ms, g_fl, g_n = ctx.profile_read(0)   # N,N class: Dyson, LU and triangular-solve GEMMs
+    a_ms, a_fl, a_n = ctx.profile_read(-1)  # remaining classes (projections)
+    a_ms, a_fl, a_n = a_ms + g_ms, a_fl + g_fl, a_n + g_n
+    achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+
+    stages = None
+    if args.breakdown:
+        stages = {}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        Pit = Wqt
+        Piw = torch.empty((nwh, s.nk, nmu, nmu), dtype=torch.complex128, device="cuda")
+        ev[0].record(stream)
+        ctx.pi(Gd, Xd, out=Pit)
+        ev[1].record(stream)
+        ctx.tau_to_w(Pit, out=Piw)
+        ev[2].record(stream)
+        ctx.dyson_w(Piw, Vd, with_terms=True)
+        ev[3].record(stream)
+        ctx.w_to_tau(Piw, out=Wqt)
+        ev[4].record(stream)
+        ctx.sigma(Gd, Wqt, Xd, hermitize=True, out=Sig)
+        ev[5].record(stream)
+        torch.cuda.synchronize()
+        for i, name in enumerate(["chi0", "tau_to_w", "dyson+ec_terms", "w_to_tau", "sigma"]):
+            stages[name] = round(ev[i].elapsed_time(ev[i + 1]), 2)
+        del Piw
+        log("bench: stages (ms) %s" % stages)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU oracle sample ...")
+        cpu = cpu_baseline(s, ia, G_pair, "FP64, all host threads")
+        log("bench: CPU baseline %.1f s per iteration (extrapolated)" % cpu["value"])
+
+    out = {
+        "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
+        "value": round(ms_step / 1000.0, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
+        "config": {"workload": "%s plain FP64 THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % args.config,
+                   "kgrid": list(kgrid), "nk": s.nk, "n_orb": no, "n_mu": nmu, "n_tau": ia.nt,
+                   "n_iw_b": ia.nw_b, "ir_table": table, "beta": beta, "parallelism": "single GPU"},
+        "Ec_Ha": ec,
+        "speedup_vs_plain": 1.0,
+        "qp_error_meV": 0.0,
+        "roofline": {"kernel": "zgemm_kernel<0,0> (complex FP64 GEMM, N,N)", "bound": "mfma",
+                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "launches": g_n, "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
+                     "share_of_step_time": round(g_ms / max(ms_step * args.steps, 1e-9), 4),
+                     "all_gemm_tflops": round(a_fl / (a_ms * 1e-3) / 1e12, 3) if a_ms > 0 else 0.0},
+        "cpu_baseline": cpu,
+        "stages_ms": stages,
+        "wall_s_timed": round(wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,12 @@
+"""coqui_amd: MI355X-native THC imaginary-time GW/RPA inner loop.
+
+The compute lives in libthcgw.so (hand-written HIP kernels for gfx950 behind the C ABI of
+include/thcgw.h).  This package holds the Python binding (thcgw.ThcGW), the IR tables
+(iaft.IAFT) and the seeded synthetic-system generator (system.make_system).
+"""
+from .iaft import IAFT, available_tables, determine_lambda  # noqa: F401
+from .system import System, make_system  # noqa: F401
+from .thcgw import EXPORTS, LIB_PATH, ThcGW, ThcgwError, load_library  # noqa: F401
+
+__all__ = ["IAFT", "available_tables", "determine_lambda", "System", "make_system", "ThcGW", "ThcgwError",
+           "load_library", "EXPORTS", "LIB_PATH"]

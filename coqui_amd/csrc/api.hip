@@ -1,0 +1,555 @@
+// C ABI of libthcgw: context, workspaces and the stage pipelines of one THC GW iteration.
+#include "../../include/thcgw.h"
+#include "thcgw_internal.hpp"
+#include "thcgw_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using thcgw::c128;
+using thcgw::Cplx;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct thcgw_ctx {
+  thcgw_dims d{};
+  int nk = 0, nt_half = 0, nw_half = 0;
+  long M = 0;  // nmu^2
+  int device = 0;
+  thcgw::host::Tables tab;  // rescaled host tables
+  double* d_tau = nullptr;  // tau values (nt)
+  c128* d_Tplus = nullptr;  // nw_half x nt_half (IAFT.icc:62-70)
+  c128* d_Tminus = nullptr; // nt_half x nw_half (IAFT.icc:117-125)
+  c128* d_Twt_ff = nullptr; // nw_f x nt
+  std::map<std::string, DevBuf> ws;
+  std::string err;
+  bool force_fallback = false;
+  int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
+  int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
+
+  template <class T>
+  T* buf(const std::string& name, size_t count) {
+    DevBuf& b = ws[name];
+    const size_t need = count * sizeof(T);
+    if (b.bytes < need) {
+      if (b.p) THCGW_HIP(hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+      hipError_t e = hipMalloc(&b.p, need);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw thcgw::Error(THCGW_ERR_ALLOC, "hipMalloc of " + std::to_string(need) + " bytes for workspace '" +
+                                                        name + "' failed: " + hipGetErrorString(e));
+      }
+      b.bytes = need;
+    }
+    return static_cast<T*>(b.p);
+  }
+  void release() {
+    for (auto& kv : ws)
+      if (kv.second.p) (void)hipFree(kv.second.p);
+    ws.clear();
+    if (d_tau) (void)hipFree(d_tau);
+    if (d_Tplus) (void)hipFree(d_Tplus);
+    if (d_Tminus) (void)hipFree(d_Tminus);
+    if (d_Twt_ff) (void)hipFree(d_Twt_ff);
+    d_tau = nullptr;
+    d_Tplus = d_Tminus = d_Twt_ff = nullptr;
+  }
+};
+
+namespace {
+
+template <class F>
+int guarded(thcgw_ctx* ctx, F&& f) {
+  try {
+    if (ctx) THCGW_HIP(hipSetDevice(ctx->device));
+    f();
+    return THCGW_OK;
+  } catch (const thcgw::Error& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return THCGW_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return THCGW_ERR_ARG;
+  }
+}
+
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+inline const c128* C(const double* p) { return reinterpret_cast<const c128*>(p); }
+inline c128* Cm(double* p) { return reinterpret_cast<c128*>(p); }
+
+const Cplx kOne{1.0, 0.0};
+const Cplx kZero{0.0, 0.0};
+const Cplx kMinusOne{-1.0, 0.0};
+
+// Free device memory the pipelines may use for batching, leaving headroom.
+size_t free_device_bytes() {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return size_t(8) << 30;
+  }
+  return fr;
+}
+
+// O_out[b,k] = X_k O_in[b,k] X_k^H for nbk = nb*nk matrices (thc_solver_comm.hpp:422,427).
+void project_p2a(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, c128* Oout, long nbk, c128* Abuf) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
+  const long sX = static_cast<long>(nmu) * no;
+  THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(X, sX, nk), no,
+               thcgw::bp(Oin, static_cast<long>(no) * no), no, kZero, Abuf, sX, no, static_cast<int>(nbk));
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, nmu, nmu, no, kOne, thcgw::bp(Abuf, sX), no, thcgw::bp(X, sX, nk), no,
+               kZero, Oout, ctx->M, nmu, static_cast<int>(nbk));
+}
+
+// O_out[b,k] = X_k^H O_in[b,k] X_k (thc_solver_comm.hpp:560,563).
+void project_a2p(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, c128* Oout, long nbk, c128* Bbuf) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
+  const long sX = static_cast<long>(nmu) * no;
+  THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
+  thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X, sX, nk), no, thcgw::bp(Oin, ctx->M), nmu,
+               kZero, Bbuf, sX, nmu, static_cast<int>(nbk));
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), nmu, thcgw::bp(X, sX, nk), no,
+               kZero, Oout, static_cast<long>(no) * no, no, static_cast<int>(nbk));
+}
+
+int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
+  if (ctx->pair_chunk > 0) return std::min(ctx->pair_chunk, ctx->nt_half);
+  const double per = double(per_pair_arrays) * ctx->nk * double(ctx->M) * 16.0;
+  const double budget = std::min(0.25 * double(free_device_bytes()), 24.0 * double(1L << 30));
+  int c = static_cast<int>(budget / per);
+  return std::max(1, std::min(c, ctx->nt_half));
+}
+
+void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const long M = ctx->M;
+  const long gs = static_cast<long>(nk) * no * no;  // one tau slice of G
+  const int P = choose_pair_chunk(ctx, 2);
+  c128* GP = ctx->buf<c128>("pi_gp", static_cast<size_t>(P) * nk * M);
+  c128* GN = ctx->buf<c128>("pi_gn", static_cast<size_t>(P) * nk * M);
+  c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(P) * nk * nmu * no);
+  const double scale = -2.0 / (double(nk) * nk);  // sp = -2 for ns = 1 (rpa_pi.icc:46)
+  for (int i0 = 0; i0 < nth; i0 += P) {
+    const int c = std::min(P, nth - i0);
+    project_p2a(ctx, s, X, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
+    // G(beta - tau_i) = G[nt-1-i] for i in [i0, i0+c): tau slots [nt-i0-c, nt-i0), reversed
+    project_p2a(ctx, s, X, G + (nt - i0 - c) * gs, GN, static_cast<long>(c) * nk, Ab);
+    for (int j = 0; j < c; ++j)
+      thcgw::pi_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
+                     GN + static_cast<long>(c - 1 - j) * nk * M, Pi + static_cast<long>(i0 + j) * nk * M, M, scale);
+  }
+}
+
+void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
+  const long M = ctx->M, nk = ctx->nk;
+  thcgw::ir_transform(s, ctx->nw_half, ctx->nt_half, ctx->d_Tplus, Pit, nk * M, M, Piw, nk * M, M, M, ctx->nk);
+}
+
+void run_w_to_tau(thcgw_ctx* ctx, const c128* Ww, c128* Wqt, hipStream_t s) {
+  const long M = ctx->M, nk = ctx->nk;
+  thcgw::ir_transform(s, ctx->nt_half, ctx->nw_half, ctx->d_Tminus, Ww, nk * M, M, Wqt, M, ctx->nt_half * M, M,
+                      ctx->nk);
+}
+
+int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
+  if (ctx->dyson_chunk > 0) return std::min(ctx->dyson_chunk, total);
+  const double per = 2.0 * double(ctx->M) * 16.0;
+  const double budget = std::min(0.3 * double(free_device_bytes()), 32.0 * double(1L << 30));
+  int c = static_cast<int>(budget / per);
+  return std::max(1, std::min(c, total));
+}
+
+// In place Pi(iW) -> W(iW) - V; optional Tr(V Pi) and ln|Re det(I - V Pi)| per matrix
+// (device arrays of length nw_half*nk).  pi_first selects A = I - Pi V (thc_rpa.icc:141)
+// instead of I - V Pi, and solve = false stops after the determinant.
+void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* d_ldet, bool solve, bool pi_first,
+               hipStream_t s) {
+  const int n = ctx->d.nmu, nq = ctx->nk;
+  const long M = ctx->M;
+  const int total = ctx->nw_half * nq;
+  const int CH = choose_dyson_chunk(ctx, total);
+  c128* LU = ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M);
+  c128* Z = solve ? ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M) : nullptr;
+  int* ipiv = ctx->buf<int>("dy_ipiv", static_cast<size_t>(CH) * n);
+  int* info_all = ctx->buf<int>("dy_info", static_cast<size_t>(total));
+  for (int b0 = 0; b0 < total; b0 += CH) {
+    const int cb = std::min(CH, total - b0);
+    // A = I - V Pi   (scr_coulomb_t.cpp:180-185)  or  A = I - Pi V  (thc_rpa.icc:141-150)
+    if (!pi_first)
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(V, M, nq, 1, b0), n,
+                   thcgw::bp(PiW + static_cast<long>(b0) * M, M), n, kZero, LU, M, n, cb, kOne);
+    else
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(PiW + static_cast<long>(b0) * M, M), n,
+                   thcgw::bp(V, M, nq, 1, b0), n, kZero, LU, M, n, cb, kOne);
+    if (d_trace) thcgw::one_minus_trace(s, LU, M, n, d_trace + b0, cb);
+    thcgw::getrf_batched(s, LU, M, n, n, ipiv, info_all + b0, cb);
+    if (d_ldet) thcgw::log_re_det(s, LU, M, n, ipiv, d_ldet + b0, cb);
+    if (solve) {
+      // (I - V Pi) Y = V  ->  W - V = Y - V   (replaces getri + GEMM, scr_coulomb_t.cpp:188-196)
+      thcgw::copy_batch(s, V, M, nq, Z, M, M, b0, cb);
+      thcgw::trsm_left_lower(s, LU, M, n, Z, M, n, n, n, cb);
+      thcgw::trsm_left_upper_unit(s, LU, M, n, Z, M, n, n, n, cb);
+      thcgw::dyson_finish(s, Z, M, V, M, nq, ipiv, n, PiW + static_cast<long>(b0) * M, M, b0, cb);
+    }
+  }
+  std::vector<int> info(total);
+  THCGW_HIP(hipMemcpyAsync(info.data(), info_all, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < total; ++b)
+    if (info[b] != 0)
+      throw thcgw::Error(THCGW_ERR_SINGULAR,
+                         "Dyson: (I - V Pi) is singular at (n, q) = (" + std::to_string(b / nq) + ", " +
+                             std::to_string(b % nq) + "), zero pivot " + std::to_string(info[b] - 1));
+}
+
+// Ec from per-(n, q) Tr and ln|Re det| (thc_rpa.icc:130-173, q_weight = 1).
+double assemble_ec(thcgw_ctx* ctx, const std::vector<std::complex<double>>& tr, const std::vector<double>& ld) {
+  const int nq = ctx->nk, nwb = ctx->d.nw_b;
+  std::vector<std::complex<double>> ew(nwb, 0.0);
+  for (int n = 0; n < ctx->nw_half; ++n)
+    for (int q = 0; q < nq; ++q) {
+      const int g = n * nq + q;
+      const int wn = nwb / 2 + n, mwn = nwb / 2 - n;
+      const std::complex<double> v = tr[g] + ld[g];
+      ew[wn] += v;
+      if (mwn != wn) ew[mwn] += v;
+    }
+  return thcgw::host::ec_from_ew(ctx->tab, ew, nq);
+}
+
+void fetch_ec_terms(thcgw_ctx* ctx, const c128* d_tr, const double* d_ld, std::vector<std::complex<double>>& tr,
+                    std::vector<double>& ld, hipStream_t s) {
+  const int total = ctx->nw_half * ctx->nk;
+  tr.resize(total);
+  ld.resize(total);
+  THCGW_HIP(hipMemcpyAsync(tr.data(), d_tr, sizeof(c128) * total, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipMemcpyAsync(ld.data(), d_ld, sizeof(double) * total, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+}
+
+void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm, hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const long M = ctx->M;
+  const long gs = static_cast<long>(nk) * no * no;
+  const int P = choose_pair_chunk(ctx, 4);
+  c128* GP = ctx->buf<c128>("sg_gp", static_cast<size_t>(P) * nk * M);
+  c128* GN = ctx->buf<c128>("sg_gn", static_cast<size_t>(P) * nk * M);
+  c128* SP = ctx->buf<c128>("sg_sp", static_cast<size_t>(P) * nk * M);
+  c128* SN = ctx->buf<c128>("sg_sn", static_cast<size_t>(P) * nk * M);
+  c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(P) * nk * nmu * no);
+  const double scale = -1.0 / (double(nk) * nk);
+  for (int i0 = 0; i0 < nth; i0 += P) {
+    const int c = std::min(P, nth - i0);
+    // second pass only for i < nt/2 (thc_gw.icc:53-62 writes Sigma(beta - tau_i) at nt-1-i)
+    const int c2 = std::max(0, std::min(c, nt / 2 - i0));
+    project_p2a(ctx, s, X, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
+    if (c2 > 0) project_p2a(ctx, s, X, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab);
+    for (int j = 0; j < c; ++j) {
+      const int i = i0 + j;
+      const bool second = j < c2;
+      thcgw::sigma_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
+                        second ? GN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, Wqt + static_cast<long>(i) * M,
+                        static_cast<long>(nth) * M, SP + static_cast<long>(j) * nk * M,
+                        second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale);
+    }
+    project_a2p(ctx, s, X, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);
+    if (c2 > 0) project_a2p(ctx, s, X, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
+  }
+  if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
+}
+
+}  // namespace
+
+extern "C" {
+
+int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ctx** out) {
+  g_create_error.clear();
+  if (!dims || !tables || !out) {
+    g_create_error = "thcgw_create: null argument";
+    return THCGW_ERR_ARG;
+  }
+  *out = nullptr;
+  thcgw_ctx* ctx = new (std::nothrow) thcgw_ctx();
+  if (!ctx) {
+    g_create_error = "thcgw_create: out of host memory";
+    return THCGW_ERR_ALLOC;
+  }
+  int rc = guarded(nullptr, [&] {
+    const thcgw_dims& d = *dims;
+    THCGW_REQUIRE(d.kgrid[0] >= 1 && d.kgrid[1] >= 1 && d.kgrid[2] >= 1, "kgrid entries must be >= 1");
+    THCGW_REQUIRE(d.ns == 1, "only ns = 1 is supported (no spin polarisation)");
+    THCGW_REQUIRE(d.no >= 1 && d.nmu >= 1 && d.nt >= 1 && d.nw_b >= 1 && d.nw_f >= 2, "sizes must be positive");
+    THCGW_REQUIRE(d.beta > 0.0, "beta must be positive");
+    THCGW_REQUIRE(d.nmu <= 2048, "nmu above 2048 is not supported by the batched LU");
+    ctx->d = d;
+    ctx->nk = d.kgrid[0] * d.kgrid[1] * d.kgrid[2];
+    THCGW_REQUIRE(thcgw::kfft_supported(d.kgrid), "k-grid not supported (cubic n<=4, or nk<=16)");
+    ctx->nt_half = (d.nt % 2 == 0) ? d.nt / 2 : d.nt / 2 + 1;
+    ctx->nw_half = (d.nw_b % 2 == 0) ? d.nw_b / 2 : d.nw_b / 2 + 1;
+    ctx->M = static_cast<long>(d.nmu) * d.nmu;
+    thcgw::host::load_tables(d, *tables, ctx->tab);  // checks PH symmetry of the tau mesh
+    THCGW_HIP(hipGetDevice(&ctx->device));
+    std::vector<std::complex<double>> Tp, Tm;
+    thcgw::host::fold_ph(ctx->tab, d, Tp, Tm);
+    THCGW_HIP(hipMalloc(&ctx->d_tau, sizeof(double) * d.nt));
+    THCGW_HIP(hipMalloc(&ctx->d_Tplus, sizeof(c128) * Tp.size()));
+    THCGW_HIP(hipMalloc(&ctx->d_Tminus, sizeof(c128) * Tm.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_tau, ctx->tab.tau.data(), sizeof(double) * d.nt, hipMemcpyHostToDevice));
+    THCGW_HIP(hipMemcpy(ctx->d_Tplus, Tp.data(), sizeof(c128) * Tp.size(), hipMemcpyHostToDevice));
+    THCGW_HIP(hipMemcpy(ctx->d_Tminus, Tm.data(), sizeof(c128) * Tm.size(), hipMemcpyHostToDevice));
+    THCGW_HIP(hipMalloc(&ctx->d_Twt_ff, sizeof(c128) * ctx->tab.Twt_ff.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_Twt_ff, ctx->tab.Twt_ff.data(), sizeof(c128) * ctx->tab.Twt_ff.size(),
+                        hipMemcpyHostToDevice));
+  });
+  if (rc != THCGW_OK) {
+    g_create_error = ctx->err.empty() ? "thcgw_create failed" : ctx->err;
+    ctx->release();
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return THCGW_OK;
+}
+
+void thcgw_destroy(thcgw_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  ctx->release();
+  delete ctx;
+}
+
+const char* thcgw_last_error(const thcgw_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+long thcgw_workspace_bytes(const thcgw_ctx* ctx) {
+  if (!ctx) return 0;
+  long b = 0;
+  for (auto& kv : ctx->ws) b += static_cast<long>(kv.second.bytes);
+  return b;
+}
+
+int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (flag == 1) ctx->force_fallback = value != 0;
+    else if (flag == 2) ctx->pair_chunk = static_cast<int>(value);
+    else if (flag == 3) ctx->dyson_chunk = static_cast<int>(value);
+    else if (flag == 4) thcgw::gemm_profile_enable(value != 0);
+    else THCGW_REQUIRE(false, "unknown flag");
+  });
+}
+
+int thcgw_profile_read(thcgw_ctx* ctx, int kernel_class, double* ms, double* flops, long* launches) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(ms && flops && launches, "thcgw_profile_read: null pointer");
+    thcgw::gemm_profile_read(kernel_class, ms, flops, launches);
+  });
+}
+
+int thcgw_update_G(thcgw_ctx* ctx, const double* Cmo, const double* E, double mu, double* G, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(Cmo && E && G, "thcgw_update_G: null pointer");
+    const int nt = ctx->d.nt, nk = ctx->nk, no = ctx->d.no;
+    const long sm = static_cast<long>(no) * no;
+    c128* CG = ctx->buf<c128>("ug_cg", static_cast<size_t>(nt) * nk * sm);
+    thcgw::scale_mo(S(stream), C(Cmo), E, ctx->d_tau, mu, ctx->d.beta, nt, nk, no, CG);
+    // G = (C g) C^H   (qp_scf_common.cpp:105-115)
+    thcgw::zgemm(S(stream), thcgw::OP_N, thcgw::OP_C, no, no, no, kOne, thcgw::bp(CG, sm), no,
+                 thcgw::bp(C(Cmo), sm, nk), no, kZero, Cm(G), sm, no, nt * nk);
+  });
+}
+
+int thcgw_primary_to_aux(thcgw_ctx* ctx, long nb, const double* X, const double* Oin, double* Oout, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(X && Oin && Oout && nb >= 0, "thcgw_primary_to_aux: bad argument");
+    const long nbk = nb * ctx->nk;
+    c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(nbk) * ctx->d.nmu * ctx->d.no);
+    project_p2a(ctx, S(stream), C(X), C(Oin), Cm(Oout), nbk, Ab);
+  });
+}
+
+int thcgw_aux_to_primary(thcgw_ctx* ctx, long nb, const double* X, const double* Oin, double* Oout, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(X && Oin && Oout && nb >= 0, "thcgw_aux_to_primary: bad argument");
+    const long nbk = nb * ctx->nk;
+    c128* Bb = ctx->buf<c128>("proj_ab", static_cast<size_t>(nbk) * ctx->d.nmu * ctx->d.no);
+    project_a2p(ctx, S(stream), C(X), C(Oin), Cm(Oout), nbk, Bb);
+  });
+}
+
+int thcgw_pi(thcgw_ctx* ctx, const double* G, const double* X, double* Pi, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(G && X && Pi, "thcgw_pi: null pointer");
+    run_pi(ctx, C(G), C(X), Cm(Pi), S(stream));
+  });
+}
+
+int thcgw_tau_to_w(thcgw_ctx* ctx, const double* Pit, double* Piw, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(Pit && Piw && Pit != Piw, "thcgw_tau_to_w: bad pointers");
+    run_tau_to_w(ctx, C(Pit), Cm(Piw), S(stream));
+  });
+}
+
+int thcgw_w_to_tau(thcgw_ctx* ctx, const double* Ww, double* Wqt, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(Ww && Wqt && Ww != Wqt, "thcgw_w_to_tau: bad pointers");
+    run_w_to_tau(ctx, C(Ww), Cm(Wqt), S(stream));
+  });
+}
+
+int thcgw_dyson_w(thcgw_ctx* ctx, double* PiW, const double* V, double* trace_out, double* logdet_out, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(PiW && V, "thcgw_dyson_w: null pointer");
+    const int total = ctx->nw_half * ctx->nk;
+    const bool terms = trace_out || logdet_out;
+    c128* dtr = terms ? ctx->buf<c128>("ec_tr", total) : nullptr;
+    double* dld = terms ? ctx->buf<double>("ec_ld", total) : nullptr;
+    run_dyson(ctx, Cm(PiW), C(V), dtr, dld, true, false, S(stream));
+    if (terms) {
+      std::vector<std::complex<double>> tr;
+      std::vector<double> ld;
+      fetch_ec_terms(ctx, dtr, dld, tr, ld, S(stream));
+      if (trace_out) std::memcpy(trace_out, tr.data(), sizeof(double) * 2 * total);
+      if (logdet_out) std::memcpy(logdet_out, ld.data(), sizeof(double) * total);
+    }
+  });
+}
+
+int thcgw_sigma(thcgw_ctx* ctx, const double* G, const double* Wqt, const double* X, double* Sig, int herm,
+                void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(G && Wqt && X && Sig, "thcgw_sigma: null pointer");
+    run_sigma(ctx, C(G), C(Wqt), C(X), Cm(Sig), herm != 0, S(stream));
+  });
+}
+
+int thcgw_hermitize(thcgw_ctx* ctx, long nb, int n, double* A, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(A && nb >= 0 && n >= 0, "thcgw_hermitize: bad argument");
+    thcgw::hermitize(S(stream), Cm(A), n, nb);
+  });
+}
+
+int thcgw_rpa_ec(thcgw_ctx* ctx, const double* Piw, const double* V, double* Ec, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(Piw && V && Ec, "thcgw_rpa_ec: null pointer");
+    const int total = ctx->nw_half * ctx->nk;
+    c128* dtr = ctx->buf<c128>("ec_tr", total);
+    double* dld = ctx->buf<double>("ec_ld", total);
+    // Pi_wqPQ is read only: factor copies of (I - Pi V) in the Dyson workspace.
+    run_dyson(ctx, const_cast<c128*>(C(Piw)), C(V), dtr, dld, false, true, S(stream));
+    std::vector<std::complex<double>> tr;
+    std::vector<double> ld;
+    fetch_ec_terms(ctx, dtr, dld, tr, ld, S(stream));
+    *Ec = assemble_ec(ctx, tr, ld);
+  });
+}
+
+int thcgw_gw_iteration(thcgw_ctx* ctx, const double* G, const double* X, const double* V, double* Sig, double* Wqt,
+                       double* Ec, int herm, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(G && X && V && Sig && Wqt, "thcgw_gw_iteration: null pointer");
+    hipStream_t s = S(stream);
+    const long M = ctx->M;
+    const int total = ctx->nw_half * ctx->nk;
+    c128* Pit = Cm(Wqt);  // W_qtPQ doubles as Pi_tqPQ scratch (same size)
+    c128* Piw = ctx->buf<c128>("it_piw", static_cast<size_t>(total) * M);
+    c128* dtr = Ec ? ctx->buf<c128>("ec_tr", total) : nullptr;
+    double* dld = Ec ? ctx->buf<double>("ec_ld", total) : nullptr;
+    run_pi(ctx, C(G), C(X), Pit, s);
+    run_tau_to_w(ctx, Pit, Piw, s);
+    run_dyson(ctx, Piw, C(V), dtr, dld, true, false, s);
+    run_w_to_tau(ctx, Piw, Cm(Wqt), s);
+    run_sigma(ctx, C(G), Cm(Wqt), C(X), Cm(Sig), herm != 0, s);
+    if (Ec) {
+      std::vector<std::complex<double>> tr;
+      std::vector<double> ld;
+      fetch_ec_terms(ctx, dtr, dld, tr, ld, s);
+      *Ec = assemble_ec(ctx, tr, ld);
+    }
+    THCGW_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int thcgw_solve_qp(thcgw_ctx* ctx, const double* Sigma, const double* Cmo, const double* Vhf, const double* E0,
+                   double mu, int nfit, double eta, double tol, double* E_out, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(Sigma && Cmo && Vhf && E0 && E_out, "thcgw_solve_qp: null pointer");
+    hipStream_t s = S(stream);
+    const int nt = ctx->d.nt, nk = ctx->nk, no = ctx->d.no;
+    const long sm = static_cast<long>(no) * no;
+    c128* T1 = ctx->buf<c128>("qp_t1", static_cast<size_t>(nt) * nk * sm);
+    c128* dg = ctx->buf<c128>("qp_diag", static_cast<size_t>(nt + 1) * nk * no);
+    // diag(C^H Sigma C) per (tau, k) and diag(C^H Vhf C) per k  (qp_scf_common.cpp:190-205, 213-231)
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, no, kOne, thcgw::bp(C(Sigma), sm), no, thcgw::bp(C(Cmo), sm, nk),
+                 no, kZero, T1, sm, no, nt * nk);
+    thcgw::mo_diag(s, T1, C(Cmo), dg, nk, no, static_cast<long>(nt) * nk);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, no, kOne, thcgw::bp(C(Vhf), sm), no, thcgw::bp(C(Cmo), sm, nk),
+                 no, kZero, T1, sm, no, nk);
+    thcgw::mo_diag(s, T1, C(Cmo), dg + static_cast<long>(nt) * nk * no, nk, no, nk);
+    std::vector<std::complex<double>> hd(static_cast<size_t>(nt + 1) * nk * no);
+    THCGW_HIP(hipMemcpyAsync(hd.data(), dg, sizeof(c128) * hd.size(), hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    thcgw::host::solve_qp_diag(ctx->tab, nk, no, hd.data(), hd.data() + static_cast<size_t>(nt) * nk * no, E0, mu,
+                               nfit, eta, tol, E_out);
+  });
+}
+
+int thcgw_gm_energy(thcgw_ctx* ctx, const double* G, const double* Sigma, double* e_corr, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(G && Sigma && e_corr, "thcgw_gm_energy: null pointer");
+    hipStream_t s = S(stream);
+    const int nt = ctx->d.nt, nw = ctx->d.nw_f, nk = ctx->nk, no = ctx->d.no;
+    const long dim = static_cast<long>(nk) * no * no;
+    c128* Gw = ctx->buf<c128>("gm_gw", static_cast<size_t>(nw) * dim);
+    c128* Sw = ctx->buf<c128>("gm_sw", static_cast<size_t>(nw) * dim);
+    c128* sg = ctx->buf<c128>("gm_sg", static_cast<size_t>(nw));
+    // tau -> iw_n on the fermionic mesh (IAFT.icc:9-26), then sum_k w_k Tr[Sigma G] per frequency
+    thcgw::ir_transform(s, nw, nt, ctx->d_Twt_ff, C(G), dim, 0, Gw, dim, 0, dim, 1);
+    thcgw::ir_transform(s, nw, nt, ctx->d_Twt_ff, C(Sigma), dim, 0, Sw, dim, 0, dim, 1);
+    thcgw::trace_prod(s, Sw, Gw, nw, nk, no, 1.0 / nk, sg);
+    std::vector<std::complex<double>> h(nw);
+    THCGW_HIP(hipMemcpyAsync(h.data(), sg, sizeof(c128) * nw, hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    *e_corr = thcgw::host::gm_from_sigmag(ctx->tab, h);
+  });
+}
+
+}  // extern "C"

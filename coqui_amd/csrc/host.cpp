@@ -1,0 +1,261 @@
+// Host-side parts of libthcgw: IR table handling, RPA energy assembly and the
+// quasiparticle equation (Thiele Pade continuation + bisection).
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "thcgw_host.hpp"
+#include "thcgw_internal.hpp"
+
+namespace thcgw {
+namespace host {
+
+void load_tables(const thcgw_dims& d, const thcgw_ir_tables& t, Tables& o) {
+  THCGW_REQUIRE(t.tau_mesh_f && t.wn_mesh_f && t.Ttw_ff && t.Twt_ff && t.Ttw_bb && t.Twt_bb && t.T_beta_t_ff,
+                "IR tables: null pointer");
+  o.nt = d.nt;
+  o.nw_b = d.nw_b;
+  o.nw_f = d.nw_f;
+  o.beta = d.beta;
+  o.x.assign(t.tau_mesh_f, t.tau_mesh_f + d.nt);
+  o.tau.resize(d.nt);
+  for (int i = 0; i < d.nt; ++i) o.tau[i] = (o.x[i] + 1.0) * d.beta / 2.0;
+  for (int i = 0; i < d.nt; ++i) {
+    const double diff = std::fabs(o.x[i]) - std::fabs(o.x[d.nt - i - 1]);
+    THCGW_REQUIRE(diff <= 1e-6, "IAFT grid is not compatible with particle-hole symmetry at tau index " +
+                                    std::to_string(i));
+  }
+  o.wn_f.assign(t.wn_mesh_f, t.wn_mesh_f + d.nw_f);
+  auto cp = [](const double* p, size_t n, double s) {
+    std::vector<cd> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = cd(p[2 * i], p[2 * i + 1]) * s;
+    return v;
+  };
+  const double r2 = std::sqrt(2.0);
+  o.Ttw_ff = cp(t.Ttw_ff, size_t(d.nt) * d.nw_f, r2 / d.beta);
+  o.Twt_ff = cp(t.Twt_ff, size_t(d.nw_f) * d.nt, d.beta / r2);
+  o.Ttw_bb = cp(t.Ttw_bb, size_t(d.nt) * d.nw_b, r2 / d.beta);
+  o.Twt_bb = cp(t.Twt_bb, size_t(d.nw_b) * d.nt, d.beta / r2);
+  o.T_beta = cp(t.T_beta_t_ff, size_t(d.nt), 1.0);
+}
+
+void fold_ph(const Tables& tab, const thcgw_dims& d, std::vector<cd>& Tp, std::vector<cd>& Tm) {
+  const int nt = d.nt, nwb = d.nw_b;
+  const int nth = (nt % 2 == 0) ? nt / 2 : nt / 2 + 1;
+  const int nwh = (nwb % 2 == 0) ? nwb / 2 : nwb / 2 + 1;
+  Tp.assign(size_t(nwh) * nth, cd(0.0));
+  Tm.assign(size_t(nth) * nwh, cd(0.0));
+  for (int n = 0; n < nwh; ++n) {
+    const int iw = nwb / 2 + n;
+    for (int it = 0; it < nth; ++it) {
+      const int imt = nt - it - 1;
+      const cd a = tab.Twt_bb[size_t(iw) * nt + it];
+      Tp[size_t(n) * nth + it] = (it == imt) ? a : a + tab.Twt_bb[size_t(iw) * nt + imt];
+    }
+  }
+  for (int it = 0; it < nth; ++it)
+    for (int n = 0; n < nwh; ++n) {
+      const int iw = nwb / 2 + n, imw = nwb / 2 - n;
+      const cd a = tab.Ttw_bb[size_t(it) * nwb + iw];
+      Tm[size_t(it) * nwh + n] = (iw == imw) ? a : a + tab.Ttw_bb[size_t(it) * nwb + imw];
+    }
+}
+
+double ec_from_ew(const Tables& tab, const std::vector<cd>& ew, int nq) {
+  cd e = 0.0;
+  for (int it = 0; it < tab.nt; ++it) {
+    cd et = 0.0;
+    for (int w = 0; w < tab.nw_b; ++w) et += tab.Ttw_bb[size_t(it) * tab.nw_b + w] * ew[w];
+    e += tab.T_beta[it] * et;
+  }
+  return e.real() / (2.0 * double(nq));
+}
+
+double gm_from_sigmag(const Tables& tab, const std::vector<cd>& sg) {
+  cd e = 0.0;
+  for (int it = 0; it < tab.nt; ++it) {
+    cd st = 0.0;
+    for (int w = 0; w < tab.nw_f; ++w) st += tab.Ttw_ff[size_t(it) * tab.nw_f + w] * sg[w];
+    e += tab.T_beta[it] * st;
+  }
+  return (-0.5 * 2.0 * e).real();  // spin factor 2 for ns = 1 (scf_common.cpp:130-131)
+}
+
+std::vector<int> pade_fit_indices(int niw, int nfit) {
+  THCGW_REQUIRE(niw % 2 == 0, "Pade: the fermionic mesh size must be even");
+  const int niw_pos = niw / 2, shift = niw / 2;
+  THCGW_REQUIRE(nfit >= 2 && nfit <= niw_pos, "Pade: nfit must be in [2, nw_f/2]");
+  std::vector<int> idx;
+  const int num_lchunk = niw_pos % nfit;
+  if (num_lchunk != 0) {
+    const int step = niw_pos / nfit + 1;
+    for (int i = 0; i < step * num_lchunk; i += step) idx.push_back(i + shift);
+    for (int i = step * num_lchunk; i < niw_pos; i += step - 1) idx.push_back(i + shift);
+  } else {
+    const int step = niw_pos / nfit;
+    for (int i = 0; i < niw_pos; i += step) idx.push_back(i + shift);
+  }
+  THCGW_REQUIRE(static_cast<int>(idx.size()) == nfit, "Pade: fit-frequency selection size mismatch");
+  return idx;
+}
+
+namespace {
+
+// Double-double arithmetic (Hida, Li & Bailey, "Algorithms for quad-double precision
+// floating point arithmetic", ARITH-15, 2001): a value is hi + lo with |lo| <= ulp(hi)/2.
+struct dd {
+  double hi, lo;
+};
+inline dd two_sum(double a, double b) {
+  const double s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+inline dd quick_two_sum(double a, double b) {
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+inline dd two_prod(double a, double b) {
+  const double p = a * b;
+  return {p, std::fma(a, b, -p)};
+}
+inline dd operator+(dd x, dd y) {
+  dd s = two_sum(x.hi, y.hi), t = two_sum(x.lo, y.lo);
+  s.lo += t.hi;
+  s = quick_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return quick_two_sum(s.hi, s.lo);
+}
+inline dd operator-(dd x) { return {-x.hi, -x.lo}; }
+inline dd operator-(dd x, dd y) { return x + (-y); }
+inline dd operator*(dd x, dd y) {
+  dd p = two_prod(x.hi, y.hi);
+  p.lo += x.hi * y.lo + x.lo * y.hi;
+  return quick_two_sum(p.hi, p.lo);
+}
+inline dd operator*(dd x, double y) {
+  dd p = two_prod(x.hi, y);
+  p.lo += x.lo * y;
+  return quick_two_sum(p.hi, p.lo);
+}
+inline dd operator/(dd x, dd y) {
+  const double q1 = x.hi / y.hi;
+  dd r = x - y * q1;
+  const double q2 = r.hi / y.hi;
+  r = r - y * q2;
+  const double q3 = r.hi / y.hi;
+  dd q = quick_two_sum(q1, q2);
+  return q + dd{q3, 0.0};
+}
+
+struct zdd {
+  dd re, im;
+};
+inline zdd operator+(zdd a, zdd b) { return {a.re + b.re, a.im + b.im}; }
+inline zdd operator-(zdd a, zdd b) { return {a.re - b.re, a.im - b.im}; }
+inline zdd operator*(zdd a, zdd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline zdd operator/(zdd a, zdd b) {
+  const dd den = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+}
+inline zdd zd(cd v) { return {dd{v.real(), 0.0}, dd{v.imag(), 0.0}}; }
+
+// Thiele continued-fraction Pade through (z_i, A_i): coefficients a_i = g_i(z_i) with
+// g_0(z_j) = A_j, g_i(z_j) = (g_{i-1}(z_{i-1}) - g_{i-1}(z_j)) / ((z_j - z_{i-1}) g_{i-1}(z_j))
+// (pade_t.hpp:176-192; J. Low Temp. Phys. 29, 179 (1977)).
+struct Thiele {
+  std::vector<zdd> z, a;
+  void fit(const std::vector<cd>& zf, const std::vector<cd>& Af) {
+    const int n = static_cast<int>(zf.size());
+    z.resize(n);
+    a.resize(n);
+    std::vector<zdd> g(Af.size());
+    for (int j = 0; j < n; ++j) {
+      z[j] = zd(zf[j]);
+      g[j] = zd(Af[j]);
+    }
+    a[0] = g[0];
+    for (int i = 1; i < n; ++i) {
+      // g holds g_{i-1}(z_j) for j >= i-1; update in place for j >= i
+      const zdd gd = g[i - 1];
+      for (int j = n - 1; j >= i; --j) g[j] = (gd - g[j]) / ((z[j] - z[i - 1]) * g[j]);
+      a[i] = g[i];
+    }
+  }
+  // A(w) = a_0 / (1 + a_1 (w - z_0) / (1 + a_2 (w - z_1) / (1 + ...)))   (pade_t.hpp:109-128)
+  cd eval(cd w) const {
+    const int n = static_cast<int>(a.size());
+    const zdd W = zd(w), one{dd{1.0, 0.0}, dd{0.0, 0.0}};
+    zdd X = a[n - 1] * (W - z[n - 2]);
+    for (int f = 1; f < n - 1; ++f) {
+      const int idx = n - f - 1;
+      X = (a[idx] * (W - z[idx - 1])) / (one + X);
+    }
+    X = a[0] / (one + X);
+    return cd(X.re.hi + X.re.lo, X.im.hi + X.im.lo);
+  }
+};
+
+// qp_eqn_bisection, qp_scf_common.cpp:679-728, on res(e) = Re[e + i eta - Vhf - Sigma(e - mu + i eta)].
+double bisection(double vhf, const Thiele& S, double mu, double eps0, double tol, double eta) {
+  auto res_f = [&](double e) { return (cd(e, eta) - vhf - S.eval(cd(e - mu, eta))).real(); };
+  const double delta = 0.01;
+  double res = res_f(eps0);
+  if (std::fabs(res) < tol) return eps0;
+  double e1, e2;
+  int guard = 0;
+  if (res >= 0) {
+    e2 = eps0;
+    e1 = eps0 - delta;
+    double r1 = res_f(e1);
+    while (r1 > 0 && guard++ < 100000) {
+      e1 -= delta;
+      r1 = res_f(e1);
+    }
+  } else {
+    e1 = eps0;
+    e2 = eps0 + delta;
+    double r2 = res_f(e2);
+    while (r2 < 0 && guard++ < 100000) {
+      e2 += delta;
+      r2 = res_f(e2);
+    }
+  }
+  double mid = 0.5 * (e1 + e2);
+  res = res_f(mid);
+  guard = 0;
+  while (std::fabs(res) > tol && guard++ < 200) {
+    if (res >= 0) e2 = mid;
+    else e1 = mid;
+    mid = 0.5 * (e1 + e2);
+    res = res_f(mid);
+  }
+  return mid;
+}
+
+}  // namespace
+
+void solve_qp_diag(const Tables& tab, int nk, int no, const cd* sig_t, const cd* vhf, const double* E0, double mu,
+                   int nfit, double eta, double tol, double* E_out) {
+  const int nt = tab.nt, nw = tab.nw_f;
+  const long dim = long(nk) * no;
+  const std::vector<int> idx = pade_fit_indices(nw, nfit);
+  std::vector<cd> zf(nfit);
+  for (int i = 0; i < nfit; ++i) zf[i] = cd(0.0, double(tab.wn_f[idx[i]]) * M_PI / tab.beta);  // IAFT.hpp:55-57
+#pragma omp parallel for schedule(dynamic)
+  for (long I = 0; I < dim; ++I) {
+    std::vector<cd> Af(nfit);
+    for (int i = 0; i < nfit; ++i) {  // Sigma(iw_n) = Twt_ff Sigma(tau) at the fit frequencies
+      cd acc = 0.0;
+      const int w = idx[i];
+      for (int t = 0; t < nt; ++t) acc += tab.Twt_ff[size_t(w) * nt + t] * sig_t[size_t(t) * dim + I];
+      Af[i] = acc;
+    }
+    Thiele p;
+    p.fit(zf, Af);
+    E_out[I] = bisection(vhf[I].real(), p, mu, E0[I], tol, eta);
+  }
+}
+
+}  // namespace host
+}  // namespace thcgw

@@ -1,0 +1,274 @@
+// Elementwise and small-reduction kernels of the THC GW/RPA path.
+#include "thcgw_internal.hpp"
+
+#include <algorithm>
+
+namespace thcgw {
+namespace {
+
+constexpr int T256 = 256;
+
+// G0 of qp_scf_common.cpp:86-93 for x = e - mu at imaginary time t.
+__device__ __forceinline__ double g0(double x, double t, double beta) {
+  if (x > 0) return -exp(-x * t) / (1.0 + exp(-x * beta));
+  return -exp(x * (beta - t)) / (1.0 + exp(x * beta));
+}
+
+// CG[t][k][i][a] = C[k][i][a] * g0(E[k][a] - mu, tau[t])
+__global__ __launch_bounds__(T256) void scale_mo_kernel(const c128* C, const double* E, const double* tau, double mu,
+                                                        double beta, int nt, int nk, int no, c128* CG) {
+  const long total = static_cast<long>(nt) * nk * no * no;
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= total) return;
+  const int a = static_cast<int>(e % no);
+  const long ki = e / no;  // (t, k, i)
+  const int k = static_cast<int>((ki / no) % nk);
+  const int t = static_cast<int>(ki / (static_cast<long>(no) * nk));
+  const c128 c = C[(static_cast<long>(k) * no + (ki % no)) * no + a];
+  const double g = g0(E[static_cast<long>(k) * no + a] - mu, tau[t], beta);
+  CG[e] = c128{c.x * g, c.y * g};
+}
+
+// out(o, blk, col) = sum_i T[o][i] * in(i, blk, col)   (imaginary-axis transforms).
+// One thread per column; OC outputs accumulated in registers per pass over the input.
+constexpr int OC = 24;
+__global__ __launch_bounds__(T256) void ir_transform_kernel(int n_out, int n_in, const c128* __restrict__ T,
+                                                            const c128* __restrict__ in, long in_si, long in_sb,
+                                                            c128* __restrict__ out, long out_so, long out_sb,
+                                                            long M) {
+  const long col = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  const long blk = blockIdx.y;
+  if (col >= M) return;
+  const c128* src = in + blk * in_sb + col;
+  c128* dst = out + blk * out_sb + col;
+  for (int o0 = 0; o0 < n_out; o0 += OC) {
+    double ar[OC], ai[OC];
+#pragma unroll
+    for (int j = 0; j < OC; ++j) ar[j] = ai[j] = 0.0;
+    for (int i = 0; i < n_in; ++i) {
+      const c128 x = src[i * in_si];
+#pragma unroll
+      for (int j = 0; j < OC; ++j) {
+        if (o0 + j < n_out) {
+          const c128 t = T[(o0 + j) * n_in + i];
+          ar[j] += t.x * x.x - t.y * x.y;
+          ai[j] += t.x * x.y + t.y * x.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < OC; ++j)
+      if (o0 + j < n_out) dst[(o0 + j) * out_so] = c128{ar[j], ai[j]};
+  }
+}
+
+// A <- (A + A^H)/2 for each n x n matrix (scf_common.hpp:70-86); thread per (i, j), i <= j.
+__global__ __launch_bounds__(T256) void hermitize_kernel(c128* A, int n, long nb) {
+  const long per = static_cast<long>(n) * n;
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= per * nb) return;
+  const long b = e / per;
+  const int i = static_cast<int>((e % per) / n), j = static_cast<int>(e % n);
+  if (i > j) return;
+  c128* Ab = A + b * per;
+  const c128 x = Ab[static_cast<long>(i) * n + j], y = Ab[static_cast<long>(j) * n + i];
+  const c128 h{0.5 * (x.x + y.x), 0.5 * (x.y - y.y)};
+  Ab[static_cast<long>(i) * n + j] = h;
+  Ab[static_cast<long>(j) * n + i] = c128{h.x, -h.y};
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < T256 / 64; ++w) s += sh[w];
+  return s;
+}
+
+// trace[b] = sum_c (1 - A[b][c][c]) = Tr(V Pi) when A = I - V Pi.
+__global__ __launch_bounds__(T256) void one_minus_trace_kernel(const c128* A, long sA, int n, c128* trace) {
+  __shared__ double sh[T256 / 64];
+  const int b = blockIdx.x;
+  const c128* Ab = A + static_cast<long>(b) * sA;
+  double re = 0.0, im = 0.0;
+  for (int c = threadIdx.x; c < n; c += T256) {
+    const c128 d = Ab[static_cast<long>(c) * n + c];
+    re += 1.0 - d.x;
+    im += -d.y;
+  }
+  re = block_sum(re, sh);
+  im = block_sum(im, sh);
+  if (threadIdx.x == 0) trace[b] = c128{re, im};
+}
+
+// log|Re det| of an LU factorisation with diagonal d_c and column swaps ipiv:
+// sum ln|d_c| + ln|cos(sum arg d_c + pi * #swaps)|  (finite for any n, no overflow).
+__global__ __launch_bounds__(T256) void log_re_det_kernel(const c128* A, long sA, int n, const int* ipiv,
+                                                          double* ldet) {
+  __shared__ double sh[T256 / 64];
+  const int b = blockIdx.x;
+  const c128* Ab = A + static_cast<long>(b) * sA;
+  const int* piv = ipiv + static_cast<long>(b) * n;
+  double la = 0.0, ph = 0.0, sw = 0.0;
+  for (int c = threadIdx.x; c < n; c += T256) {
+    const c128 d = Ab[static_cast<long>(c) * n + c];
+    la += log(hypot(d.x, d.y));
+    ph += atan2(d.y, d.x);
+    sw += (piv[c] != c) ? 1.0 : 0.0;
+  }
+  la = block_sum(la, sh);
+  ph = block_sum(ph, sh);
+  sw = block_sum(sw, sh);
+  if (threadIdx.x == 0) {
+    const double phase = ph + M_PI * fmod(sw, 2.0);
+    ldet[b] = la + log(fabs(cos(phase)));
+  }
+}
+
+// Row gather that undoes the column pivoting of A = Lt Ut Pt in a solve:
+// Y = S_0 S_1 ... S_{n-1} Z, i.e. Y[r] = Z[idx[r]] with idx built by applying the swaps in
+// reverse order to the identity; then W = Y - V (scr_coulomb_t.cpp:191-196 stores W - V).
+__global__ __launch_bounds__(T256) void dyson_finish_kernel(const c128* Z, long sZ, const c128* V, long sV, int vmod,
+                                                            const int* ipiv, int n, c128* W, long sW, int b0) {
+  extern __shared__ int idx[];
+  const int b = blockIdx.y;
+  const int* piv = ipiv + static_cast<long>(b) * n;
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < n; ++r) idx[r] = r;
+    for (int c = n - 1; c >= 0; --c) {
+      const int p = piv[c];
+      const int tmp = idx[c];
+      idx[c] = idx[p];
+      idx[p] = tmp;
+    }
+  }
+  __syncthreads();
+  const c128* Zb = Z + static_cast<long>(b) * sZ;
+  const c128* Vb = V + static_cast<long>((b0 + b) % vmod) * sV;
+  c128* Wb = W + static_cast<long>(b) * sW;
+  const long per = static_cast<long>(n) * n;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < per; e += static_cast<long>(gridDim.x) * T256) {
+    const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
+    const c128 z = Zb[static_cast<long>(idx[r]) * n + c];
+    const c128 v = Vb[e];
+    Wb[e] = c128{z.x - v.x, z.y - v.y};
+  }
+}
+
+__global__ __launch_bounds__(T256) void copy_batch_kernel(const c128* src, long sS, int smod, c128* dst, long sD,
+                                                          long per, int b0) {
+  const int b = blockIdx.y;
+  const c128* s = src + static_cast<long>((b0 + b) % smod) * sS;
+  c128* d = dst + static_cast<long>(b) * sD;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < per; e += static_cast<long>(gridDim.x) * T256)
+    d[e] = s[e];
+}
+
+// out[b][a] = sum_i conj(C[k][i][a]) T1[b][i][a], k = b % nk: the diagonal of C^H T1.
+__global__ __launch_bounds__(T256) void mo_diag_kernel(const c128* T1, const c128* C, c128* out, int nk, int no,
+                                                       long total) {
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= total) return;
+  const long b = e / no;
+  const int a = static_cast<int>(e % no);
+  const int k = static_cast<int>(b % nk);
+  const c128* Ck = C + static_cast<long>(k) * no * no;
+  const c128* Tb = T1 + b * no * no;
+  double re = 0.0, im = 0.0;
+  for (int i = 0; i < no; ++i) {
+    const c128 c = Ck[static_cast<long>(i) * no + a], t = Tb[static_cast<long>(i) * no + a];
+    re += c.x * t.x + c.y * t.y;
+    im += c.x * t.y - c.y * t.x;
+  }
+  out[e] = c128{re, im};
+}
+
+// out[w] = scale * sum_{k,i,j} A[w][k][i][j] B[w][k][j][i]   (one block per w).
+__global__ __launch_bounds__(T256) void trace_prod_kernel(const c128* A, const c128* B, int nk, int no, double scale,
+                                                          c128* out) {
+  __shared__ double sh[T256 / 64];
+  const long per = static_cast<long>(no) * no;
+  const c128* Aw = A + static_cast<long>(blockIdx.x) * nk * per;
+  const c128* Bw = B + static_cast<long>(blockIdx.x) * nk * per;
+  double re = 0.0, im = 0.0;
+  for (long e = threadIdx.x; e < nk * per; e += T256) {
+    const long k = e / per;
+    const int i = static_cast<int>((e % per) / no), j = static_cast<int>(e % no);
+    const c128 a = Aw[e], b = Bw[k * per + static_cast<long>(j) * no + i];
+    re += a.x * b.x - a.y * b.y;
+    im += a.x * b.y + a.y * b.x;
+  }
+  re = block_sum(re, sh);
+  im = block_sum(im, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = c128{re * scale, im * scale};
+}
+
+}  // namespace
+
+void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb) {
+  const long total = nb * no;
+  if (total == 0) return;
+  hipLaunchKernelGGL(mo_diag_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, T1, C,
+                     out, nk, no, total);
+  THCGW_HIP(hipGetLastError());
+}
+
+void trace_prod(hipStream_t s, const c128* A, const c128* B, int nw, int nk, int no, double scale, c128* out) {
+  hipLaunchKernelGGL(trace_prod_kernel, dim3(nw), dim3(T256), 0, s, A, B, nk, no, scale, out);
+  THCGW_HIP(hipGetLastError());
+}
+
+void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, double mu, double beta, int nt, int nk,
+              int no, c128* CG) {
+  const long total = static_cast<long>(nt) * nk * no * no;
+  hipLaunchKernelGGL(scale_mo_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, C, E,
+                     tau, mu, beta, nt, nk, no, CG);
+  THCGW_HIP(hipGetLastError());
+}
+
+void ir_transform(hipStream_t s, int n_out, int n_in, const c128* T, const c128* in, long in_si, long in_sb,
+                  c128* out, long out_so, long out_sb, long M, int nblk) {
+  if (M <= 0 || nblk <= 0) return;
+  hipLaunchKernelGGL(ir_transform_kernel, dim3(static_cast<unsigned>((M + T256 - 1) / T256), nblk), dim3(T256), 0, s,
+                     n_out, n_in, T, in, in_si, in_sb, out, out_so, out_sb, M);
+  THCGW_HIP(hipGetLastError());
+}
+
+void hermitize(hipStream_t s, c128* A, int n, long nb) {
+  const long total = static_cast<long>(n) * n * nb;
+  if (total == 0) return;
+  hipLaunchKernelGGL(hermitize_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, A, n,
+                     nb);
+  THCGW_HIP(hipGetLastError());
+}
+
+void one_minus_trace(hipStream_t s, const c128* A, long sA, int n, c128* trace, int batch) {
+  hipLaunchKernelGGL(one_minus_trace_kernel, dim3(batch), dim3(T256), 0, s, A, sA, n, trace);
+  THCGW_HIP(hipGetLastError());
+}
+
+void log_re_det(hipStream_t s, const c128* A, long sA, int n, const int* ipiv, double* ldet, int batch) {
+  hipLaunchKernelGGL(log_re_det_kernel, dim3(batch), dim3(T256), 0, s, A, sA, n, ipiv, ldet);
+  THCGW_HIP(hipGetLastError());
+}
+
+void dyson_finish(hipStream_t s, const c128* Z, long sZ, const c128* V, long sV, int vmod, const int* ipiv, int n,
+                  c128* W, long sW, int b0, int batch) {
+  const long per = static_cast<long>(n) * n;
+  const unsigned gx = static_cast<unsigned>(std::min<long>((per + T256 - 1) / T256, 64));
+  hipLaunchKernelGGL(dyson_finish_kernel, dim3(gx, batch), dim3(T256), sizeof(int) * n, s, Z, sZ, V, sV, vmod, ipiv, n,
+                     W, sW, b0);
+  THCGW_HIP(hipGetLastError());
+}
+
+void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, long sD, long per, int b0, int batch) {
+  const unsigned gx = static_cast<unsigned>(std::min<long>((per + T256 - 1) / T256, 64));
+  hipLaunchKernelGGL(copy_batch_kernel, dim3(gx, batch), dim3(T256), 0, s, src, sS, smod, dst, sD, per, b0);
+  THCGW_HIP(hipGetLastError());
+}
+
+}  // namespace thcgw

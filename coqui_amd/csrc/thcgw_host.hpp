@@ -1,0 +1,47 @@
+// Host-side parts of libthcgw: IR tables, Ec assembly, quasiparticle solver.
+#pragma once
+
+#include <complex>
+#include <vector>
+
+#include "../../include/thcgw.h"
+
+namespace thcgw {
+namespace host {
+
+using cd = std::complex<double>;
+
+// IR tables rescaled with beta as in src/numerics/imag_axes_ft/ir/ir_driver.hpp:68-73.
+struct Tables {
+  int nt = 0, nw_b = 0, nw_f = 0;
+  double beta = 0.0;
+  std::vector<double> x;    // tau mesh in [-1, 1]
+  std::vector<double> tau;  // (x + 1) beta / 2   (qp_scf_common.cpp:96-98)
+  std::vector<long> wn_f;   // fermionic Matsubara indices
+  std::vector<cd> Ttw_ff, Twt_ff, Ttw_bb, Twt_bb, T_beta;
+};
+
+// Copies and rescales the tables; checks that the tau mesh is symmetric about beta/2
+// (scr_coulomb_t.cpp:26-34, thc_gw.cpp:58-67).
+void load_tables(const thcgw_dims& d, const thcgw_ir_tables& t, Tables& out);
+
+// Particle-hole folded bosonic transforms: Tp (nw_half x nt_half), IAFT.icc:62-70, and
+// Tm (nt_half x nw_half), IAFT.icc:117-125.
+void fold_ph(const Tables& tab, const thcgw_dims& d, std::vector<cd>& Tp, std::vector<cd>& Tm);
+
+// Ec = Re[T_beta_t_ff . (Ttw_bb e_w)] / (2 nq)   (thc_rpa.icc:171-173)
+double ec_from_ew(const Tables& tab, const std::vector<cd>& ew, int nq);
+
+// Quasiparticle energies from the MO-diagonal self-energy Sigma_nn(tau) (nt x nk*no),
+// the MO-diagonal Vhf (nk*no) and starting energies E0 (nk*no).
+void solve_qp_diag(const Tables& tab, int nk, int no, const cd* sigma_diag_t, const cd* vhf_diag, const double* E0,
+                   double mu, int nfit, double eta, double tol, double* E_out);
+
+// GM energy from SigmaG(iw_n) = sum_k w_k Tr[Sigma(iw_n) G(iw_n)] (nw_f values), scf_common.cpp:119-128.
+double gm_from_sigmag(const Tables& tab, const std::vector<cd>& sg);
+
+// Pade frequency selection of pade_driver::init (pade_driver.hpp:38-60) for a symmetric mesh.
+std::vector<int> pade_fit_indices(int niw, int nfit);
+
+}  // namespace host
+}  // namespace thcgw

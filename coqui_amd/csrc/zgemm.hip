@@ -1,0 +1,352 @@
+// Batched complex<double> GEMM on the CDNA4 FP64 matrix cores.
+//
+//   C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I,   op in {N, T, C}
+//
+// All matrices are row-major with explicit leading dimensions.  One 256-thread
+// workgroup computes a 64 x 64 tile of C; each of its four waves owns a 32 x 32
+// sub-tile made of 2 x 2 v_mfma_f64_16x16x4_f64 blocks, with separate real and
+// imaginary accumulators (a complex multiply-add is four real MFMAs).  K is
+// staged through LDS in chunks of 8 complex values, double buffered, so that
+// the next chunk's global loads overlap the current chunk's MFMAs.
+//
+// Workgroups are dispatched round-robin over the 8 XCDs; the block index is
+// remapped so that each XCD receives a contiguous range of (batch, tile) work.
+// The tiles of one matrix then share their A row panels and B column panels
+// through that XCD's L2.
+#include "thcgw_internal.hpp"
+
+#include <atomic>
+#include <mutex>
+#include <vector>
+
+namespace thcgw {
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64;
+constexpr int BN = 64;
+constexpr int BK = 8;
+constexpr int NT = 256;
+constexpr int NXCD = 8;
+
+struct Args {
+  int M, N, K, batch, tiles_m, tiles_n;
+  double ar, ai, br, bi, gr, gi;
+  BatchPtr A, B;
+  long lda, ldb;
+  c128* C;
+  long strideC, ldc;
+};
+
+__device__ __forceinline__ const c128* batch_ptr(const BatchPtr& p, int b) {
+  return p.p + static_cast<long>(((b + p.off) / p.div) % p.mod) * p.stride;
+}
+
+__device__ __forceinline__ c128 ld_elem(const c128* X, long ld, int r, int c, int R, int Cn, bool conj) {
+  if (r < R && c < Cn) {
+    c128 v = X[static_cast<long>(r) * ld + c];
+    if (conj) v.y = -v.y;
+    return v;
+  }
+  return c128{0.0, 0.0};
+}
+
+// Load the BM x BK tile of op(A) (rows m0.., cols k0..) into two registers per thread.
+// OP_N: A is M x K, read rows (8 complex = 128 B per row).  OP_T/OP_C: A is K x M.
+template <int OPA>
+__device__ __forceinline__ void load_a(const c128* A, long lda, int M, int K, int m0, int k0, c128 (&r)[2]) {
+  const int t = threadIdx.x;
+  if (OPA == OP_N) {
+    const int m = t >> 2, kk = (t & 3) * 2;
+    r[0] = ld_elem(A, lda, m0 + m, k0 + kk, M, K, false);
+    r[1] = ld_elem(A, lda, m0 + m, k0 + kk + 1, M, K, false);
+  } else {
+    const int k = t >> 5, m = (t & 31) * 2;
+    r[0] = ld_elem(A, lda, k0 + k, m0 + m, K, M, OPA == OP_C);
+    r[1] = ld_elem(A, lda, k0 + k, m0 + m + 1, K, M, OPA == OP_C);
+  }
+}
+
+template <int OPA>
+__device__ __forceinline__ void store_a(c128 (*As)[BM], const c128 (&r)[2]) {
+  const int t = threadIdx.x;
+  if (OPA == OP_N) {
+    const int m = t >> 2, kk = (t & 3) * 2;
+    As[kk][m] = r[0];
+    As[kk + 1][m] = r[1];
+  } else {
+    const int k = t >> 5, m = (t & 31) * 2;
+    As[k][m] = r[0];
+    As[k][m + 1] = r[1];
+  }
+}
+
+// Load the BK x BN tile of op(B).  OP_N: B is K x N.  OP_T/OP_C: B is N x K.
+template <int OPB>
+__device__ __forceinline__ void load_b(const c128* B, long ldb, int N, int K, int n0, int k0, c128 (&r)[2]) {
+  const int t = threadIdx.x;
+  if (OPB == OP_N) {
+    const int k = t >> 5, n = (t & 31) * 2;
+    r[0] = ld_elem(B, ldb, k0 + k, n0 + n, K, N, false);
+    r[1] = ld_elem(B, ldb, k0 + k, n0 + n + 1, K, N, false);
+  } else {
+    const int n = t >> 2, kk = (t & 3) * 2;
+    r[0] = ld_elem(B, ldb, n0 + n, k0 + kk, N, K, OPB == OP_C);
+    r[1] = ld_elem(B, ldb, n0 + n, k0 + kk + 1, N, K, OPB == OP_C);
+  }
+}
+
+template <int OPB>
+__device__ __forceinline__ void store_b(c128 (*Bs)[BN], const c128 (&r)[2]) {
+  const int t = threadIdx.x;
+  if (OPB == OP_N) {
+    const int k = t >> 5, n = (t & 31) * 2;
+    Bs[k][n] = r[0];
+    Bs[k][n + 1] = r[1];
+  } else {
+    const int n = t >> 2, kk = (t & 3) * 2;
+    Bs[kk][n] = r[0];
+    Bs[kk + 1][n] = r[1];
+  }
+}
+
+template <int OPA, int OPB>
+__global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
+  __shared__ c128 As[2][BK][BM];
+  __shared__ c128 Bs[2][BK][BN];
+
+  // XCD-aware remap of the hardware block id to a logical (batch, tile) index.
+  const int nblk = gridDim.x;
+  const int hw = blockIdx.x;
+  const int xcd = hw % NXCD, loc = hw / NXCD;
+  const int base = nblk / NXCD, rem = nblk % NXCD;
+  const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
+
+  const int tiles = p.tiles_m * p.tiles_n;
+  const int b = logical / tiles;
+  const int tile = logical % tiles;
+  const int m0 = (tile / p.tiles_n) * BM;
+  const int n0 = (tile % p.tiles_n) * BN;
+
+  const c128* A = batch_ptr(p.A, b);
+  const c128* B = batch_ptr(p.B, b);
+  c128* C = p.C + static_cast<long>(b) * p.strideC;
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  f64x4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cr[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+      ci[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  const int nk = (p.K + BK - 1) / BK;
+  c128 ra[2], rb[2];
+  if (nk > 0) {
+    load_a<OPA>(A, p.lda, p.M, p.K, m0, 0, ra);
+    load_b<OPB>(B, p.ldb, p.N, p.K, n0, 0, rb);
+    store_a<OPA>(As[0], ra);
+    store_b<OPB>(Bs[0], rb);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) {
+      load_a<OPA>(A, p.lda, p.M, p.K, m0, (kt + 1) * BK, ra);
+      load_b<OPB>(B, p.ldb, p.N, p.K, n0, (kt + 1) * BK, rb);
+    }
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      double a_r[2], a_i[2], a_n[2], b_r[2], b_i[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        c128 v = As[cur][ks * 4 + lk][wm * 32 + i * 16 + lr];
+        a_r[i] = v.x;
+        a_i[i] = v.y;
+        a_n[i] = -v.y;
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        c128 v = Bs[cur][ks * 4 + lk][wn * 32 + j * 16 + lr];
+        b_r[j] = v.x;
+        b_i[j] = v.y;
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], cr[i][j], 0, 0, 0);
+          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[i], b_i[j], cr[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_i[j], ci[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], ci[i][j], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < nk) {
+      store_a<OPA>(As[cur ^ 1], ra);
+      store_b<OPB>(Bs[cur ^ 1], rb);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: lane holds rows 4*lk + v (v = 0..3) and column lr of each 16 x 16 block.
+  const bool beta0 = (p.br == 0.0 && p.bi == 0.0);
+  const bool gamma0 = (p.gr == 0.0 && p.gi == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = m0 + wm * 32 + i * 16 + lk * 4 + v;
+        const int c = n0 + wn * 32 + j * 16 + lr;
+        if (r < p.M && c < p.N) {
+          const double xr = cr[i][j][v], xi = ci[i][j][v];
+          double outr = p.ar * xr - p.ai * xi;
+          double outi = p.ar * xi + p.ai * xr;
+          c128* dst = C + static_cast<long>(r) * p.ldc + c;
+          if (!beta0) {
+            c128 old = *dst;
+            outr += p.br * old.x - p.bi * old.y;
+            outi += p.br * old.y + p.bi * old.x;
+          }
+          if (!gamma0 && r == c) {
+            outr += p.gr;
+            outi += p.gi;
+          }
+          *dst = c128{outr, outi};
+        }
+      }
+}
+
+std::atomic<long> g_launches{0};
+std::atomic<long long> g_flops_milli{0};
+
+// Optional per-launch timing with HIP events recorded on the launch stream.
+struct ProfRec {
+  hipEvent_t a, b;
+  double flops;
+  int cls;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_pool;
+
+hipEvent_t pool_event() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  THCGW_HIP(hipEventCreate(&e));
+  return e;
+}
+
+template <int OPA, int OPB>
+void launch(hipStream_t s, const Args& a, int nblocks) {
+  hipLaunchKernelGGL((zgemm_kernel<OPA, OPB>), dim3(nblocks), dim3(NT), 0, s, a);
+}
+
+}  // namespace
+
+void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
+           BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch, Cplx gamma) {
+  THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "zgemm: negative size");
+  THCGW_REQUIRE(opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2, "zgemm: bad op");
+  if (M == 0 || N == 0 || batch == 0) return;
+  Args a;
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.batch = batch;
+  a.tiles_m = (M + BM - 1) / BM;
+  a.tiles_n = (N + BN - 1) / BN;
+  a.ar = alpha.re;
+  a.ai = alpha.im;
+  a.br = beta.re;
+  a.bi = beta.im;
+  a.gr = gamma.re;
+  a.gi = gamma.im;
+  a.A = A;
+  a.B = B;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.C = C;
+  a.strideC = strideC;
+  a.ldc = ldc;
+  const long nblocks = static_cast<long>(batch) * a.tiles_m * a.tiles_n;
+  THCGW_REQUIRE(nblocks < (1L << 31), "zgemm: grid too large");
+  const int nb = static_cast<int>(nblocks);
+  const double flops = 8.0 * double(M) * N * K * batch;
+  std::unique_lock<std::mutex> lk(g_prof_mu, std::defer_lock);
+  ProfRec rec{nullptr, nullptr, flops, opA * 3 + opB};
+  if (g_prof_on) {
+    lk.lock();
+    rec.a = pool_event();
+    rec.b = pool_event();
+    THCGW_HIP(hipEventRecord(rec.a, s));
+  }
+  switch (opA * 3 + opB) {
+    case 0: launch<OP_N, OP_N>(s, a, nb); break;
+    case 1: launch<OP_N, OP_T>(s, a, nb); break;
+    case 2: launch<OP_N, OP_C>(s, a, nb); break;
+    case 3: launch<OP_T, OP_N>(s, a, nb); break;
+    case 4: launch<OP_T, OP_T>(s, a, nb); break;
+    case 5: launch<OP_T, OP_C>(s, a, nb); break;
+    case 6: launch<OP_C, OP_N>(s, a, nb); break;
+    case 7: launch<OP_C, OP_T>(s, a, nb); break;
+    default: launch<OP_C, OP_C>(s, a, nb); break;
+  }
+  THCGW_HIP(hipGetLastError());
+  if (g_prof_on) {
+    THCGW_HIP(hipEventRecord(rec.b, s));
+    g_prof.push_back(rec);
+  }
+  g_launches.fetch_add(1);
+  g_flops_milli.fetch_add(static_cast<long long>(flops / 1000.0));
+}
+
+GemmStats gemm_stats() { return GemmStats{g_launches.load(), 1000.0 * double(g_flops_milli.load())}; }
+void gemm_stats_reset() {
+  g_launches = 0;
+  g_flops_milli = 0;
+}
+
+void gemm_profile_enable(bool on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on;
+}
+
+// Sum of launch durations (ms), modelled flops and launch count of the recorded launches
+// of class cls (opA*3 + opB; -1 = all), then forget all records.
+void gemm_profile_read(int cls, double* ms, double* flops, long* launches) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  double t = 0.0, f = 0.0;
+  long n = 0;
+  for (const ProfRec& r : g_prof) {
+    THCGW_HIP(hipEventSynchronize(r.b));
+    if (cls < 0 || r.cls == cls) {
+      float e = 0.0f;
+      THCGW_HIP(hipEventElapsedTime(&e, r.a, r.b));
+      t += e;
+      f += r.flops;
+      ++n;
+    }
+    g_pool.push_back(r.a);
+    g_pool.push_back(r.b);
+  }
+  g_prof.clear();
+  *ms = t;
+  *flops = f;
+  *launches = n;
+}
+
+}  // namespace thcgw

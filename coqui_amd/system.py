@@ -1,0 +1,73 @@
+"""Seeded synthetic THC systems.
+
+One generator serves the GPU path, the CPU oracle and the benchmark, so both sides of a
+parity check see the same bytes.  A system is:
+
+  * a Gamma-centred kgrid[0] x kgrid[1] x kgrid[2] k-mesh (lattice vectors = identity),
+    ns = 1, no symmetry, div_treatment = ignore_g0;
+  * H(k) = U diag(e) U^H with U Haar-random (QR of a complex Gaussian matrix with the
+    phase of R's diagonal moved into Q) and e uniform in [-1, 1] Ha, sorted; mu = 0.
+    The MO coefficients are C = U and the energies E = e;
+  * X^k[mu, i] complex Gaussian with each row normalised over i;
+  * V_q = Z_q Z_q^H / N_mu with Z_q complex Gaussian N_mu x N_mu.  V_q is used as the
+    reference's thc_reader_t::Z(iq) (thc_reader_t.hpp:228-230), i.e. the N_k factor is
+    taken to be included in it.
+
+Seeds are 20261202 + 1000 * system_id.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BASE_SEED = 20261202
+
+
+@dataclass
+class System:
+    kgrid: tuple
+    no: int
+    nmu: int
+    seed: int
+    C: np.ndarray = field(repr=False)    # (nk, no, no) MO coefficients, column a = orbital
+    E: np.ndarray = field(repr=False)    # (nk, no) MO energies
+    X: np.ndarray = field(repr=False)    # (nk, nmu, no)
+    V: np.ndarray = field(repr=False)    # (nk, nmu, nmu), or None when built on the device
+    Z: np.ndarray = field(repr=False)    # (nk, nmu, nmu) factor of V
+    mu: float = 0.0
+
+    @property
+    def nk(self):
+        return int(np.prod(self.kgrid))
+
+    @property
+    def H(self):
+        """H(k) = C diag(E) C^H: the primary-basis Hamiltonian used as Vhf in the QP equation."""
+        return np.einsum("kia,ka,kja->kij", self.C, self.E, self.C.conj())
+
+
+def _cgauss(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
+
+
+def make_system(kgrid, no, nmu, system_id=0, with_v=True):
+    """Build a synthetic system.  with_v=False skips V (the caller forms Z Z^H / N_mu itself)."""
+    kgrid = tuple(int(v) for v in kgrid)
+    nk = int(np.prod(kgrid))
+    seed = BASE_SEED + 1000 * int(system_id)
+    rng = np.random.default_rng(seed)
+    A = _cgauss(rng, (nk, no, no))
+    Q, R = np.linalg.qr(A)
+    d = np.diagonal(R, axis1=1, axis2=2)
+    U = Q * (d / np.abs(d))[:, None, :]
+    e = rng.uniform(-1.0, 1.0, (nk, no))
+    order = np.argsort(e, axis=1)
+    E = np.take_along_axis(e, order, axis=1)
+    C = np.ascontiguousarray(np.take_along_axis(U, order[:, None, :], axis=2))
+    X = _cgauss(rng, (nk, nmu, no))
+    X /= np.linalg.norm(X, axis=2, keepdims=True)
+    Z = _cgauss(rng, (nk, nmu, nmu))
+    V = None
+    if with_v:
+        V = np.matmul(Z, np.conj(np.swapaxes(Z, 1, 2))) / nmu
+    return System(kgrid=kgrid, no=no, nmu=nmu, seed=seed, C=C, E=np.ascontiguousarray(E), X=np.ascontiguousarray(X),
+                  V=V, Z=Z)

@@ -1,0 +1,270 @@
+"""Python binding of libthcgw (ctypes over the C ABI in include/thcgw.h).
+
+Device arrays are torch complex128 tensors on the current HIP device; torch provides the
+memory and the stream, the library does the work.  The method names follow the reference
+routines they replace (scr_coulomb_t::update_w, gw_t::evaluate, gw_t::rpa_energy, ...).
+
+There is no CPU fallback: if libthcgw.so is missing or no GPU is present, construction
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libthcgw.so")
+
+#: every symbol declared in include/thcgw.h
+EXPORTS = (
+    "thcgw_create", "thcgw_destroy", "thcgw_last_error", "thcgw_update_G", "thcgw_primary_to_aux",
+    "thcgw_aux_to_primary", "thcgw_pi", "thcgw_tau_to_w", "thcgw_dyson_w", "thcgw_w_to_tau", "thcgw_sigma",
+    "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
+    "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read",
+)
+
+ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
+
+
+class ThcgwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "error"), code, msg))
+        self.code = code
+
+
+class _Dims(ctypes.Structure):
+    _fields_ = [("kgrid", ctypes.c_int * 3), ("no", ctypes.c_int), ("nmu", ctypes.c_int), ("ns", ctypes.c_int),
+                ("nt", ctypes.c_int), ("nw_b", ctypes.c_int), ("nw_f", ctypes.c_int), ("beta", ctypes.c_double)]
+
+
+class _Tables(ctypes.Structure):
+    _fields_ = [("tau_mesh_f", ctypes.c_void_p), ("wn_mesh_f", ctypes.c_void_p), ("Ttw_ff", ctypes.c_void_p),
+                ("Twt_ff", ctypes.c_void_p), ("Ttw_bb", ctypes.c_void_p), ("Twt_bb", ctypes.c_void_p),
+                ("T_beta_t_ff", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libthcgw.so and declare its signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("libthcgw.so is not built (%s); run __graft_entry__.build() or make -C coqui_amd/csrc" % path)
+    lib = ctypes.CDLL(path)
+    i, l, d, p = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+    sig = {
+        "thcgw_create": (i, [ctypes.POINTER(_Dims), ctypes.POINTER(_Tables), ctypes.POINTER(p)]),
+        "thcgw_destroy": (None, [p]),
+        "thcgw_last_error": (ctypes.c_char_p, [p]),
+        "thcgw_update_G": (i, [p, p, p, d, p, p]),
+        "thcgw_primary_to_aux": (i, [p, l, p, p, p, p]),
+        "thcgw_aux_to_primary": (i, [p, l, p, p, p, p]),
+        "thcgw_pi": (i, [p, p, p, p, p]),
+        "thcgw_tau_to_w": (i, [p, p, p, p]),
+        "thcgw_dyson_w": (i, [p, p, p, p, p, p]),
+        "thcgw_w_to_tau": (i, [p, p, p, p]),
+        "thcgw_sigma": (i, [p, p, p, p, p, i, p]),
+        "thcgw_hermitize": (i, [p, l, i, p, p]),
+        "thcgw_rpa_ec": (i, [p, p, p, p, p]),
+        "thcgw_gw_iteration": (i, [p, p, p, p, p, p, p, i, p]),
+        "thcgw_solve_qp": (i, [p, p, p, p, p, d, i, d, d, p, p]),
+        "thcgw_gm_energy": (i, [p, p, p, p, p]),
+        "thcgw_workspace_bytes": (l, [p]),
+        "thcgw_set_flag": (i, [p, i, l]),
+        "thcgw_profile_read": (i, [p, i, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class ThcGW:
+    """One system shape on the current GPU (thcgw_create / thcgw_destroy)."""
+
+    def __init__(self, kgrid, no, nmu, iaft):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("coqui_amd.ThcGW needs a HIP device; there is no CPU fallback")
+        self._torch = torch
+        self.lib = load_library()
+        self.kgrid = tuple(int(v) for v in kgrid)
+        self.nk = int(np.prod(self.kgrid))
+        self.no, self.nmu = int(no), int(nmu)
+        self.iaft = iaft
+        dims = _Dims((ctypes.c_int * 3)(*self.kgrid), self.no, self.nmu, 1, iaft.nt, iaft.nw_b, iaft.nw_f,
+                     iaft.beta)
+        # keep the host tables alive for the duration of the create call
+        self._keep = [iaft.tau_mesh, iaft.wn_mesh_f, iaft.Ttw_ff, iaft.Twt_ff, iaft.Ttw_bb, iaft.Twt_bb,
+                      iaft.T_beta_t_ff]
+        tabs = _Tables(*[_np_ptr(a) for a in self._keep])
+        h = ctypes.c_void_p()
+        torch.cuda.current_device()  # make sure the HIP runtime is initialised on this device
+        rc = self.lib.thcgw_create(ctypes.byref(dims), ctypes.byref(tabs), ctypes.byref(h))
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.thcgw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ helpers
+    def _check(self, rc):
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_last_error(self.h).decode())
+
+    def _stream(self, stream=None):
+        s = stream if stream is not None else self._torch.cuda.current_stream()
+        return ctypes.c_void_p(s.cuda_stream)
+
+    def _dev(self, a):
+        t = self._torch
+        if isinstance(a, t.Tensor):
+            assert a.is_cuda and a.dtype == t.complex128 and a.is_contiguous(), "expected a contiguous complex128 GPU tensor"
+            return a
+        return t.from_numpy(np.ascontiguousarray(a, np.complex128)).cuda()
+
+    def _empty(self, *shape):
+        return self._torch.empty(shape, dtype=self._torch.complex128, device="cuda")
+
+    @property
+    def nt(self):
+        return self.iaft.nt
+
+    @property
+    def nt_half(self):
+        return self.iaft.nt_half
+
+    @property
+    def nw_half(self):
+        return self.iaft.nw_half
+
+    def set_flag(self, flag, value):
+        self._check(self.lib.thcgw_set_flag(self.h, int(flag), int(value)))
+
+    def profile_read(self, kernel_class=-1):
+        """(ms, flops, launches) of the GEMM launches recorded since the last read (flag 4)."""
+        ms, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        self._check(self.lib.thcgw_profile_read(self.h, int(kernel_class), ctypes.byref(ms), ctypes.byref(fl),
+                                                ctypes.byref(n)))
+        return ms.value, fl.value, n.value
+
+    def workspace_bytes(self):
+        return int(self.lib.thcgw_workspace_bytes(self.h))
+
+    # ------------------------------------------------------------------ stages
+    def update_G(self, C, E, mu=0.0, out=None, stream=None):
+        C = self._dev(C)
+        E = self._torch.as_tensor(np.ascontiguousarray(E, np.float64)).cuda() if not isinstance(E, self._torch.Tensor) else E
+        G = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        self._check(self.lib.thcgw_update_G(self.h, _ptr(C), _ptr(E), ctypes.c_double(mu), _ptr(G), self._stream(stream)))
+        return G
+
+    def primary_to_aux(self, X, O, stream=None):
+        X, O = self._dev(X), self._dev(O)
+        nb = O.numel() // (self.nk * self.no * self.no)
+        out = self._empty(nb, self.nk, self.nmu, self.nmu)
+        self._check(self.lib.thcgw_primary_to_aux(self.h, nb, _ptr(X), _ptr(O), _ptr(out), self._stream(stream)))
+        return out
+
+    def aux_to_primary(self, X, O, stream=None):
+        X, O = self._dev(X), self._dev(O)
+        nb = O.numel() // (self.nk * self.nmu * self.nmu)
+        out = self._empty(nb, self.nk, self.no, self.no)
+        self._check(self.lib.thcgw_aux_to_primary(self.h, nb, _ptr(X), _ptr(O), _ptr(out), self._stream(stream)))
+        return out
+
+    def pi(self, G, X, out=None, stream=None):
+        G, X = self._dev(G), self._dev(X)
+        P = out if out is not None else self._empty(self.nt_half, self.nk, self.nmu, self.nmu)
+        self._check(self.lib.thcgw_pi(self.h, _ptr(G), _ptr(X), _ptr(P), self._stream(stream)))
+        return P
+
+    def tau_to_w(self, Pi_t, out=None, stream=None):
+        Pi_t = self._dev(Pi_t)
+        P = out if out is not None else self._empty(self.nw_half, self.nk, self.nmu, self.nmu)
+        self._check(self.lib.thcgw_tau_to_w(self.h, _ptr(Pi_t), _ptr(P), self._stream(stream)))
+        return P
+
+    def dyson_w(self, PiW, V, with_terms=False, stream=None):
+        """In place: PiW <- W - V.  Returns (trace, logdet) host arrays when with_terms."""
+        PiW, V = self._dev(PiW), self._dev(V)
+        n = self.nw_half * self.nk
+        tr = np.zeros(n, np.complex128) if with_terms else None
+        ld = np.zeros(n, np.float64) if with_terms else None
+        self._check(self.lib.thcgw_dyson_w(self.h, _ptr(PiW), _ptr(V), _np_ptr(tr) if with_terms else None,
+                                           _np_ptr(ld) if with_terms else None, self._stream(stream)))
+        return (tr, ld) if with_terms else PiW
+
+    def w_to_tau(self, W_w, out=None, stream=None):
+        W_w = self._dev(W_w)
+        W = out if out is not None else self._empty(self.nk, self.nt_half, self.nmu, self.nmu)
+        self._check(self.lib.thcgw_w_to_tau(self.h, _ptr(W_w), _ptr(W), self._stream(stream)))
+        return W
+
+    def sigma(self, G, W_qt, X, hermitize=True, out=None, stream=None):
+        G, W_qt, X = self._dev(G), self._dev(W_qt), self._dev(X)
+        S = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        self._check(self.lib.thcgw_sigma(self.h, _ptr(G), _ptr(W_qt), _ptr(X), _ptr(S), int(bool(hermitize)),
+                                         self._stream(stream)))
+        return S
+
+    def hermitize(self, A, stream=None):
+        A = self._dev(A)
+        n = A.shape[-1]
+        self._check(self.lib.thcgw_hermitize(self.h, A.numel() // (n * n), n, _ptr(A), self._stream(stream)))
+        return A
+
+    def rpa_ec(self, Pi_w, V, stream=None):
+        Pi_w, V = self._dev(Pi_w), self._dev(V)
+        ec = ctypes.c_double()
+        self._check(self.lib.thcgw_rpa_ec(self.h, _ptr(Pi_w), _ptr(V), ctypes.byref(ec), self._stream(stream)))
+        return ec.value
+
+    def gw_iteration(self, G, X, V, with_ec=True, hermitize=True, Sigma=None, W_qt=None, stream=None):
+        """update_w + evaluate: returns (Sigma_tskij, W_qtPQ, Ec or None)."""
+        G, X, V = self._dev(G), self._dev(X), self._dev(V)
+        S = Sigma if Sigma is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        W = W_qt if W_qt is not None else self._empty(self.nk, self.nt_half, self.nmu, self.nmu)
+        ec = ctypes.c_double()
+        self._check(self.lib.thcgw_gw_iteration(self.h, _ptr(G), _ptr(X), _ptr(V), _ptr(S), _ptr(W),
+                                                ctypes.byref(ec) if with_ec else None, int(bool(hermitize)),
+                                                self._stream(stream)))
+        return S, W, (ec.value if with_ec else None)
+
+    def solve_qp(self, Sigma, C, Vhf, E0, mu=0.0, nfit=18, eta=1e-4, tol=1e-8, stream=None):
+        Sigma, C, Vhf = self._dev(Sigma), self._dev(C), self._dev(Vhf)
+        E0 = np.ascontiguousarray(E0, np.float64)
+        E = np.zeros_like(E0)
+        self._check(self.lib.thcgw_solve_qp(self.h, _ptr(Sigma), _ptr(C), _ptr(Vhf), _np_ptr(E0), ctypes.c_double(mu),
+                                            int(nfit), ctypes.c_double(eta), ctypes.c_double(tol), _np_ptr(E),
+                                            self._stream(stream)))
+        return E
+
+    def gm_energy(self, G, Sigma, stream=None):
+        G, Sigma = self._dev(G), self._dev(Sigma)
+        e = ctypes.c_double()
+        self._check(self.lib.thcgw_gm_energy(self.h, _ptr(G), _ptr(Sigma), ctypes.byref(e), self._stream(stream)))
+        return e.value

@@ -1,0 +1,180 @@
+/*
+ * thcgw.h - C ABI of the MI355X THC imaginary-time GW/RPA library (libthcgw.so).
+ *
+ * The library computes the inner loop of CoQui's THC GW/RPA drivers on one GPU:
+ * the polarizability chi0(tau,q), the screened interaction W (Dyson), the
+ * self-energy Sigma(tau,k), the RPA correlation energy and the G0W0
+ * quasiparticle energies.  Each entry point replaces one host routine of the
+ * reference (paths relative to the CoQui repository root) and keeps its array
+ * layout and conventions.
+ *
+ * Conventions
+ *   - complex arrays are interleaved complex<double> (re, im), C order;
+ *   - device pointers unless stated otherwise; the caller owns every buffer;
+ *   - `stream` is a hipStream_t (0 = default stream); work is enqueued on it.
+ *     Entry points that return a host value (Ec, energies) synchronise the stream;
+ *     the others return once the work is enqueued unless stated otherwise;
+ *   - every function returns THCGW_OK (0) or an error code; the message is
+ *     available from thcgw_last_error(ctx).  No C++ exception crosses the ABI;
+ *   - ns = 1 (no spin polarisation), npol = 1, no k-point symmetry: k-points form
+ *     a Gamma-centred kgrid[0] x kgrid[1] x kgrid[2] mesh in C order with
+ *     lattice vectors = identity, and q-points = k-points.
+ *
+ * Layouts (names as in the reference):
+ *   G_tskij     (nt, 1, nk, no, no)      Green's function on the fermionic IR tau mesh
+ *   X_skPa      (1, nk, nmu, no)         THC collocation X^k[P,a]   (thc_reader_t::X)
+ *   V_qPQ       (nk, nmu, nmu)           THC Coulomb, incl. the N_k factor (thc_reader_t::Z)
+ *   Pi_tqPQ     (nt_half, nk, nmu, nmu)  polarizability on tau in [0, beta/2]
+ *   Pi_wqPQ     (nw_half, nk, nmu, nmu)  polarizability / W - V on bosonic iW_n >= 0
+ *   W_qtPQ      (nk, nt_half, nmu, nmu)  W - V on tau (MBState::dW_qtPQ)
+ *   Sigma_tskij (nt, 1, nk, no, no)      self-energy on the fermionic tau mesh
+ * with nt_half = ceil(nt/2), nw_half = ceil(nw_b/2).
+ */
+#ifndef THCGW_H
+#define THCGW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define THCGW_OK 0
+#define THCGW_ERR_ARG 1
+#define THCGW_ERR_HIP 2
+#define THCGW_ERR_SINGULAR 3
+#define THCGW_ERR_UNSUPPORTED 4
+#define THCGW_ERR_ALLOC 5
+
+typedef struct thcgw_ctx thcgw_ctx;
+
+typedef struct thcgw_dims {
+  int kgrid[3]; /* k-mesh; nk = kgrid[0]*kgrid[1]*kgrid[2] */
+  int no;       /* primary basis size (MF::nbnd) */
+  int nmu;      /* THC auxiliary basis size (thc_reader_t::Np) */
+  int ns;       /* spin channels; must be 1 */
+  int nt;       /* IR tau points, fermionic = bosonic (IAFT::nt_f, nt_b) */
+  int nw_b;     /* bosonic Matsubara points (IAFT::nw_b) */
+  int nw_f;     /* fermionic Matsubara points (IAFT::nw_f) */
+  double beta;  /* inverse temperature (Ha^-1) */
+} thcgw_dims;
+
+/* IR tables in the dimensionless form stored in the reference's ir/data/<lambda>.<eps>.h5 files.
+ * They are rescaled with beta as in src/numerics/imag_axes_ft/ir/ir_driver.hpp:68-73.
+ * Complex tables are interleaved (re, im) host arrays. */
+typedef struct thcgw_ir_tables {
+  const double* tau_mesh_f; /* nt, x in [-1, 1] (fermion/tau_mesh) */
+  const long* wn_mesh_f;    /* nw_f Matsubara indices (fermion/wn_mesh) */
+  const double* Ttw_ff;     /* nt x nw_f */
+  const double* Twt_ff;     /* nw_f x nt */
+  const double* Ttw_bb;     /* nt x nw_b */
+  const double* Twt_bb;     /* nw_b x nt */
+  const double* T_beta_t_ff; /* nt (fermion/T_betat) */
+} thcgw_ir_tables;
+
+/* Create a context for one system shape.  Replaces the construction of
+ * imag_axes_ft::IAFT (src/numerics/imag_axes_ft/IAFT.hpp:34-42) and the shape checks
+ * of scr_coulomb_t (src/methods/scr_coulomb/scr_coulomb_t.cpp:12-35): the tau mesh must
+ * be symmetric about beta/2 and nt_f == nt_b.  Selects the current HIP device. */
+int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ctx** out);
+void thcgw_destroy(thcgw_ctx* ctx);
+/* Message of the last failed call on ctx (or of the last failed thcgw_create if ctx is NULL). */
+const char* thcgw_last_error(const thcgw_ctx* ctx);
+
+/* a1: G_tskij from MO coefficients C (nk, no, no) [C_ia: column a = orbital] and
+ * energies E (nk, no, double) at chemical potential mu, by the Fermi-function formula.
+ * Replaces update_G, src/methods/SCF/qp_scf_common.cpp:82-118. */
+int thcgw_update_G(thcgw_ctx* ctx, const double* C, const double* E, double mu, double* G_tskij, void* stream);
+
+/* a2: O_PQ = X O_ab X^H for nb blocks of nk matrices: O_in (nb, nk, no, no) ->
+ * O_out (nb, nk, nmu, nmu).  Replaces thc_solver_comm::primary_to_aux,
+ * src/methods/HF/thc_solver_comm.hpp:98-132,363-429. */
+int thcgw_primary_to_aux(thcgw_ctx* ctx, long nb, const double* X_skPa, const double* O_in, double* O_out,
+                         void* stream);
+
+/* a10: O_ab = X^H O_PQ X: O_in (nb, nk, nmu, nmu) -> O_out (nb, nk, no, no).
+ * Replaces thc_solver_comm::aux_to_primary, thc_solver_comm.hpp:240-304,516-569. */
+int thcgw_aux_to_primary(thcgw_ctx* ctx, long nb, const double* X_skPa, const double* O_in, double* O_out,
+                         void* stream);
+
+/* a2-a4: RPA polarizability Pi_tqPQ from G_tskij and X.  Replaces
+ * scr_coulomb_t::eval_Pi_rpa_Rspace, src/methods/scr_coulomb/rpa_pi.icc:22-180. */
+int thcgw_pi(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa, double* Pi_tqPQ, void* stream);
+
+/* a5: Pi_tqPQ -> Pi_wqPQ with the particle-hole folded bosonic transform.  Replaces
+ * scr_coulomb_t::tau_to_w (scr_coulomb_t.cpp:349-376) -> IAFT::tau_to_w_PHsym (IAFT.icc:49-73). */
+int thcgw_tau_to_w(thcgw_ctx* ctx, const double* Pi_tqPQ, double* Pi_wqPQ, void* stream);
+
+/* a6: in place Pi_wqPQ -> W_wqPQ = [(I - V Pi)^{-1} - I] V for every (iW_n, q).
+ * Replaces scr_coulomb_t::dyson_W_in_place, scr_coulomb_t.cpp:121-206.
+ * If trace_out / logdet_out are non-NULL they receive, per (n, q) in Pi_wqPQ order,
+ * Tr(V Pi) (complex, host array of 2*nw_half*nk doubles) and ln|Re det(I - V Pi)|
+ * (host array of nw_half*nk doubles), the two terms of the RPA energy
+ * (thc_rpa.icc:141-163).  Synchronises the stream; THCGW_ERR_SINGULAR if a factor is singular. */
+int thcgw_dyson_w(thcgw_ctx* ctx, double* PiW_wqPQ, const double* V_qPQ, double* trace_out, double* logdet_out,
+                  void* stream);
+
+/* a7 + a8: W_wqPQ -> W_qtPQ (q-major, the layout of MBState::dW_qtPQ).  Replaces
+ * scr_coulomb_t::w_to_tau (scr_coulomb_t.cpp:421-448) -> IAFT::w_to_tau_PHsym
+ * (IAFT.icc:104-128) and the transpose in update_w (scr_coulomb_t.cpp:73-86). */
+int thcgw_w_to_tau(thcgw_ctx* ctx, const double* W_wqPQ, double* W_qtPQ, void* stream);
+
+/* a9-a12: Sigma_tskij from G_tskij, W_qtPQ and X, R-space algorithm with both
+ * particle-hole passes; hermitize != 0 applies (S + S^H)/2 afterwards.  Replaces
+ * gw_t::eval_Sigma_all (src/methods/GW/thc_gw.cpp:152-178 -> thc_gw.icc:70-213),
+ * gw_t::Sigma_div_correction for div_treatment = ignore_g0 (thc_gw.icc:460-461) and
+ * hermitize (src/methods/SCF/scf_common.hpp:70-86). */
+int thcgw_sigma(thcgw_ctx* ctx, const double* G_tskij, const double* W_qtPQ, const double* X_skPa,
+                double* Sigma_tskij, int hermitize, void* stream);
+
+/* a12: (A + A^H)/2 for nb square n x n matrices, in place. */
+int thcgw_hermitize(thcgw_ctx* ctx, long nb, int n, double* A, void* stream);
+
+/* a13: RPA correlation energy from Pi_wqPQ and V.  Replaces gw_t::thc_rpa_energy_all_impl,
+ * src/methods/GW/thc_rpa.icc:72-181 (A = Pi V, Tr A + ln Re det(I - A), w_to_tau, tau_to_beta).
+ * Synchronises the stream. */
+int thcgw_rpa_ec(thcgw_ctx* ctx, const double* Pi_wqPQ, const double* V_qPQ, double* Ec, void* stream);
+
+/* One GW iteration (the timed unit): chi0 -> tau_to_w -> Dyson -> w_to_tau -> Sigma, plus
+ * the RPA energy from the same LU factors when Ec != NULL.  Mirrors
+ * scr_coulomb_t::update_w + gw_t::evaluate as called by scf_loop
+ * (src/methods/SCF/scf_driver.cpp:119-122) and gw_t::rpa_energy (thc_rpa.cpp:22-61).
+ * W_qtPQ (caller-owned, nk*nt_half*nmu^2 complex) receives W - V; it also serves as
+ * scratch for Pi_tqPQ.  Synchronises the stream. */
+int thcgw_gw_iteration(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa, const double* V_qPQ,
+                       double* Sigma_tskij, double* W_qtPQ, double* Ec, int hermitize, void* stream);
+
+/* a15: G0W0 quasiparticle energies.  Sigma_tskij (nt,1,nk,no,no), MO coefficients C
+ * (nk,no,no) and Vhf (nk,no,no, primary basis) are device arrays; E0 (nk,no) starting
+ * energies and E_out (nk,no) are host arrays.  Sigma and Vhf are projected on the MO
+ * diagonal, Sigma is taken to iw_n, continued by an nfit-point Thiele Pade and the QP
+ * equation is solved by bisection with tolerance tol at broadening eta.  Replaces
+ * solve_qp_eqn (src/methods/SCF/qp_scf_common.cpp:150-311) with pade_driver / pade_t
+ * (src/numerics/ac/pade/pade_driver.hpp:24-65, pade_t.hpp:34-168) and qp_eqn_bisection
+ * (qp_scf_common.cpp:679-728).  Synchronises the stream. */
+int thcgw_solve_qp(thcgw_ctx* ctx, const double* Sigma_tskij, const double* C, const double* Vhf,
+                   const double* E0, double mu, int nfit, double eta, double tol, double* E_out, void* stream);
+
+/* a16: Galitskii-Migdal correlation energy from device G_tskij and Sigma_tskij.  Replaces
+ * eval_corr_energy, src/methods/SCF/scf_common.cpp:87-136 (k weights 1/nk).
+ * Synchronises the stream. */
+int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_tskij, double* e_corr,
+                    void* stream);
+
+/* Diagnostics (no counterpart in the reference): bytes of device workspace held by ctx,
+ * control flags and GEMM launch timing.
+ *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;
+ *   flag 2: tau pairs per projection batch (0 = automatic);
+ *   flag 3: matrices per Dyson batch (0 = automatic);
+ *   flag 4: record HIP events around every complex-FP64 GEMM launch (process-wide).
+ * thcgw_profile_read returns, for the GEMM launches recorded since the last read whose
+ * class (opA*3 + opB, op 0 = N, 1 = T, 2 = C) equals kernel_class (-1 = all), the summed
+ * kernel time in ms, the modelled flops (8 M N K per complex product) and the launch count,
+ * then drops the records.  It waits for the recorded launches to finish. */
+long thcgw_workspace_bytes(const thcgw_ctx* ctx);
+int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value);
+int thcgw_profile_read(thcgw_ctx* ctx, int kernel_class, double* ms, double* flops, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* THCGW_H */

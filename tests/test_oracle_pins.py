@@ -1,0 +1,310 @@
+"""Pins of the FP64 CPU oracle (no GPU).
+
+Each test checks an oracle function against data or mathematics that does not come from
+the oracle itself: the reference's IAFT golden file, closed forms, the reference's second
+algorithm for the same quantity, or numpy/LAPACK on small shapes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from coqui_amd.system import make_system
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300)
+
+
+def small_system(kgrid=(2, 2, 1), no=6, nmu=10, sid=7):
+    return make_system(kgrid, no, nmu, system_id=sid)
+
+
+# ----------------------------------------------------------------- IR transforms
+@pytest.mark.parametrize("prec,table,tol", [("high", "1e4.1e-15", 1e-11), ("medium", "1e4.1e-10", 1e-9)])
+def test_iaft_golden_fixture(orc, prec, table, tol):
+    """test_iaft.cpp:46-119: tau<->iw round trip and beta^- against the sparse_ir fixture."""
+    z = np.load(os.path.join(GOLDEN, "iaft_si_kp222_%s.npz" % prec))
+    tab = orc.load_ir(1000.0, None, table=table)
+    Gt, Gw, Dm = z["Gt"], z["Gw"], z["Dm"]
+    Gw_c = orc.transform(tab["Twt_ff"], Gt)
+    Gt_c = orc.transform(tab["Ttw_ff"], Gw)
+    # ARRAY_EQUAL(a, b, tol) of test_common.hpp: elementwise WithinRel(tol) || WithinAbs(tol)
+    assert np.all((np.abs(Gw_c - Gw) <= tol) | (np.abs(Gw_c - Gw) <= tol * np.abs(Gw)))
+    assert np.all((np.abs(Gt_c - Gt) <= tol) | (np.abs(Gt_c - Gt) <= tol * np.abs(Gt)))
+    Dm_c = -orc.tau_to_beta(tab, Gt)
+    assert np.all(np.abs(Dm_c - Dm) <= tol)
+
+
+@pytest.mark.parametrize("table", ["1e2.1e-15", "1e3.1e-15", "1e4.1e-15", "1e5.1e-06"])
+def test_iaft_identity(orc, table):
+    """test_iaft.cpp:40-43: Ttw_ff Twt_ff = I (1e-10); tables rescaled as ir_driver.hpp:68-73."""
+    tab = orc.load_ir(1000.0, None, table=table)
+    eye = tab["Ttw_ff"] @ tab["Twt_ff"]
+    assert np.max(np.abs(eye - np.eye(tab["nt"]))) < 1e-10
+    eyeb = tab["Ttw_bb"] @ tab["Twt_bb"]
+    assert np.max(np.abs(eyeb - np.eye(tab["nt"]))) < 1e-9
+
+
+def single_pole_t(tab, e, beta):
+    """g(tau) of qp_scf_common.cpp:86-93 (mu = 0) on the fermionic mesh, written directly."""
+    t = tab["tau"]
+    if e > 0:
+        return -np.exp(-e * t) / (1 + np.exp(-e * beta))
+    return -np.exp(e * (beta - t)) / (1 + np.exp(e * beta))
+
+
+def test_iaft_single_pole(orc):
+    """Closed form: a single pole g(tau) maps to 1/(iw_n - e) on the IR Matsubara mesh."""
+    beta = 100.0
+    tab = orc.load_ir(beta, None, table="1e3.1e-15")  # lambda = 1000 -> wmax = 10
+    for e in (-0.7, -0.05, 0.3, 2.0):
+        gt = single_pole_t(tab, e, beta)
+        gw = orc.transform(tab["Twt_ff"], gt[:, None])[:, 0]
+        iw = 1j * tab["wn_f"] * np.pi / beta
+        assert np.max(np.abs(gw - 1.0 / (iw - e))) < 1e-10
+        # tau -> beta^-: g(beta^-) = -f(e) (occupation), IAFT.icc:219-238
+        gb = orc.tau_to_beta(tab, gt[:, None])[0]
+        assert abs(gb + 1.0 / (1.0 + np.exp(beta * e))) < 1e-11
+
+
+def test_ph_folding_matches_full_transform(orc):
+    """IAFT.icc:49-73/104-128: for a bosonic function symmetric under tau -> beta - tau the
+    folded half-mesh transforms equal the full-grid ones."""
+    beta = 1000.0
+    tab = orc.load_ir(beta, None, table="1e2.1e-15")
+    nt, nth = tab["nt"], (tab["nt"] + 1) // 2
+    t = tab["tau"]
+    f = np.stack([np.exp(-a * t) + np.exp(-a * (beta - t)) for a in (0.01, 0.05, 0.2)], axis=1).astype(complex)
+    f[:, 1] *= 1 + 0.5j
+    full_w = orc.transform(tab["Twt_bb"], f)
+    half_w = orc.tau_to_w_phsym(tab, f[:nth])
+    nwb = tab["nw_b"]
+    assert rel(half_w, full_w[nwb // 2:]) < 1e-12
+    # and back: w_to_tau_PHsym of the half frequency data reproduces the tau data
+    back = orc.w_to_tau_phsym(tab, full_w[nwb // 2:])
+    assert rel(back, f[:nth]) < 1e-10
+
+
+# ----------------------------------------------------------------- k <-> R
+@pytest.mark.parametrize("kgrid", [(2, 2, 2), (3, 3, 3), (4, 4, 4), (2, 3, 1)])
+def test_kR_coefficients(orc, kgrid):
+    fRk, fkR = orc.kR_coefficients(kgrid)
+    nk = int(np.prod(kgrid))
+    assert np.max(np.abs(fkR @ fRk - np.eye(nk))) < 1e-13
+    x = np.random.default_rng(1).standard_normal((nk, 3)) + 0j
+    via_fft = np.fft.fftn(x.reshape(tuple(kgrid) + (3,)), axes=(0, 1, 2)).reshape(nk, 3) / nk
+    assert np.max(np.abs(fRk @ x - via_fft)) < 1e-13
+
+
+# ----------------------------------------------------------------- G and projections
+def test_update_G_closed_form(orc):
+    """qp_scf_common.cpp:82-118: G = C g C^H; G(0+) + G(beta-) -> -I, tau_to_beta(G) = -C f C^H."""
+    s = small_system()
+    beta = 1000.0
+    tab = orc.load_ir(beta, None, table="1e3.1e-15")
+    G = orc.update_G(tab, s.C, s.E)
+    nk, no = s.E.shape
+    for k in range(nk):
+        for it in (0, 5, tab["nt"] - 1):
+            g = np.array([single_pole_t({"tau": tab["tau"][it:it + 1]}, e, beta)[0] for e in s.E[k]])
+            ref = (s.C[k] * g) @ s.C[k].conj().T
+            assert np.max(np.abs(G[it, 0, k] - ref)) < 1e-13
+    assert np.max(np.abs(G - np.conj(np.swapaxes(G, -1, -2)))) < 1e-14
+    Gb = orc.tau_to_beta(tab, G)
+    occ = 0.5 * (1.0 - np.tanh(0.5 * beta * s.E))
+    Dm = np.einsum("kia,ka,kja->kij", s.C, occ, s.C.conj())
+    assert np.max(np.abs(-Gb[0] - Dm)) < 1e-10
+
+
+def test_projections_brute_force(orc):
+    s = small_system()
+    rng = np.random.default_rng(3)
+    nk, no, nmu = s.nk, s.no, s.nmu
+    O = rng.standard_normal((2, nk, no, no)) + 1j * rng.standard_normal((2, nk, no, no))
+    P = orc.primary_to_aux(s.X, O)
+    ref = np.einsum("kpa,nkab,kqb->nkpq", s.X, O, s.X.conj())
+    assert rel(P, ref) < 1e-14
+    Q = rng.standard_normal((2, nk, nmu, nmu)) + 1j * rng.standard_normal((2, nk, nmu, nmu))
+    A = orc.aux_to_primary(s.X, Q)
+    ref2 = np.einsum("kpa,nkpq,kqb->nkab", s.X.conj(), Q, s.X)
+    assert rel(A, ref2) < 1e-14
+    # X = [I; 0]: X O X^H is O padded with zeros
+    Xi = np.zeros((nk, nmu, no), complex)
+    Xi[:, :no, :no] = np.eye(no)
+    Pi = orc.primary_to_aux(Xi, O)
+    assert np.max(np.abs(Pi[:, :, :no, :no] - O)) == 0.0
+    assert np.max(np.abs(Pi[:, :, no:, :])) == 0.0
+
+
+# ----------------------------------------------------------------- chi0
+def qminus(kgrid, q):
+    n = np.array(kgrid)
+    idx = np.array(np.unravel_index(q, kgrid))
+    return int(np.ravel_multi_index(tuple((-idx) % n), kgrid))
+
+
+@pytest.mark.parametrize("kgrid", [(2, 2, 2), (3, 1, 2)])
+def test_pi_rspace_vs_kspace(orc, kgrid):
+    """rpa_pi.icc: the R-space and k-space algorithms satisfy Pi^k_PQ(q) = Pi^R_QP(-q)."""
+    s = make_system(kgrid, 5, 7, system_id=11)
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    G = orc.update_G(tab, s.C, s.E)
+    PR = orc.pi_rspace(G, s.X, kgrid)
+    PK = orc.pi_kspace(G, s.X, kgrid)
+    nk = s.nk
+    PRm = PR[:, [qminus(kgrid, q) for q in range(nk)]]
+    assert rel(PK, np.swapaxes(PRm, -1, -2)) < 1e-12
+    # Hermitian in PQ for Hermitian G
+    assert rel(PR, np.conj(np.swapaxes(PR, -1, -2))) < 1e-12
+
+
+def test_pi_brute_force_convolution(orc):
+    """Pi(tau_i, q) = (sp/N_k) sum_k G_PQ(tau_i, k+q) conj(G_PQ(beta - tau_i, k)), sp = -2."""
+    kgrid = (2, 2, 1)
+    s = make_system(kgrid, 4, 5, system_id=12)
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    G = orc.update_G(tab, s.C, s.E)
+    PR = orc.pi_rspace(G, s.X, kgrid)
+    nt, nk = tab["nt"], s.nk
+    GP = np.einsum("kpa,tkab,kqb->tkpq", s.X, G[:, 0], s.X.conj())
+    ref = np.zeros_like(PR)
+    for it in range((nt + 1) // 2):
+        for q in range(nk):
+            for k in range(nk):
+                kq = np.ravel_multi_index(tuple((np.array(np.unravel_index(k, kgrid)) + np.array(np.unravel_index(q, kgrid))) % np.array(kgrid)), kgrid)
+                ref[it, q] += -2.0 / nk * GP[it, kq] * np.conj(GP[nt - 1 - it, k])
+    assert rel(PR, ref) < 1e-13
+
+
+# ----------------------------------------------------------------- Dyson
+def test_dyson_closed_form_and_lapack(orc):
+    rng = np.random.default_rng(5)
+    # N_mu = 1: W = V/(1 - V Pi) - V
+    Pi = rng.standard_normal((3, 2, 1, 1)) + 1j * rng.standard_normal((3, 2, 1, 1))
+    V = np.abs(rng.standard_normal((2, 1, 1))) + 0j
+    W = orc.dyson(Pi, V)
+    ref = V[None] / (1 - V[None] * Pi) - V[None]
+    assert np.max(np.abs(W - ref)) < 1e-14
+    # small shapes vs numpy.linalg.inv (LAPACK getrf/getri), scr_coulomb_t.cpp:180-196
+    s = make_system((2, 1, 1), 4, 12, system_id=13)
+    Pi = -(rng.standard_normal((2, 2, 12, 12)) + 1j * rng.standard_normal((2, 2, 12, 12))) * 0.1
+    W = orc.dyson(Pi, s.V)
+    for n in range(2):
+        for q in range(2):
+            A = np.linalg.inv(np.eye(12) - s.V[q] @ Pi[n, q]) - np.eye(12)
+            assert rel(W[n, q], A @ s.V[q]) < 1e-13
+
+
+# ----------------------------------------------------------------- Sigma
+@pytest.mark.parametrize("nt_table", ["1e2.1e-15", "1e5.1e-06"])
+def test_sigma_rspace_vs_kspace(orc, nt_table):
+    """thc_gw.icc: R-space (:70-213) and k-space (:342-443) self-energies agree; odd nt covered."""
+    kgrid = (2, 2, 1)
+    s = make_system(kgrid, 5, 6, system_id=14)
+    tab = orc.load_ir(1000.0, None, table=nt_table)
+    G = orc.update_G(tab, s.C, s.E)
+    rng = np.random.default_rng(9)
+    nth = (tab["nt"] + 1) // 2
+    Wqt = rng.standard_normal((s.nk, nth, 6, 6)) + 1j * rng.standard_normal((s.nk, nth, 6, 6))
+    SR = orc.sigma_rspace(G, s.X, Wqt, kgrid)
+    SK = orc.sigma_kspace(G, s.X, Wqt, kgrid)
+    assert rel(SR, SK) < 1e-12
+    assert np.all(np.isfinite(SR)) and np.max(np.abs(SR)) > 0
+
+
+# ----------------------------------------------------------------- Ec
+def test_rpa_ec_closed_form_nmu1(orc):
+    """N_mu = 1: Ec = T_beta . Ttw_bb e_w / (2 N_k), e_w(+-n) = sum_q [Pi V + ln(1 - Pi V)]."""
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    rng = np.random.default_rng(6)
+    nq, nwh = 3, (tab["nw_b"] + 1) // 2
+    Pi = -np.abs(rng.standard_normal((nwh, nq, 1, 1))) + 0j
+    V = np.abs(rng.standard_normal((nq, 1, 1))) + 0j
+    ec, ew = orc.rpa_ec(tab, Pi, V, return_ew=True)
+    x = (Pi[:, :, 0, 0] * V[None, :, 0, 0]).real
+    e_half = np.sum(x + np.log(1 - x), axis=1)
+    nwb = tab["nw_b"]
+    ew_ref = np.zeros(nwb, complex)
+    for n in range(nwh):
+        ew_ref[nwb // 2 + n] += e_half[n]
+        if n:
+            ew_ref[nwb // 2 - n] += e_half[n]
+    assert np.max(np.abs(ew - ew_ref)) < 1e-13
+    ec_ref = (tab["T_beta"] @ (tab["Ttw_bb"] @ ew_ref)).real / (2 * nq)
+    assert abs(ec - ec_ref) < 1e-13 * max(1, abs(ec_ref))
+
+
+def test_rpa_ec_logdet_vs_numpy(orc):
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    s = make_system((2, 1, 1), 4, 9, system_id=15)
+    rng = np.random.default_rng(8)
+    nwh = (tab["nw_b"] + 1) // 2
+    B = rng.standard_normal((nwh, 2, 9, 9)) + 1j * rng.standard_normal((nwh, 2, 9, 9))
+    Pi = -np.einsum("nqij,nqkj->nqik", B, B.conj()) * 0.05  # Hermitian negative semidefinite
+    ec, ew = orc.rpa_ec(tab, Pi, s.V, return_ew=True)
+    nwb = tab["nw_b"]
+    ref = np.zeros(nwb, complex)
+    for n in range(nwh):
+        for q in range(2):
+            A = Pi[n, q] @ s.V[q]
+            sign, ld = np.linalg.slogdet(np.eye(9) - A)
+            v = np.trace(A) + ld + np.log(abs(np.cos(np.angle(sign))))
+            ref[nwb // 2 + n] += v
+            if n:
+                ref[nwb // 2 - n] += v
+    assert np.max(np.abs(ew - ref)) < 1e-12
+
+
+# ----------------------------------------------------------------- Pade / QP / GM
+def test_pade_reproduces_rational(orc):
+    """Thiele's continued fraction through N points reproduces a rational function of
+    degree <= (N-1)/2 exactly (pade_t.hpp:176-192)."""
+    beta = 1000.0
+    z = 1j * (2 * np.arange(18) + 1) * np.pi / beta * 5
+    poles, res = np.array([-0.3, 0.1, 0.45]), np.array([0.2, 0.5, 0.3])
+    f = lambda w: np.sum(res[None, :] / (w[:, None] - poles[None, :]), axis=1)
+    w = np.array([0.05 + 1e-4j, -0.2 + 1e-4j, 0.7 + 0.01j, 1j])
+    out = orc.pade_eval(z, f(z), w)
+    assert np.max(np.abs(out - f(w))) < 1e-9
+
+
+def test_qp_single_pole_root(orc):
+    """QP equation with a one-pole self-energy Sigma(w) = a/(w - p): the root of
+    e - e0 - Re Sigma(e + i eta) = 0 is known in closed form (qp_scf_common.cpp:679-728)."""
+    beta = 1000.0
+    tab = orc.load_ir(beta, None, table="1e3.1e-15")
+    a, p = 0.01, -0.8
+    # Sigma(tau) of a single pole at p with weight a (fermionic): a * g_p(tau)
+    sig_t = a * single_pole_t(tab, p, beta)
+    nk, no = 1, 1
+    Sigma = np.zeros((tab["nt"], 1, nk, no, no), complex)
+    Sigma[:, 0, 0, 0, 0] = sig_t
+    C = np.ones((1, 1, 1), complex)
+    e0 = 0.2
+    Vhf = np.full((1, 1, 1), e0, complex)
+    E = orc.solve_qp(tab, Sigma, C, Vhf, np.array([[e0]]), mu=0.0, nfit=18, eta=1e-4, tol=1e-10)
+    # e - e0 - a/(e - p) = 0  ->  e^2 - (e0 + p) e + e0 p - a = 0 ; root near e0
+    disc = np.sqrt((e0 + p) ** 2 - 4 * (e0 * p - a))
+    root = ((e0 + p) + disc) / 2
+    assert abs(E[0, 0] - root) < 1e-6
+
+
+def test_gm_energy_diagonal(orc):
+    """eval_corr_energy with G = g(e) and Sigma = s * g(e') on one orbital:
+    E = -sum_n Sigma(iw_n) G(iw_n)/beta through the IR tau->beta^- interpolation."""
+    beta = 100.0
+    tab = orc.load_ir(beta, None, table="1e3.1e-15")
+    e1, e2, s = -0.3, 0.4, 0.05
+    G = np.zeros((tab["nt"], 1, 1, 1, 1), complex)
+    S = np.zeros_like(G)
+    G[:, 0, 0, 0, 0] = single_pole_t(tab, e1, beta)
+    S[:, 0, 0, 0, 0] = s * single_pole_t(tab, e2, beta)
+    e = orc.gm_energy(tab, G, S)
+    # (1/beta) sum_n s/((iw-e1)(iw-e2)) e^{iw 0+} = s (f(e1) - f(e2)) / (e1 - e2);
+    # tau_to_beta(x) = -(1/beta) sum_n x(iw_n), and E = -0.5 * 2 * tau_to_beta(Sigma G)
+    f = lambda x: 1.0 / (1.0 + np.exp(beta * x))
+    ref = s * (f(e1) - f(e2)) / (e1 - e2)
+    assert abs(e - ref) < 1e-9
