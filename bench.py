@@ -10,12 +10,15 @@ measured with HIP events on its launch stream during the timed steps, and the CP
 timed on a bounded sample of the same workload and extrapolated to one iteration.
 """
 import argparse
+import faulthandler
 import json
 import os
 import sys
 import time
 
 import numpy as np
+
+faulthandler.enable()
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -33,6 +36,39 @@ CONFIGS = {
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_child(path):
+    """Runs in a separate process that never touches the GPU (python bench.py --cpu-child F)."""
+    from coqui_amd import IAFT
+
+    z = np.load(path)
+    kgrid = tuple(int(v) for v in z["kgrid"])
+
+    class _S:
+        pass
+
+    s = _S()
+    s.kgrid, s.nk, s.nmu, s.X, s.V_host2 = kgrid, int(np.prod(kgrid)), int(z["nmu"]), z["X"], z["V2"]
+    ia = IAFT.from_table(float(z["beta"]), str(z["table"]))
+    print(json.dumps(cpu_baseline(s, ia, z["G_pair"], "FP64, all host threads")), flush=True)
+
+
+def run_cpu_baseline(s, ia, G_pair):
+    """The oracle sample in a child process: a host-side crash cannot take the GPU run down."""
+    import subprocess
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cpu_inputs.npz")
+        np.savez(path, kgrid=np.array(s.kgrid), nmu=s.nmu, X=s.X, V2=s.V_host2, G_pair=G_pair, beta=ia.beta,
+                 table=ia.name)
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", path], capture_output=True,
+                           text=True, timeout=900)
+        if r.returncode != 0:
+            log("bench: CPU baseline child failed (rc %d): %s" % (r.returncode, r.stderr[-2000:]))
+            return None
+        return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def cpu_baseline(s, ia, G_host_pair, budget_note):
@@ -60,7 +96,7 @@ def cpu_baseline(s, ia, G_host_pair, budget_note):
     t0 = time.perf_counter()
     oracle.dyson(Pi, V2)
     t["dyson_2"] = time.perf_counter() - t0
-    tab2 = dict(tab)
+    tab2 = dict(tab, nw_b=1, Ttw_bb=np.ascontiguousarray(tab["Ttw_bb"][:, :1]))  # one iW_n: two (q, iW) matrices
     t0 = time.perf_counter()
     oracle.rpa_ec(tab2, Pi, V2)
     t["ec_2"] = time.perf_counter() - t0
@@ -92,7 +128,11 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_baseline_child(args.cpu_child)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU sharding); use --gpus 1", file=sys.stderr)
@@ -127,7 +167,7 @@ def main():
     for i in range(args.warmup):
         _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
         log("bench: warmup %d done, Ec = %.12f" % (i, ec))
-    ctx.profile_read(-1)
+    ctx.profile_read()
     ctx.set_flag(4, 1)  # GEMM launch events during the timed steps
     torch.cuda.synchronize()
     t_start = time.perf_counter()
@@ -142,10 +182,14 @@ def main():
     wall = time.perf_counter() - t_start
     ms_step = ev0.elapsed_time(ev1) / args.steps
     ctx.set_flag(4, 0)
-    g_ms, g_fl, g_n = ctx.profile_read(0)   # N,N class: Dyson, LU and triangular-solve GEMMs
-    a_ms, a_fl, a_n = ctx.profile_read(-1)  # remaining classes (projections)
-    a_ms, a_fl, a_n = a_ms + g_ms, a_fl + g_fl, a_n + g_n
+    prof = ctx.profile_read()
+    g_ms, g_fl, g_n = prof.get("NN", (0.0, 0.0, 0))  # Dyson, LU and triangular-solve GEMMs
+    a_ms = sum(v[0] for v in prof.values())
+    a_fl = sum(v[1] for v in prof.values())
     achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    gemm_classes = {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
+                        "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
+                    for k, v in prof.items()}
 
     stages = None
     if args.breakdown:
@@ -173,8 +217,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU oracle sample ...")
-        cpu = cpu_baseline(s, ia, G_pair, "FP64, all host threads")
-        log("bench: CPU baseline %.1f s per iteration (extrapolated)" % cpu["value"])
+        cpu = run_cpu_baseline(s, ia, G_pair)
+        if cpu:
+            log("bench: CPU baseline %.1f s per iteration (extrapolated)" % cpu["value"])
 
     out = {
         "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
@@ -200,7 +245,8 @@ def main():
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
                      "launches": g_n, "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
                      "share_of_step_time": round(g_ms / max(ms_step * args.steps, 1e-9), 4),
-                     "all_gemm_tflops": round(a_fl / (a_ms * 1e-3) / 1e12, 3) if a_ms > 0 else 0.0},
+                     "all_gemm_tflops": round(a_fl / (a_ms * 1e-3) / 1e12, 3) if a_ms > 0 else 0.0,
+                     "gemm_classes": gemm_classes},
         "cpu_baseline": cpu,
         "stages_ms": stages,
         "wall_s_timed": round(wall, 3),

@@ -294,7 +294,8 @@ int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ct
     g_create_error = "thcgw_create: out of host memory";
     return THCGW_ERR_ALLOC;
   }
-  int rc = guarded(nullptr, [&] {
+  int rc = THCGW_OK;
+  try {
     const thcgw_dims& d = *dims;
     THCGW_REQUIRE(d.kgrid[0] >= 1 && d.kgrid[1] >= 1 && d.kgrid[2] >= 1, "kgrid entries must be >= 1");
     THCGW_REQUIRE(d.ns == 1, "only ns = 1 is supported (no spin polarisation)");
@@ -320,7 +321,13 @@ int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ct
     THCGW_HIP(hipMalloc(&ctx->d_Twt_ff, sizeof(c128) * ctx->tab.Twt_ff.size()));
     THCGW_HIP(hipMemcpy(ctx->d_Twt_ff, ctx->tab.Twt_ff.data(), sizeof(c128) * ctx->tab.Twt_ff.size(),
                         hipMemcpyHostToDevice));
-  });
+  } catch (const thcgw::Error& e) {
+    ctx->err = e.what();
+    rc = e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    rc = THCGW_ERR_ARG;
+  }
   if (rc != THCGW_OK) {
     g_create_error = ctx->err.empty() ? "thcgw_create failed" : ctx->err;
     ctx->release();
@@ -361,11 +368,11 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
   });
 }
 
-int thcgw_profile_read(thcgw_ctx* ctx, int kernel_class, double* ms, double* flops, long* launches) {
+int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, long* launches) {
   if (!ctx) return THCGW_ERR_ARG;
   return guarded(ctx, [&] {
     THCGW_REQUIRE(ms && flops && launches, "thcgw_profile_read: null pointer");
-    thcgw::gemm_profile_read(kernel_class, ms, flops, launches);
+    thcgw::gemm_profile_read(ms, flops, launches);
   });
 }
 

@@ -65,7 +65,7 @@ struct GemmStats {
 GemmStats gemm_stats();
 void gemm_stats_reset();
 void gemm_profile_enable(bool on);
-void gemm_profile_read(int cls, double* ms, double* flops, long* launches);
+void gemm_profile_read(double* ms, double* flops, long* launches);
 
 // kfft.hip
 bool kfft_supported(const int* kgrid);

@@ -195,7 +195,8 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
     __syncthreads();
   }
 
-  // Epilogue: lane holds rows 4*lk + v (v = 0..3) and column lr of each 16 x 16 block.
+  // Epilogue: for v_mfma_f64_16x16x4_f64 accumulator register v of lane l holds row
+  // l/16 + 4v and column l%16 of its 16 x 16 block (checked on gfx950).
   const bool beta0 = (p.br == 0.0 && p.bi == 0.0);
   const bool gamma0 = (p.gr == 0.0 && p.gi == 0.0);
 #pragma unroll
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = m0 + wm * 32 + i * 16 + lk * 4 + v;
+        const int r = m0 + wm * 32 + i * 16 + lk + 4 * v;
         const int c = n0 + wn * 32 + j * 16 + lr;
         if (r < p.M && c < p.N) {
           const double xr = cr[i][j][v], xi = ci[i][j][v];
@@ -325,28 +326,26 @@ void gemm_profile_enable(bool on) {
   g_prof_on = on;
 }
 
-// Sum of launch durations (ms), modelled flops and launch count of the recorded launches
-// of class cls (opA*3 + opB; -1 = all), then forget all records.
-void gemm_profile_read(int cls, double* ms, double* flops, long* launches) {
+// Per class (opA*3 + opB, 9 classes): summed launch durations (ms), modelled flops and
+// launch counts of the recorded launches; the records are then dropped.
+void gemm_profile_read(double* ms, double* flops, long* launches) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  double t = 0.0, f = 0.0;
-  long n = 0;
+  for (int c = 0; c < 9; ++c) {
+    ms[c] = 0.0;
+    flops[c] = 0.0;
+    launches[c] = 0;
+  }
   for (const ProfRec& r : g_prof) {
     THCGW_HIP(hipEventSynchronize(r.b));
-    if (cls < 0 || r.cls == cls) {
-      float e = 0.0f;
-      THCGW_HIP(hipEventElapsedTime(&e, r.a, r.b));
-      t += e;
-      f += r.flops;
-      ++n;
-    }
+    float e = 0.0f;
+    THCGW_HIP(hipEventElapsedTime(&e, r.a, r.b));
+    ms[r.cls] += e;
+    flops[r.cls] += r.flops;
+    launches[r.cls] += 1;
     g_pool.push_back(r.a);
     g_pool.push_back(r.b);
   }
   g_prof.clear();
-  *ms = t;
-  *flops = f;
-  *launches = n;
 }
 
 }  // namespace thcgw

@@ -74,7 +74,7 @@ def load_library(path=LIB_PATH):
         "thcgw_gm_energy": (i, [p, p, p, p, p]),
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
-        "thcgw_profile_read": (i, [p, i, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
+        "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -164,12 +164,15 @@ class ThcGW:
     def set_flag(self, flag, value):
         self._check(self.lib.thcgw_set_flag(self.h, int(flag), int(value)))
 
-    def profile_read(self, kernel_class=-1):
-        """(ms, flops, launches) of the GEMM launches recorded since the last read (flag 4)."""
-        ms, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
-        self._check(self.lib.thcgw_profile_read(self.h, int(kernel_class), ctypes.byref(ms), ctypes.byref(fl),
-                                                ctypes.byref(n)))
-        return ms.value, fl.value, n.value
+    def profile_read(self):
+        """{class: (ms, flops, launches)} of the GEMM launches recorded since the last read
+        (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B)."""
+        ms = (ctypes.c_double * 9)()
+        fl = (ctypes.c_double * 9)()
+        n = (ctypes.c_long * 9)()
+        self._check(self.lib.thcgw_profile_read(self.h, ms, fl, n))
+        names = [a + b for a in "NTC" for b in "NTC"]
+        return {names[c]: (ms[c], fl[c], n[c]) for c in range(9) if n[c]}
 
     def workspace_bytes(self):
         return int(self.lib.thcgw_workspace_bytes(self.h))

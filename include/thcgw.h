@@ -165,13 +165,13 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_t
  *   flag 2: tau pairs per projection batch (0 = automatic);
  *   flag 3: matrices per Dyson batch (0 = automatic);
  *   flag 4: record HIP events around every complex-FP64 GEMM launch (process-wide).
- * thcgw_profile_read returns, for the GEMM launches recorded since the last read whose
- * class (opA*3 + opB, op 0 = N, 1 = T, 2 = C) equals kernel_class (-1 = all), the summed
- * kernel time in ms, the modelled flops (8 M N K per complex product) and the launch count,
- * then drops the records.  It waits for the recorded launches to finish. */
+ * thcgw_profile_read fills, for each of the 9 GEMM classes c = opA*3 + opB (op 0 = N,
+ * 1 = T, 2 = C), the summed kernel time ms[c] of the launches recorded since the last read,
+ * their modelled flops[c] (8 M N K per complex product) and launches[c], then drops the
+ * records.  It waits for the recorded launches to finish. */
 long thcgw_workspace_bytes(const thcgw_ctx* ctx);
 int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value);
-int thcgw_profile_read(thcgw_ctx* ctx, int kernel_class, double* ms, double* flops, long* launches);
+int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, long* launches);
 
 #ifdef __cplusplus
 }

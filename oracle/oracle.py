@@ -292,6 +292,8 @@ def hermitize(A):
 def rpa_ec(tab, Piw, V, return_ew=False):
     Piw, V = _c(Piw), _c(V)
     nq, nmu = V.shape[0], V.shape[1]
+    if Piw.shape != ((tab["nw_b"] + 1) // 2, nq, nmu, nmu) or tab["Ttw_bb"].shape != (tab["nt"], tab["nw_b"]):
+        raise ValueError("rpa_ec: Pi_w %s does not match nw_half of the table and V %s" % (Piw.shape, V.shape))
     ew = np.zeros(tab["nw_b"], np.complex128)
     Ttw = _c(tab["Ttw_bb"])
     Tb = _c(tab["T_beta"])
