@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
     args = ap.parse_args()
     if args.cpu_child:
         cpu_baseline_child(args.cpu_child)
@@ -191,6 +193,15 @@ def main():
                         "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
                     for k, v in prof.items()}
 
+    if args.copy_calib:
+        xa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        ya = torch.empty_like(xa)
+        torch.cuda.synchronize()
+        ya.copy_(xa)
+        torch.cuda.synchronize()
+        log("bench: copied 1 GiB (calibration)")
+        del xa, ya
+
     stages = None
     if args.breakdown:
         stages = {}
@@ -221,6 +232,12 @@ def main():
         if cpu:
             log("bench: CPU baseline %.1f s per iteration (extrapolated)" % cpu["value"])
 
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "zgemm_nn_%s_traffic.json" % args.config)
+    if os.path.exists(tpath):  # HBM bytes per dispatch from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+        traffic = json.load(open(tpath))["traffic_bytes_per_dispatch"]
+        traffic_src = os.path.relpath(tpath, ROOT)
+
     out = {
         "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
         "value": round(ms_step / 1000.0, 6),
@@ -242,7 +259,9 @@ def main():
         "qp_error_meV": 0.0,
         "roofline": {"kernel": "zgemm_kernel<0,0> (complex FP64 GEMM, N,N)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "flops_per_launch": round(g_fl / max(g_n, 1)),
                      "launches": g_n, "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
                      "share_of_step_time": round(g_ms / max(ms_step * args.steps, 1e-9), 4),
                      "all_gemm_tflops": round(a_fl / (a_ms * 1e-3) / 1e12, 3) if a_ms > 0 else 0.0,
