@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--gemm-tile", type=int, default=0, help="GEMM tile shape: 0 auto, 1 64x64, 2 128x64, 3 128x128")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
     args = ap.parse_args()
@@ -152,6 +153,7 @@ def main():
     s = make_system(kgrid, no, nmu, system_id=0, with_v=False)
     ia = IAFT.from_table(beta, table)
     ctx = ThcGW(kgrid, no, nmu, ia)
+    ctx.set_flag(5, args.gemm_tile)
     Zd = torch.from_numpy(s.Z).cuda()
     Vd = torch.matmul(Zd, Zd.conj().transpose(1, 2)) / nmu  # input preparation, V = Z Z^H / N_mu
     del Zd

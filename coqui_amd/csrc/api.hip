@@ -160,15 +160,22 @@ void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t 
   }
 }
 
+// Pi(iW_n, q, PQ) = sum_i T+[n][i] Pi(tau_i, q, PQ): one GEMM (nw_half x nt_half) x (nt_half x N_q N_mu^2).
 void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
-  const long M = ctx->M, nk = ctx->nk;
-  thcgw::ir_transform(s, ctx->nw_half, ctx->nt_half, ctx->d_Tplus, Pit, nk * M, M, Piw, nk * M, M, M, ctx->nk);
+  const long cols = ctx->M * ctx->nk;
+  THCGW_REQUIRE(cols < (1L << 31), "tau_to_w: N_q N_mu^2 too large for one GEMM");
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, ctx->nw_half, static_cast<int>(cols), ctx->nt_half, kOne,
+               thcgw::bp(ctx->d_Tplus, 0), ctx->nt_half, thcgw::bp(Pit, 0), cols, kZero, Piw, 0, cols, 1);
 }
 
+// W(q, tau_i, PQ) = sum_n T-[i][n] W(iW_n, q, PQ): per q a GEMM (nt_half x nw_half) x (nw_half x N_mu^2)
+// whose output rows land at stride N_mu^2 inside the q block of W_qtPQ.
 void run_w_to_tau(thcgw_ctx* ctx, const c128* Ww, c128* Wqt, hipStream_t s) {
   const long M = ctx->M, nk = ctx->nk;
-  thcgw::ir_transform(s, ctx->nt_half, ctx->nw_half, ctx->d_Tminus, Ww, nk * M, M, Wqt, M, ctx->nt_half * M, M,
-                      ctx->nk);
+  THCGW_REQUIRE(M < (1L << 31), "w_to_tau: N_mu^2 too large for one GEMM");
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, ctx->nt_half, static_cast<int>(M), ctx->nw_half, kOne,
+               thcgw::bp(ctx->d_Tminus, 0), ctx->nw_half, thcgw::bp(Ww, M), nk * M, kZero, Wqt, ctx->nt_half * M, M,
+               ctx->nk);
 }
 
 int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
@@ -364,6 +371,7 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     else if (flag == 2) ctx->pair_chunk = static_cast<int>(value);
     else if (flag == 3) ctx->dyson_chunk = static_cast<int>(value);
     else if (flag == 4) thcgw::gemm_profile_enable(value != 0);
+    else if (flag == 5) thcgw::gemm_set_tile_mode(static_cast<int>(value));
     else THCGW_REQUIRE(false, "unknown flag");
   });
 }
@@ -549,8 +557,10 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G, const double* Sigma, double
     c128* Sw = ctx->buf<c128>("gm_sw", static_cast<size_t>(nw) * dim);
     c128* sg = ctx->buf<c128>("gm_sg", static_cast<size_t>(nw));
     // tau -> iw_n on the fermionic mesh (IAFT.icc:9-26), then sum_k w_k Tr[Sigma G] per frequency
-    thcgw::ir_transform(s, nw, nt, ctx->d_Twt_ff, C(G), dim, 0, Gw, dim, 0, dim, 1);
-    thcgw::ir_transform(s, nw, nt, ctx->d_Twt_ff, C(Sigma), dim, 0, Sw, dim, 0, dim, 1);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nw, static_cast<int>(dim), nt, kOne, thcgw::bp(ctx->d_Twt_ff, 0), nt,
+                 thcgw::bp(C(G), 0), dim, kZero, Gw, 0, dim, 1);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nw, static_cast<int>(dim), nt, kOne, thcgw::bp(ctx->d_Twt_ff, 0), nt,
+                 thcgw::bp(C(Sigma), 0), dim, kZero, Sw, 0, dim, 1);
     thcgw::trace_prod(s, Sw, Gw, nw, nk, no, 1.0 / nk, sg);
     std::vector<std::complex<double>> h(nw);
     THCGW_HIP(hipMemcpyAsync(h.data(), sg, sizeof(c128) * nw, hipMemcpyDeviceToHost, s));

@@ -29,39 +29,6 @@ __global__ __launch_bounds__(T256) void scale_mo_kernel(const c128* C, const dou
   CG[e] = c128{c.x * g, c.y * g};
 }
 
-// out(o, blk, col) = sum_i T[o][i] * in(i, blk, col)   (imaginary-axis transforms).
-// One thread per column; OC outputs accumulated in registers per pass over the input.
-constexpr int OC = 24;
-__global__ __launch_bounds__(T256) void ir_transform_kernel(int n_out, int n_in, const c128* __restrict__ T,
-                                                            const c128* __restrict__ in, long in_si, long in_sb,
-                                                            c128* __restrict__ out, long out_so, long out_sb,
-                                                            long M) {
-  const long col = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
-  const long blk = blockIdx.y;
-  if (col >= M) return;
-  const c128* src = in + blk * in_sb + col;
-  c128* dst = out + blk * out_sb + col;
-  for (int o0 = 0; o0 < n_out; o0 += OC) {
-    double ar[OC], ai[OC];
-#pragma unroll
-    for (int j = 0; j < OC; ++j) ar[j] = ai[j] = 0.0;
-    for (int i = 0; i < n_in; ++i) {
-      const c128 x = src[i * in_si];
-#pragma unroll
-      for (int j = 0; j < OC; ++j) {
-        if (o0 + j < n_out) {
-          const c128 t = T[(o0 + j) * n_in + i];
-          ar[j] += t.x * x.x - t.y * x.y;
-          ai[j] += t.x * x.y + t.y * x.x;
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < OC; ++j)
-      if (o0 + j < n_out) dst[(o0 + j) * out_so] = c128{ar[j], ai[j]};
-  }
-}
-
 // A <- (A + A^H)/2 for each n x n matrix (scf_common.hpp:70-86); thread per (i, j), i <= j.
 __global__ __launch_bounds__(T256) void hermitize_kernel(c128* A, int n, long nb) {
   const long per = static_cast<long>(n) * n;
@@ -227,14 +194,6 @@ void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, 
   const long total = static_cast<long>(nt) * nk * no * no;
   hipLaunchKernelGGL(scale_mo_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, C, E,
                      tau, mu, beta, nt, nk, no, CG);
-  THCGW_HIP(hipGetLastError());
-}
-
-void ir_transform(hipStream_t s, int n_out, int n_in, const c128* T, const c128* in, long in_si, long in_sb,
-                  c128* out, long out_so, long out_sb, long M, int nblk) {
-  if (M <= 0 || nblk <= 0) return;
-  hipLaunchKernelGGL(ir_transform_kernel, dim3(static_cast<unsigned>((M + T256 - 1) / T256), nblk), dim3(T256), 0, s,
-                     n_out, n_in, T, in, in_si, in_sb, out, out_so, out_sb, M);
   THCGW_HIP(hipGetLastError());
 }
 

@@ -65,6 +65,7 @@ struct GemmStats {
 GemmStats gemm_stats();
 void gemm_stats_reset();
 void gemm_profile_enable(bool on);
+void gemm_set_tile_mode(int mode);  // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
 void gemm_profile_read(double* ms, double* flops, long* launches);
 
 // kfft.hip
@@ -86,8 +87,6 @@ void trsm_left_upper_unit(hipStream_t s, const c128* U, long sU, int ldu, c128* 
 // misc.hip
 void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, double mu, double beta, int nt, int nk,
               int no, c128* CG);
-void ir_transform(hipStream_t s, int n_out, int n_in, const c128* T, const c128* in, long in_si, long in_sb,
-                  c128* out, long out_so, long out_sb, long M, int nblk);
 void hermitize(hipStream_t s, c128* A, int n, long nb);
 void one_minus_trace(hipStream_t s, const c128* A, long sA, int n, c128* trace, int batch);
 void log_re_det(hipStream_t s, const c128* A, long sA, int n, const int* ipiv, double* ldet, int batch);

@@ -3,16 +3,17 @@
 //   C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I,   op in {N, T, C}
 //
 // All matrices are row-major with explicit leading dimensions.  One 256-thread
-// workgroup computes a 64 x 64 tile of C; each of its four waves owns a 32 x 32
-// sub-tile made of 2 x 2 v_mfma_f64_16x16x4_f64 blocks, with separate real and
-// imaginary accumulators (a complex multiply-add is four real MFMAs).  K is
-// staged through LDS in chunks of 8 complex values, double buffered, so that
-// the next chunk's global loads overlap the current chunk's MFMAs.
+// workgroup (four waves, 2 x 2) computes a BM x BN tile of C; each wave owns a
+// (BM/2) x (BN/2) sub-tile made of v_mfma_f64_16x16x4_f64 blocks with separate real
+// and imaginary accumulators (a complex multiply-add is four real MFMAs).  Two tile
+// shapes are built: 128 x 128 (4 x 4 MFMA blocks per wave, accumulators in 256 AGPRs,
+// one wave per SIMD) for large products and 64 x 64 (2 x 2 blocks) for small ones.
+// K is staged through LDS in chunks of 8 complex values, double buffered, so the next
+// chunk's global loads overlap the current chunk's MFMAs.
 //
-// Workgroups are dispatched round-robin over the 8 XCDs; the block index is
-// remapped so that each XCD receives a contiguous range of (batch, tile) work.
-// The tiles of one matrix then share their A row panels and B column panels
-// through that XCD's L2.
+// Workgroups are dispatched round-robin over the 8 XCDs; the block index is remapped
+// so that each XCD receives a contiguous range of (batch, tile) work.  The tiles of one
+// matrix then share their A row panels and B column panels through that XCD's L2.
 #include "thcgw_internal.hpp"
 
 #include <atomic>
@@ -24,8 +25,6 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64;
-constexpr int BN = 64;
 constexpr int BK = 8;
 constexpr int NT = 256;
 constexpr int NXCD = 8;
@@ -52,67 +51,59 @@ __device__ __forceinline__ c128 ld_elem(const c128* X, long ld, int r, int c, in
   return c128{0.0, 0.0};
 }
 
-// Load the BM x BK tile of op(A) (rows m0.., cols k0..) into two registers per thread.
-// OP_N: A is M x K, read rows (8 complex = 128 B per row).  OP_T/OP_C: A is K x M.
-template <int OPA>
-__device__ __forceinline__ void load_a(const c128* A, long lda, int M, int K, int m0, int k0, c128 (&r)[2]) {
-  const int t = threadIdx.x;
-  if (OPA == OP_N) {
-    const int m = t >> 2, kk = (t & 3) * 2;
-    r[0] = ld_elem(A, lda, m0 + m, k0 + kk, M, K, false);
-    r[1] = ld_elem(A, lda, m0 + m, k0 + kk + 1, M, K, false);
-  } else {
-    const int k = t >> 5, m = (t & 31) * 2;
-    r[0] = ld_elem(A, lda, k0 + k, m0 + m, K, M, OPA == OP_C);
-    r[1] = ld_elem(A, lda, k0 + k, m0 + m + 1, K, M, OPA == OP_C);
+// Tile of an operand: BX elements along its free dimension x (m for op(A), n for op(B))
+// times BK along k, staged in LDS as S[k][x].  ROWS: element (x, k) is stored at X[x][k]
+// (op(A) = A, op(B) = B^T or B^H); otherwise at X[k][x] (op(A) = A^T or A^H, op(B) = B).
+// CONJ conjugates on load.  Pairs of adjacent elements along the contiguous direction
+// are loaded together; each thread loads P = BX*BK/(2*NT) pairs.
+template <bool ROWS, bool CONJ, int BX>
+struct TileIO {
+  static constexpr int P = BX * BK / (2 * NT);
+  __device__ static void coords(int t, int p, int& x, int& k) {
+    const int q = t + p * NT;  // pair id
+    if (ROWS) {                // contiguous along k: BK/2 pairs per x
+      x = q / (BK / 2);
+      k = (q % (BK / 2)) * 2;
+    } else {  // contiguous along x: BX/2 pairs per k
+      k = q / (BX / 2);
+      x = (q % (BX / 2)) * 2;
+    }
   }
-}
-
-template <int OPA>
-__device__ __forceinline__ void store_a(c128 (*As)[BM], const c128 (&r)[2]) {
-  const int t = threadIdx.x;
-  if (OPA == OP_N) {
-    const int m = t >> 2, kk = (t & 3) * 2;
-    As[kk][m] = r[0];
-    As[kk + 1][m] = r[1];
-  } else {
-    const int k = t >> 5, m = (t & 31) * 2;
-    As[k][m] = r[0];
-    As[k][m + 1] = r[1];
+  __device__ static void load(const c128* X, long ld, int nx, int nk, int x0, int k0, c128 (&r)[2 * P]) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      int x, k;
+      coords(threadIdx.x, p, x, k);
+      if (ROWS) {
+        r[2 * p] = ld_elem(X, ld, x0 + x, k0 + k, nx, nk, CONJ);
+        r[2 * p + 1] = ld_elem(X, ld, x0 + x, k0 + k + 1, nx, nk, CONJ);
+      } else {
+        r[2 * p] = ld_elem(X, ld, k0 + k, x0 + x, nk, nx, CONJ);
+        r[2 * p + 1] = ld_elem(X, ld, k0 + k, x0 + x + 1, nk, nx, CONJ);
+      }
+    }
   }
-}
-
-// Load the BK x BN tile of op(B).  OP_N: B is K x N.  OP_T/OP_C: B is N x K.
-template <int OPB>
-__device__ __forceinline__ void load_b(const c128* B, long ldb, int N, int K, int n0, int k0, c128 (&r)[2]) {
-  const int t = threadIdx.x;
-  if (OPB == OP_N) {
-    const int k = t >> 5, n = (t & 31) * 2;
-    r[0] = ld_elem(B, ldb, k0 + k, n0 + n, K, N, false);
-    r[1] = ld_elem(B, ldb, k0 + k, n0 + n + 1, K, N, false);
-  } else {
-    const int n = t >> 2, kk = (t & 3) * 2;
-    r[0] = ld_elem(B, ldb, n0 + n, k0 + kk, N, K, OPB == OP_C);
-    r[1] = ld_elem(B, ldb, n0 + n, k0 + kk + 1, N, K, OPB == OP_C);
+  __device__ static void store(c128 (*S)[BX], const c128 (&r)[2 * P]) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      int x, k;
+      coords(threadIdx.x, p, x, k);
+      if (ROWS) {
+        S[k][x] = r[2 * p];
+        S[k + 1][x] = r[2 * p + 1];
+      } else {
+        S[k][x] = r[2 * p];
+        S[k][x + 1] = r[2 * p + 1];
+      }
+    }
   }
-}
+};
 
-template <int OPB>
-__device__ __forceinline__ void store_b(c128 (*Bs)[BN], const c128 (&r)[2]) {
-  const int t = threadIdx.x;
-  if (OPB == OP_N) {
-    const int k = t >> 5, n = (t & 31) * 2;
-    Bs[k][n] = r[0];
-    Bs[k][n + 1] = r[1];
-  } else {
-    const int n = t >> 2, kk = (t & 3) * 2;
-    Bs[kk][n] = r[0];
-    Bs[kk + 1][n] = r[1];
-  }
-}
-
-template <int OPA, int OPB>
+template <int OPA, int OPB, int BM, int BN>
 __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
+  constexpr int TM = BM / 32, TN = BN / 32;  // 16x16 MFMA blocks per wave (waves are 2 x 2)
+  using IOA = TileIO<OPA == OP_N, OPA == OP_C, BM>;  // op(A)(m,k): A[m][k] or A[k][m]
+  using IOB = TileIO<OPB != OP_N, OPB == OP_C, BN>;  // op(B)(k,n): B[k][n] or B[n][k]
   __shared__ c128 As[2][BK][BM];
   __shared__ c128 Bs[2][BK][BN];
 
@@ -137,60 +128,65 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  f64x4 cr[2][2], ci[2][2];
+  f64x4 cr[TM][TN], ci[TM][TN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TN; ++j) {
       cr[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
       ci[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
     }
 
   const int nk = (p.K + BK - 1) / BK;
-  c128 ra[2], rb[2];
+  c128 ra[2 * IOA::P], rb[2 * IOB::P];
   if (nk > 0) {
-    load_a<OPA>(A, p.lda, p.M, p.K, m0, 0, ra);
-    load_b<OPB>(B, p.ldb, p.N, p.K, n0, 0, rb);
-    store_a<OPA>(As[0], ra);
-    store_b<OPB>(Bs[0], rb);
+    IOA::load(A, p.lda, p.M, p.K, m0, 0, ra);
+    IOB::load(B, p.ldb, p.N, p.K, n0, 0, rb);
+    IOA::store(As[0], ra);
+    IOB::store(Bs[0], rb);
   }
   __syncthreads();
 
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) {
-      load_a<OPA>(A, p.lda, p.M, p.K, m0, (kt + 1) * BK, ra);
-      load_b<OPB>(B, p.ldb, p.N, p.K, n0, (kt + 1) * BK, rb);
+      IOA::load(A, p.lda, p.M, p.K, m0, (kt + 1) * BK, ra);
+      IOB::load(B, p.ldb, p.N, p.K, n0, (kt + 1) * BK, rb);
     }
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
-      double a_r[2], a_i[2], a_n[2], b_r[2], b_i[2];
+      double a_r[TM], a_i[TM], a_n[TM], b_r[TN], b_i[TN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        c128 v = As[cur][ks * 4 + lk][wm * 32 + i * 16 + lr];
+      for (int i = 0; i < TM; ++i) {
+        const c128 v = As[cur][ks * 4 + lk][wm * (BM / 2) + i * 16 + lr];
         a_r[i] = v.x;
         a_i[i] = v.y;
         a_n[i] = -v.y;
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        c128 v = Bs[cur][ks * 4 + lk][wn * 32 + j * 16 + lr];
+      for (int j = 0; j < TN; ++j) {
+        const c128 v = Bs[cur][ks * 4 + lk][wn * (BN / 2) + j * 16 + lr];
         b_r[j] = v.x;
         b_i[j] = v.y;
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TN; ++j) {
           cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], cr[i][j], 0, 0, 0);
-          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[i], b_i[j], cr[i][j], 0, 0, 0);
           ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_i[j], ci[i][j], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[i], b_i[j], cr[i][j], 0, 0, 0);
           ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], ci[i][j], 0, 0, 0);
         }
     }
     if (kt + 1 < nk) {
-      store_a<OPA>(As[cur ^ 1], ra);
-      store_b<OPB>(Bs[cur ^ 1], rb);
+      IOA::store(As[cur ^ 1], ra);
+      IOB::store(Bs[cur ^ 1], rb);
     }
     __syncthreads();
   }
@@ -200,20 +196,20 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
   const bool beta0 = (p.br == 0.0 && p.bi == 0.0);
   const bool gamma0 = (p.gr == 0.0 && p.gi == 0.0);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = m0 + wm * 32 + i * 16 + lk + 4 * v;
-        const int c = n0 + wn * 32 + j * 16 + lr;
+        const int r = m0 + wm * (BM / 2) + i * 16 + lk + 4 * v;
+        const int c = n0 + wn * (BN / 2) + j * 16 + lr;
         if (r < p.M && c < p.N) {
           const double xr = cr[i][j][v], xi = ci[i][j][v];
           double outr = p.ar * xr - p.ai * xi;
           double outi = p.ar * xi + p.ai * xr;
           c128* dst = C + static_cast<long>(r) * p.ldc + c;
           if (!beta0) {
-            c128 old = *dst;
+            const c128 old = *dst;
             outr += p.br * old.x - p.bi * old.y;
             outi += p.br * old.y + p.bi * old.x;
           }
@@ -251,9 +247,30 @@ hipEvent_t pool_event() {
   return e;
 }
 
+// Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128.
 template <int OPA, int OPB>
-void launch(hipStream_t s, const Args& a, int nblocks) {
-  hipLaunchKernelGGL((zgemm_kernel<OPA, OPB>), dim3(nblocks), dim3(NT), 0, s, a);
+void launch(hipStream_t s, Args a, int shape) {
+  const int bm = shape == 0 ? 64 : 128, bn = shape == 2 ? 128 : 64;
+  a.tiles_m = (a.M + bm - 1) / bm;
+  a.tiles_n = (a.N + bn - 1) / bn;
+  const long nblocks = static_cast<long>(a.batch) * a.tiles_m * a.tiles_n;
+  THCGW_REQUIRE(nblocks < (1L << 31), "zgemm: grid too large");
+  const dim3 g(static_cast<unsigned>(nblocks)), t(NT);
+  if (shape == 0)
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, 64, 64>), g, t, 0, s, a);
+  else if (shape == 1)
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, 128, 64>), g, t, 0, s, a);
+  else
+    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, 128, 128>), g, t, 0, s, a);
+}
+
+std::atomic<int> g_tile_mode{0};  // 0 = automatic, else shape + 1
+
+// 64 x 64 tiles by default: at three waves per SIMD they hide load latency better than
+// the larger shapes at one wave per SIMD (measured on cfg2: 45.8 vs 26.4 TFLOP/s).
+int tile_shape(int, int, int) {
+  const int forced = g_tile_mode.load();
+  return forced > 0 ? forced - 1 : 0;
 }
 
 }  // namespace
@@ -268,8 +285,6 @@ void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
   a.N = N;
   a.K = K;
   a.batch = batch;
-  a.tiles_m = (M + BM - 1) / BM;
-  a.tiles_n = (N + BN - 1) / BN;
   a.ar = alpha.re;
   a.ai = alpha.im;
   a.br = beta.re;
@@ -283,9 +298,7 @@ void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
   a.C = C;
   a.strideC = strideC;
   a.ldc = ldc;
-  const long nblocks = static_cast<long>(batch) * a.tiles_m * a.tiles_n;
-  THCGW_REQUIRE(nblocks < (1L << 31), "zgemm: grid too large");
-  const int nb = static_cast<int>(nblocks);
+  const int nb = tile_shape(M, N, K);
   const double flops = 8.0 * double(M) * N * K * batch;
   std::unique_lock<std::mutex> lk(g_prof_mu, std::defer_lock);
   ProfRec rec{nullptr, nullptr, flops, opA * 3 + opB};
@@ -320,6 +333,8 @@ void gemm_stats_reset() {
   g_launches = 0;
   g_flops_milli = 0;
 }
+
+void gemm_set_tile_mode(int mode) { g_tile_mode = mode; }
 
 void gemm_profile_enable(bool on) {
   std::lock_guard<std::mutex> lk(g_prof_mu);

@@ -164,7 +164,8 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_t
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;
  *   flag 2: tau pairs per projection batch (0 = automatic);
  *   flag 3: matrices per Dyson batch (0 = automatic);
- *   flag 4: record HIP events around every complex-FP64 GEMM launch (process-wide).
+ *   flag 4: record HIP events around every complex-FP64 GEMM launch (process-wide);
+ *   flag 5: GEMM tile shape (process-wide): 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128.
  * thcgw_profile_read fills, for each of the 9 GEMM classes c = opA*3 + opB (op 0 = N,
  * 1 = T, 2 = C), the summed kernel time ms[c] of the launches recorded since the last read,
  * their modelled flops[c] (8 M N K per complex product) and launches[c], then drops the
