@@ -2,12 +2,21 @@
 
 Workload (default): N_k = 64 (4x4x4), N_orb = 52, N_mu = 800, beta = 1000, complex128
 throughout, on the shipped IR table with 72 tau points (the nearest to 80; lambda = 1e3,
-eps = 1e-15).  One step = chi0 -> tau_to_w -> Dyson W (+ RPA Ec from the same LU factors)
+eps = 1e-15).  One step = chi0 -> tau_to_w -> Dyson W (+ RPA Ec from the same factors)
 -> w_to_tau -> Sigma (+ hermitize), with G(tau,k), X and V resident in HBM.
 
-Prints one JSON line.  Also reported: the complex-FP64 GEMM kernel (the dominant kernel)
-measured with HIP events on its launch stream during the timed steps, and the CPU oracle
-timed on a bounded sample of the same workload and extrapolated to one iteration.
+Forms (--forms, the first is the headline value):
+  plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm);
+  hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
+  mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
+             transforms, Cholesky factorisations and ln det.
+Every further form is reported with its speedup over the first and its errors against it
+(W at q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
+
+Prints one JSON line.  Also reported: the complex-FP64 GEMM kernel (the dominant kernel of
+the plain form) measured with HIP events on its launch stream during the timed steps, and
+the CPU oracle timed on a bounded sample of the same workload and extrapolated to one
+iteration.
 """
 import argparse
 import faulthandler
@@ -32,6 +41,11 @@ CONFIGS = {
     "cfg1": ((2, 2, 2), 26, 200, "1e2.1e-15"),
     "cfg3": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
 }
+
+
+# form: (flag 6 Dyson algorithm, flag 7 precision)
+FORMS = {"plain": (0, 0), "hermitian": (1, 0), "mixed": (1, 1)}
+HA_PER_MEV = 1.0 / 27211.386245988
 
 
 def log(*a):
@@ -132,7 +146,13 @@ def main():
     ap.add_argument("--gemm-tile", type=int, default=0, help="GEMM tile shape: 0 auto, 1 64x64, 2 128x64, 3 128x128")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
+    ap.add_argument("--forms", default="plain,mixed",
+                    help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
     args = ap.parse_args()
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    for f in forms:
+        if f not in FORMS:
+            ap.error("unknown form %r (choose from %s)" % (f, ", ".join(FORMS)))
     if args.cpu_child:
         cpu_baseline_child(args.cpu_child)
         return
@@ -168,32 +188,69 @@ def main():
     log("bench: inputs ready in %.1f s" % (time.perf_counter() - t0))
 
     stream = torch.cuda.current_stream()
-    for i in range(args.warmup):
-        _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
-        log("bench: warmup %d done, Ec = %.12f" % (i, ec))
-    ctx.profile_read()
-    ctx.set_flag(4, 1)  # GEMM launch events during the timed steps
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for i in range(args.steps):
-        _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
-        log("bench: step %d done" % i)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    ms_step = ev0.elapsed_time(ev1) / args.steps
-    ctx.set_flag(4, 0)
-    prof = ctx.profile_read()
+
+    def run_form(form, steps, warmup):
+        ctx.set_flag(6, FORMS[form][0])
+        ctx.set_flag(7, FORMS[form][1])
+        ec = None
+        for i in range(warmup):
+            _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
+            log("bench: %s warmup %d done, Ec = %.12f" % (form, i, ec))
+        ctx.profile_read()
+        ctx.set_flag(4, 1)  # GEMM launch events during the timed steps
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for i in range(steps):
+            _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
+            log("bench: %s step %d done" % (form, i))
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        ctx.set_flag(4, 0)
+        return ev0.elapsed_time(ev1) / steps, wall, ec, ctx.profile_read()
+
+    ms_step, wall, ec, prof = run_form(forms[0], args.steps, args.warmup)
+    log("bench: %s %.1f ms per step" % (forms[0], ms_step))
     g_ms, g_fl, g_n = prof.get("NN", (0.0, 0.0, 0))  # Dyson, LU and triangular-solve GEMMs
-    a_ms = sum(v[0] for v in prof.values())
-    a_fl = sum(v[1] for v in prof.values())
+    a_ms = sum(v[0] for k, v in prof.items() if not k.endswith("32"))
+    a_fl = sum(v[1] for k, v in prof.items() if not k.endswith("32"))
     achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     gemm_classes = {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
                         "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
                     for k, v in prof.items()}
+
+    other_forms = {}
+    if len(forms) > 1:
+        # reference results of the headline form: Sigma, W at q = 0, Ec, QP energies
+        H = torch.from_numpy(s.H).cuda()
+        Cd = torch.from_numpy(s.C).cuda()
+        Sig_ref = Sig.clone()
+        W0_ref = Wqt[0].clone()
+        ec_ref = ec
+        E_ref = ctx.solve_qp(Sig_ref, Cd, H, s.E, mu=s.mu)
+        for form in forms[1:]:
+            f_ms, f_wall, f_ec, f_prof = run_form(form, args.steps, 1)
+            E_f = ctx.solve_qp(Sig, Cd, H, s.E, mu=s.mu)
+            rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
+            other_forms[form] = {
+                "ms_per_step": round(f_ms, 3),
+                "speedup_vs_%s" % forms[0]: round(ms_step / f_ms, 3),
+                "err_W_q0_rel": rel(Wqt[0], W0_ref),
+                "err_Sigma_rel": rel(Sig, Sig_ref),
+                "err_Ec_Ha": abs(f_ec - ec_ref),
+                "err_qp_meV": float(np.max(np.abs(E_f - E_ref)) / HA_PER_MEV),
+                "gemm_classes": {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
+                                     "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
+                                 for k, v in f_prof.items()},
+            }
+            log("bench: %s %.1f ms per step, %s" % (form, f_ms, json.dumps(
+                {k: v for k, v in other_forms[form].items() if k != "gemm_classes"})))
+        del Sig_ref, W0_ref, H, Cd
+        ctx.set_flag(6, FORMS[forms[0]][0])
+        ctx.set_flag(7, FORMS[forms[0]][1])
 
     if args.copy_calib:
         xa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -253,12 +310,14 @@ def main():
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
-        "config": {"workload": "%s plain FP64 THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % args.config,
+        "config": {"workload": "%s %s THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % (args.config, forms[0]),
                    "kgrid": list(kgrid), "nk": s.nk, "n_orb": no, "n_mu": nmu, "n_tau": ia.nt,
                    "n_iw_b": ia.nw_b, "ir_table": table, "beta": beta, "parallelism": "single GPU"},
         "Ec_Ha": ec,
-        "speedup_vs_plain": 1.0,
-        "qp_error_meV": 0.0,
+        "form": forms[0],
+        "speedup_vs_plain": 1.0 if forms[0] == "plain" else None,
+        "qp_error_meV": 0.0 if forms[0] == "plain" else None,
+        "forms": other_forms,
         "roofline": {"kernel": "zgemm_kernel<0,0> (complex FP64 GEMM, N,N)", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,

@@ -12,7 +12,10 @@
 #include <vector>
 
 using thcgw::c128;
+using thcgw::c64;
 using thcgw::Cplx;
+using thcgw::DT_C128;
+using thcgw::DT_C64;
 
 namespace {
 
@@ -40,6 +43,8 @@ struct thcgw_ctx {
   bool force_fallback = false;
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
+  int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky)
+  int precision = 0;    // 0 = FP64 throughout, 1 = mixed (FP32 products, FP64 factorisations and IR)
 
   template <class T>
   T* buf(const std::string& name, size_t count) {
@@ -132,6 +137,38 @@ void project_a2p(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, 
                kZero, Oout, static_cast<long>(no) * no, no, static_cast<int>(nbk));
 }
 
+// Mixed precision: X rounded to complex<float> once per call, FP32 products.
+const c64* x_c64(thcgw_ctx* ctx, const c128* X, hipStream_t s) {
+  const long n = static_cast<long>(ctx->nk) * ctx->d.nmu * ctx->d.no;
+  c64* X64 = ctx->buf<c64>("mx_x64", static_cast<size_t>(n));
+  thcgw::to_c64(s, X, X64, n);
+  return X64;
+}
+
+// O_out[b,k] = X_k O_in[b,k] X_k^H with O_in complex<double>, result complex<float>.
+void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* Oin, c64* Oout, long nbk, c64* Abuf) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
+  const long sX = static_cast<long>(nmu) * no;
+  THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
+  thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
+               thcgw::bp(Oin, static_cast<long>(no) * no), DT_C128, no, kZero, Abuf, DT_C64, sX, no,
+               static_cast<int>(nbk));
+  thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, nmu, nmu, no, kOne, thcgw::bp(Abuf, sX), DT_C64, no,
+               thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C64, ctx->M, nmu, static_cast<int>(nbk));
+}
+
+// O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>.
+void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c64* Oin, c128* Oout, long nbk, c64* Bbuf) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
+  const long sX = static_cast<long>(nmu) * no;
+  THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
+  thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
+               thcgw::bp(Oin, ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu, static_cast<int>(nbk));
+  thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), DT_C64, nmu,
+               thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C128, static_cast<long>(no) * no, no,
+               static_cast<int>(nbk));
+}
+
 int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
   if (ctx->pair_chunk > 0) return std::min(ctx->pair_chunk, ctx->nt_half);
   const double per = double(per_pair_arrays) * ctx->nk * double(ctx->M) * 16.0;
@@ -140,7 +177,28 @@ int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
   return std::max(1, std::min(c, ctx->nt_half));
 }
 
+void run_pi_f32(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const long M = ctx->M;
+  const long gs = static_cast<long>(nk) * no * no;
+  const int P = choose_pair_chunk(ctx, 1);
+  const c64* X64 = x_c64(ctx, X, s);
+  c64* GP = ctx->buf<c64>("mx_gp", static_cast<size_t>(P) * nk * M);
+  c64* GN = ctx->buf<c64>("mx_gn", static_cast<size_t>(P) * nk * M);
+  c64* Ab = ctx->buf<c64>("mx_ab", static_cast<size_t>(P) * nk * nmu * no);
+  const double scale = -2.0 / (double(nk) * nk);
+  for (int i0 = 0; i0 < nth; i0 += P) {
+    const int c = std::min(P, nth - i0);
+    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
+    project_p2a_f32(ctx, s, X64, G + (nt - i0 - c) * gs, GN, static_cast<long>(c) * nk, Ab);
+    for (int j = 0; j < c; ++j)
+      thcgw::pi_kfft_f32(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
+                         GN + static_cast<long>(c - 1 - j) * nk * M, Pi + static_cast<long>(i0 + j) * nk * M, M, scale);
+  }
+}
+
 void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
+  if (ctx->precision == 1) return run_pi_f32(ctx, G, X, Pi, s);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
   const long gs = static_cast<long>(nk) * no * no;  // one tau slice of G
@@ -186,38 +244,145 @@ int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
   return std::max(1, std::min(c, total));
 }
 
+// One chunk of matrices b0 .. b0+cb of the LU form: A = I - V Pi (or I - Pi V) -> getrf ->
+// ln|Re det|; with solve, (I - V Pi) Y = V and W - V = Y - V replace getri + GEMM
+// (scr_coulomb_t.cpp:180-196, thc_rpa.icc:141-163).
+void dyson_chunk_lu(thcgw_ctx* ctx, c128* PiW, const c128* V, int b0, int cb, c128* d_trace, double* d_ldet,
+                    bool solve, bool pi_first, c128* LU, c128* Z, int* ipiv, int* info, hipStream_t s) {
+  const int n = ctx->d.nmu, nq = ctx->nk;
+  const long M = ctx->M;
+  if (!pi_first)
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(V, M, nq, 1, b0), n,
+                 thcgw::bp(PiW + static_cast<long>(b0) * M, M), n, kZero, LU, M, n, cb, kOne);
+  else
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(PiW + static_cast<long>(b0) * M, M), n,
+                 thcgw::bp(V, M, nq, 1, b0), n, kZero, LU, M, n, cb, kOne);
+  if (d_trace) thcgw::one_minus_trace(s, LU, M, n, d_trace + b0, cb);
+  thcgw::getrf_batched(s, LU, M, n, n, ipiv, info, cb);
+  if (d_ldet) thcgw::log_re_det(s, LU, M, n, ipiv, d_ldet + b0, cb);
+  if (solve) {
+    thcgw::copy_batch(s, V, M, nq, Z, M, M, b0, cb);
+    thcgw::trsm_left_lower(s, LU, M, n, Z, M, n, n, n, cb);
+    thcgw::trsm_left_upper_unit(s, LU, M, n, Z, M, n, n, n, cb);
+    thcgw::dyson_finish(s, Z, M, V, M, nq, ipiv, n, PiW + static_cast<long>(b0) * M, M, b0, cb);
+  }
+}
+
+bool any_nonzero(const int* d_info, int count, hipStream_t s) {
+  std::vector<int> h(count);
+  THCGW_HIP(hipMemcpyAsync(h.data(), d_info, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  for (int x : h)
+    if (x != 0) return true;
+  return false;
+}
+
+// Lower Cholesky factors L_q of V_q (strict upper triangles zeroed); false when some V_q
+// is not positive definite.
+bool factor_v(thcgw_ctx* ctx, const c128* V, c128* Lv, hipStream_t s) {
+  const int n = ctx->d.nmu, nq = ctx->nk;
+  const long M = ctx->M;
+  int* info = ctx->buf<int>("dy_vinfo", static_cast<size_t>(nq));
+  thcgw::copy_batch(s, V, M, nq, Lv, M, M, 0, nq);
+  thcgw::potrf_batched(s, Lv, M, n, n, info, nq);
+  thcgw::zero_upper(s, Lv, M, n, nq);
+  return !any_nonzero(info, nq, s);
+}
+
+// One chunk of the Hermitian form (see chol.hip): M = I - L^H Pi L, M = L_M L_M^H,
+// ln det = 2 sum ln L_M[j][j], W = K K^H - V with K = L L_M^-H.  Returns false, leaving
+// the chunk's Pi slots untouched, when some M is not positive definite.
+bool dyson_chunk_hpd(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Lv, int b0, int cb, c128* d_trace,
+                     double* d_ldet, bool solve, c128* Mb, c128* Z, int* info, hipStream_t s) {
+  const int n = ctx->d.nmu, nq = ctx->nk;
+  const long M = ctx->M;
+  c128* P = PiW + static_cast<long>(b0) * M;
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(P, M), n, thcgw::bp(Lv, M, nq, 1, b0), n, kZero,
+               Z, M, n, cb, kZero, thcgw::GEMM_TRI_KFROM_COL);
+  thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(Lv, M, nq, 1, b0), n, thcgw::bp(Z, M), n,
+               kZero, Mb, M, n, cb, kOne, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW);
+  if (d_trace) thcgw::one_minus_trace(s, Mb, M, n, d_trace + b0, cb);
+  thcgw::potrf_batched(s, Mb, M, n, n, info, cb);
+  if (any_nonzero(info, cb, s)) return false;
+  if (d_ldet) thcgw::chol_logdet(s, Mb, M, n, d_ldet + b0, cb);
+  if (solve) {
+    thcgw::copy_batch(s, Lv, M, nq, Z, M, M, b0, cb);
+    thcgw::trsm_right_lower_conj(s, Mb, M, n, Z, M, n, n, n, cb);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), n, thcgw::bp(Z, M), n, kZero, P, M, n,
+                 cb, kZero, thcgw::GEMM_TRI_LOWER);
+    thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
+  }
+  return true;
+}
+
+// The Hermitian form with the products in FP32 (L, C and K stored as complex<float>) and
+// M, its Cholesky factor and ln det in FP64.
+bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv64, int b0, int cb, c128* d_trace,
+                         double* d_ldet, bool solve, c128* Mb, c64* Z, int* info, hipStream_t s) {
+  const int n = ctx->d.nmu, nq = ctx->nk;
+  const long M = ctx->M;
+  c128* P = PiW + static_cast<long>(b0) * M;
+  // M feeds ln det: both products keep FP64 sums across K chunks
+  thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(P, M), DT_C128, n, thcgw::bp(Lv64, M, nq, 1, b0),
+               DT_C64, n, kZero, Z, DT_C64, M, n, cb, kZero, thcgw::GEMM_TRI_KFROM_COL | thcgw::GEMM_FOLD64);
+  thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(Lv64, M, nq, 1, b0), DT_C64, n,
+               thcgw::bp(Z, M), DT_C64, n, kZero, Mb, DT_C128, M, n, cb, kOne,
+               thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW | thcgw::GEMM_FOLD64);
+  if (d_trace) thcgw::one_minus_trace(s, Mb, M, n, d_trace + b0, cb);
+  thcgw::potrf_batched(s, Mb, M, n, n, info, cb);
+  if (any_nonzero(info, cb, s)) return false;
+  if (d_ldet) thcgw::chol_logdet(s, Mb, M, n, d_ldet + b0, cb);
+  if (solve) {
+    thcgw::copy_batch(s, Lv64, M, nq, Z, M, M, b0, cb);
+    thcgw::trsm_right_lower_conj_f32(s, Mb, M, n, Z, M, n, n, n, cb);
+    thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), DT_C64, n, thcgw::bp(Z, M), DT_C64, n,
+                 kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER);
+    thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
+  }
+  return true;
+}
+
 // In place Pi(iW) -> W(iW) - V; optional Tr(V Pi) and ln|Re det(I - V Pi)| per matrix
 // (device arrays of length nw_half*nk).  pi_first selects A = I - Pi V (thc_rpa.icc:141)
-// instead of I - V Pi, and solve = false stops after the determinant.
+// instead of I - V Pi, and solve = false stops after the determinant.  The Hermitian
+// form is used when selected and V and every M are positive definite; chunks where it
+// does not apply run the LU form.
 void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* d_ldet, bool solve, bool pi_first,
                hipStream_t s) {
   const int n = ctx->d.nmu, nq = ctx->nk;
   const long M = ctx->M;
   const int total = ctx->nw_half * nq;
   const int CH = choose_dyson_chunk(ctx, total);
+  const bool mixed = ctx->precision == 1;
+  const bool want_hpd = mixed || ctx->dyson_algo == 1;
   c128* LU = ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M);
-  c128* Z = solve ? ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M) : nullptr;
-  int* ipiv = ctx->buf<int>("dy_ipiv", static_cast<size_t>(CH) * n);
   int* info_all = ctx->buf<int>("dy_info", static_cast<size_t>(total));
+  bool hpd = false;
+  c128* Lv = nullptr;
+  c64* Lv64 = nullptr;
+  if (want_hpd) {
+    Lv = ctx->buf<c128>("dy_lv", static_cast<size_t>(nq) * M);
+    hpd = factor_v(ctx, V, Lv, s);
+    if (hpd && mixed) {
+      Lv64 = ctx->buf<c64>("dy_lv64", static_cast<size_t>(nq) * M);
+      thcgw::to_c64(s, Lv, Lv64, static_cast<long>(nq) * M);
+    }
+  }
+  c128* Z = nullptr;
+  c64* Z64 = nullptr;
+  int* ipiv = nullptr;
   for (int b0 = 0; b0 < total; b0 += CH) {
     const int cb = std::min(CH, total - b0);
-    // A = I - V Pi   (scr_coulomb_t.cpp:180-185)  or  A = I - Pi V  (thc_rpa.icc:141-150)
-    if (!pi_first)
-      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(V, M, nq, 1, b0), n,
-                   thcgw::bp(PiW + static_cast<long>(b0) * M, M), n, kZero, LU, M, n, cb, kOne);
-    else
-      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(PiW + static_cast<long>(b0) * M, M), n,
-                   thcgw::bp(V, M, nq, 1, b0), n, kZero, LU, M, n, cb, kOne);
-    if (d_trace) thcgw::one_minus_trace(s, LU, M, n, d_trace + b0, cb);
-    thcgw::getrf_batched(s, LU, M, n, n, ipiv, info_all + b0, cb);
-    if (d_ldet) thcgw::log_re_det(s, LU, M, n, ipiv, d_ldet + b0, cb);
-    if (solve) {
-      // (I - V Pi) Y = V  ->  W - V = Y - V   (replaces getri + GEMM, scr_coulomb_t.cpp:188-196)
-      thcgw::copy_batch(s, V, M, nq, Z, M, M, b0, cb);
-      thcgw::trsm_left_lower(s, LU, M, n, Z, M, n, n, n, cb);
-      thcgw::trsm_left_upper_unit(s, LU, M, n, Z, M, n, n, n, cb);
-      thcgw::dyson_finish(s, Z, M, V, M, nq, ipiv, n, PiW + static_cast<long>(b0) * M, M, b0, cb);
+    if (hpd && mixed) {
+      if (!Z64) Z64 = ctx->buf<c64>("dy_z64", static_cast<size_t>(CH) * M);
+      if (dyson_chunk_hpd_f32(ctx, PiW, V, Lv64, b0, cb, d_trace, d_ldet, solve, LU, Z64, info_all + b0, s)) continue;
+    } else if (hpd) {
+      if (!Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
+      if (dyson_chunk_hpd(ctx, PiW, V, Lv, b0, cb, d_trace, d_ldet, solve, LU, Z, info_all + b0, s)) continue;
     }
+    if (!ipiv) ipiv = ctx->buf<int>("dy_ipiv", static_cast<size_t>(CH) * n);
+    if (solve && !Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
+    dyson_chunk_lu(ctx, PiW, V, b0, cb, d_trace, d_ldet, solve, pi_first, LU, Z, ipiv, info_all + b0, s);
   }
   std::vector<int> info(total);
   THCGW_HIP(hipMemcpyAsync(info.data(), info_all, sizeof(int) * total, hipMemcpyDeviceToHost, s));
@@ -254,7 +419,41 @@ void fetch_ec_terms(thcgw_ctx* ctx, const c128* d_tr, const double* d_ld, std::v
   THCGW_HIP(hipStreamSynchronize(s));
 }
 
+void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm,
+                   hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const long M = ctx->M;
+  const long gs = static_cast<long>(nk) * no * no;
+  const int P = choose_pair_chunk(ctx, 2);
+  const c64* X64 = x_c64(ctx, X, s);
+  c64* GP = ctx->buf<c64>("mx_gp", static_cast<size_t>(P) * nk * M);
+  c64* GN = ctx->buf<c64>("mx_gn", static_cast<size_t>(P) * nk * M);
+  c64* SP = ctx->buf<c64>("mx_sp", static_cast<size_t>(P) * nk * M);
+  c64* SN = ctx->buf<c64>("mx_sn", static_cast<size_t>(P) * nk * M);
+  c64* Ab = ctx->buf<c64>("mx_ab", static_cast<size_t>(P) * nk * nmu * no);
+  const double scale = -1.0 / (double(nk) * nk);
+  for (int i0 = 0; i0 < nth; i0 += P) {
+    const int c = std::min(P, nth - i0);
+    const int c2 = std::max(0, std::min(c, nt / 2 - i0));
+    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
+    if (c2 > 0) project_p2a_f32(ctx, s, X64, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab);
+    for (int j = 0; j < c; ++j) {
+      const int i = i0 + j;
+      const bool second = j < c2;
+      thcgw::sigma_kfft_f32(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
+                            second ? GN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr,
+                            Wqt + static_cast<long>(i) * M, static_cast<long>(nth) * M,
+                            SP + static_cast<long>(j) * nk * M,
+                            second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale);
+    }
+    project_a2p_f32(ctx, s, X64, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);
+    if (c2 > 0) project_a2p_f32(ctx, s, X64, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
+  }
+  if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
+}
+
 void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm, hipStream_t s) {
+  if (ctx->precision == 1) return run_sigma_f32(ctx, G, Wqt, X, Sig, herm, s);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
   const long gs = static_cast<long>(nk) * no * no;
@@ -372,6 +571,13 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     else if (flag == 3) ctx->dyson_chunk = static_cast<int>(value);
     else if (flag == 4) thcgw::gemm_profile_enable(value != 0);
     else if (flag == 5) thcgw::gemm_set_tile_mode(static_cast<int>(value));
+    else if (flag == 6) {
+      THCGW_REQUIRE(value == 0 || value == 1, "flag 6 (Dyson algorithm) takes 0 (LU) or 1 (Hermitian)");
+      ctx->dyson_algo = static_cast<int>(value);
+    } else if (flag == 7) {
+      THCGW_REQUIRE(value == 0 || value == 1, "flag 7 (precision) takes 0 (FP64) or 1 (mixed)");
+      ctx->precision = static_cast<int>(value);
+    }
     else THCGW_REQUIRE(false, "unknown flag");
   });
 }

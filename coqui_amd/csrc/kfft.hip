@@ -20,40 +20,53 @@
 namespace thcgw {
 namespace {
 
-__device__ __forceinline__ c128 cadd(c128 a, c128 b) { return c128{a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ c128 csub(c128 a, c128 b) { return c128{a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ c128 cmul(c128 a, c128 b) {
-  return c128{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+template <typename RC>
+using scal_t = decltype(RC{}.x);
+
+template <typename RC>
+__device__ __forceinline__ RC cadd(RC a, RC b) { return RC{a.x + b.x, a.y + b.y}; }
+template <typename RC>
+__device__ __forceinline__ RC csub(RC a, RC b) { return RC{a.x - b.x, a.y - b.y}; }
+template <typename RC>
+__device__ __forceinline__ RC cmul(RC a, RC b) {
+  return RC{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
-__device__ __forceinline__ c128 cmulconj(c128 a, c128 b) {  // a * conj(b)
-  return c128{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+template <typename RC>
+__device__ __forceinline__ RC cmulconj(RC a, RC b) {  // a * conj(b)
+  return RC{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
 }
-__device__ __forceinline__ c128 cscale(c128 a, double s) { return c128{a.x * s, a.y * s}; }
+template <typename RC>
+__device__ __forceinline__ RC cscale(RC a, scal_t<RC> s) { return RC{a.x * s, a.y * s}; }
 // multiply by SIGN * i
-template <int SIGN>
-__device__ __forceinline__ c128 mul_si(c128 a) {
-  return SIGN > 0 ? c128{-a.y, a.x} : c128{a.y, -a.x};
+template <int SIGN, typename RC>
+__device__ __forceinline__ RC mul_si(RC a) {
+  return SIGN > 0 ? RC{-a.y, a.x} : RC{a.y, -a.x};
+}
+template <typename RC, typename T>
+__device__ __forceinline__ RC cvt(T v) {
+  return RC{static_cast<scal_t<RC>>(v.x), static_cast<scal_t<RC>>(v.y)};
 }
 
 // y_m = sum_j x_j w^{jm}, w = e^{SIGN 2 pi i / N}, on elements base + j*stride of x.
-template <int N, int SIGN, int L>
-__device__ __forceinline__ void dft_line(c128 (&x)[L], int base, int stride) {
+template <int N, int SIGN, int L, typename RC>
+__device__ __forceinline__ void dft_line(RC (&x)[L], int base, int stride) {
+  using S = scal_t<RC>;
   if constexpr (N == 2) {
-    c128 a = x[base], b = x[base + stride];
+    RC a = x[base], b = x[base + stride];
     x[base] = cadd(a, b);
     x[base + stride] = csub(a, b);
   } else if constexpr (N == 3) {
-    const double h = 0.86602540378443864676;  // sqrt(3)/2
-    c128 x0 = x[base], x1 = x[base + stride], x2 = x[base + 2 * stride];
-    c128 a = cadd(x1, x2), b = csub(x1, x2);
-    c128 m = c128{x0.x - 0.5 * a.x, x0.y - 0.5 * a.y};
-    c128 t = mul_si<SIGN>(cscale(b, h));
+    const S h = static_cast<S>(0.86602540378443864676);  // sqrt(3)/2
+    RC x0 = x[base], x1 = x[base + stride], x2 = x[base + 2 * stride];
+    RC a = cadd(x1, x2), b = csub(x1, x2);
+    RC m = RC{x0.x - S(0.5) * a.x, x0.y - S(0.5) * a.y};
+    RC t = mul_si<SIGN>(cscale(b, h));
     x[base] = cadd(x0, a);
     x[base + stride] = cadd(m, t);
     x[base + 2 * stride] = csub(m, t);
   } else if constexpr (N == 4) {
-    c128 x0 = x[base], x1 = x[base + stride], x2 = x[base + 2 * stride], x3 = x[base + 3 * stride];
-    c128 a0 = cadd(x0, x2), a1 = csub(x0, x2), b0 = cadd(x1, x3), b1 = mul_si<SIGN>(csub(x1, x3));
+    RC x0 = x[base], x1 = x[base + stride], x2 = x[base + 2 * stride], x3 = x[base + 3 * stride];
+    RC a0 = cadd(x0, x2), a1 = csub(x0, x2), b0 = cadd(x1, x3), b1 = mul_si<SIGN>(csub(x1, x3));
     x[base] = cadd(a0, b0);
     x[base + stride] = cadd(a1, b1);
     x[base + 2 * stride] = csub(a0, b0);
@@ -63,33 +76,37 @@ __device__ __forceinline__ void dft_line(c128 (&x)[L], int base, int stride) {
 
 // Value of the lane selected by the quad_perm control CTRL within each group of 4 lanes.
 template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v) {
+__device__ __forceinline__ double dpp_s(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
   lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
   hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 template <int CTRL>
-__device__ __forceinline__ c128 dpp_c(c128 v) {
-  return c128{dpp_d<CTRL>(v.x), dpp_d<CTRL>(v.y)};
+__device__ __forceinline__ float dpp_s(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL, typename RC>
+__device__ __forceinline__ RC dpp_c(RC v) {
+  return RC{dpp_s<CTRL>(v.x), dpp_s<CTRL>(v.y)};
 }
 
 // DFT along the first k index when it is split over S lanes (S == N1): lane h holds
 // plane i = h.  Every lane forms all N1 outputs and keeps output plane h.
-template <int N1, int SIGN, int L>
-__device__ __forceinline__ void dft_split(c128 (&x)[L], int h) {
+template <int N1, int SIGN, int L, typename RC>
+__device__ __forceinline__ void dft_split(RC (&x)[L], int h) {
   static_assert(N1 == 2 || N1 == 4, "split DFT needs N1 in {2,4}");
 #pragma unroll
   for (int e = 0; e < L; ++e) {
-    c128 v = x[e];
+    RC v = x[e];
     if constexpr (N1 == 2) {
-      c128 v0 = dpp_c<0xA0>(v);  // quad_perm [0,0,2,2]
-      c128 v1 = dpp_c<0xF5>(v);  // quad_perm [1,1,3,3]
+      RC v0 = dpp_c<0xA0>(v);  // quad_perm [0,0,2,2]
+      RC v1 = dpp_c<0xF5>(v);  // quad_perm [1,1,3,3]
       x[e] = (h == 0) ? cadd(v0, v1) : csub(v0, v1);
     } else {
-      c128 v0 = dpp_c<0x00>(v), v1 = dpp_c<0x55>(v), v2 = dpp_c<0xAA>(v), v3 = dpp_c<0xFF>(v);
-      c128 a0 = cadd(v0, v2), a1 = csub(v0, v2), b0 = cadd(v1, v3), b1 = mul_si<SIGN>(csub(v1, v3));
-      c128 y;
+      RC v0 = dpp_c<0x00>(v), v1 = dpp_c<0x55>(v), v2 = dpp_c<0xAA>(v), v3 = dpp_c<0xFF>(v);
+      RC a0 = cadd(v0, v2), a1 = csub(v0, v2), b0 = cadd(v1, v3), b1 = mul_si<SIGN>(csub(v1, v3));
+      RC y;
       if (h == 0) y = cadd(a0, b0);
       else if (h == 1) y = cadd(a1, b1);
       else if (h == 2) y = csub(a0, b0);
@@ -101,8 +118,8 @@ __device__ __forceinline__ void dft_split(c128 (&x)[L], int h) {
 
 // Separable 3-D DFT of the thread's share: P = N1/S planes of N2 x N3 points,
 // element index e = (p * N2 + j) * N3 + l.
-template <int N1, int N2, int N3, int S, int SIGN>
-__device__ __forceinline__ void dft3(c128 (&x)[(N1 / S) * N2 * N3], int h) {
+template <int N1, int N2, int N3, int S, int SIGN, typename RC>
+__device__ __forceinline__ void dft3(RC (&x)[(N1 / S) * N2 * N3], int h) {
   constexpr int P = N1 / S;
   constexpr int L = P * N2 * N3;
   if constexpr (N3 > 1) {
@@ -125,74 +142,77 @@ __device__ __forceinline__ void dft3(c128 (&x)[(N1 / S) * N2 * N3], int h) {
   }
 }
 
-template <int N1, int N2, int N3, int S>
-__device__ __forceinline__ void load_col(c128 (&x)[(N1 / S) * N2 * N3], const c128* src, long kstride,
-                                         long col, int h) {
+template <int N1, int N2, int N3, int S, typename RC, typename T>
+__device__ __forceinline__ void load_col(RC (&x)[(N1 / S) * N2 * N3], const T* src, long kstride, long col,
+                                         int h) {
   constexpr int P = N1 / S;
 #pragma unroll
   for (int p = 0; p < P; ++p)
 #pragma unroll
     for (int jl = 0; jl < N2 * N3; ++jl) {
       const int i = (S == 1) ? p : h;
-      x[p * N2 * N3 + jl] = src[static_cast<long>(i * N2 * N3 + jl) * kstride + col];
+      x[p * N2 * N3 + jl] = cvt<RC>(src[static_cast<long>(i * N2 * N3 + jl) * kstride + col]);
     }
 }
 
-template <int N1, int N2, int N3, int S>
-__device__ __forceinline__ void store_col(const c128 (&x)[(N1 / S) * N2 * N3], c128* dst, long kstride,
-                                          long col, int h) {
+template <int N1, int N2, int N3, int S, typename RC, typename T>
+__device__ __forceinline__ void store_col(const RC (&x)[(N1 / S) * N2 * N3], T* dst, long kstride, long col,
+                                          int h) {
   constexpr int P = N1 / S;
 #pragma unroll
   for (int p = 0; p < P; ++p)
 #pragma unroll
     for (int jl = 0; jl < N2 * N3; ++jl) {
       const int i = (S == 1) ? p : h;
-      dst[static_cast<long>(i * N2 * N3 + jl) * kstride + col] = x[p * N2 * N3 + jl];
+      dst[static_cast<long>(i * N2 * N3 + jl) * kstride + col] = cvt<T>(x[p * N2 * N3 + jl]);
     }
 }
 
 constexpr int KT = 256;
 
 // Pi(:, col) = scale * F+[ F-'[Gp] o conj(F-'[Gn]) ], F-' the unscaled forward DFT; the
-// caller passes scale = sp / N_k^2.
-template <int N1, int N2, int N3, int S>
-__global__ __launch_bounds__(KT) void pi_kfft_kernel(const c128* __restrict__ Gp, const c128* __restrict__ Gn,
-                                                     c128* __restrict__ Pi, long M, double scale) {
+// caller passes scale = sp / N_k^2.  Arithmetic in the scalar type of RC; TG and TO are
+// the storage types of G and Pi.
+template <int N1, int N2, int N3, int S, typename RC, typename TG, typename TO>
+__global__ __launch_bounds__(KT) void pi_kfft_kernel(const TG* __restrict__ Gp, const TG* __restrict__ Gn,
+                                                     TO* __restrict__ Pi, long M, double scale) {
   constexpr int L = (N1 / S) * N2 * N3;
   const long gid = static_cast<long>(blockIdx.x) * KT + threadIdx.x;
   const long col = gid / S;
   const int h = static_cast<int>(gid % S);
   if (col >= M) return;  // a column's S lanes are adjacent, so groups exit together
-  c128 a[L], b[L];
+  RC a[L], b[L];
   load_col<N1, N2, N3, S>(a, Gp, M, col, h);
   load_col<N1, N2, N3, S>(b, Gn, M, col, h);
   dft3<N1, N2, N3, S, -1>(a, h);
   dft3<N1, N2, N3, S, -1>(b, h);
+  const scal_t<RC> sc = static_cast<scal_t<RC>>(scale);
 #pragma unroll
-  for (int e = 0; e < L; ++e) a[e] = cscale(cmulconj(a[e], b[e]), scale);
+  for (int e = 0; e < L; ++e) a[e] = cscale(cmulconj(a[e], b[e]), sc);
   dft3<N1, N2, N3, S, +1>(a, h);
   store_col<N1, N2, N3, S>(a, Pi, M, col, h);
 }
 
 // S(:, col) = scale * F+[ F-'[G] o F-'[W] ] for up to two G arrays sharing W; scale = -1/N_k^2.
-template <int N1, int N2, int N3, int S>
-__global__ __launch_bounds__(KT) void sigma_kfft_kernel(const c128* __restrict__ G1, const c128* __restrict__ G2,
-                                                        const c128* __restrict__ W, long w_kstride,
-                                                        c128* __restrict__ S1, c128* __restrict__ S2, long M,
+template <int N1, int N2, int N3, int S, typename RC, typename TG, typename TW, typename TO>
+__global__ __launch_bounds__(KT) void sigma_kfft_kernel(const TG* __restrict__ G1, const TG* __restrict__ G2,
+                                                        const TW* __restrict__ W, long w_kstride,
+                                                        TO* __restrict__ S1, TO* __restrict__ S2, long M,
                                                         double scale) {
   constexpr int L = (N1 / S) * N2 * N3;
   const long gid = static_cast<long>(blockIdx.x) * KT + threadIdx.x;
   const long col = gid / S;
   const int h = static_cast<int>(gid % S);
   if (col >= M) return;
-  c128 w[L], g[L];
+  RC w[L], g[L];
   load_col<N1, N2, N3, S>(w, W, w_kstride, col, h);
   dft3<N1, N2, N3, S, -1>(w, h);
+  const scal_t<RC> sc = static_cast<scal_t<RC>>(scale);
 #pragma unroll
-  for (int e = 0; e < L; ++e) w[e] = cscale(w[e], scale);
+  for (int e = 0; e < L; ++e) w[e] = cscale(w[e], sc);
   for (int pass = 0; pass < 2; ++pass) {
-    const c128* G = pass == 0 ? G1 : G2;
-    c128* O = pass == 0 ? S1 : S2;
+    const TG* G = pass == 0 ? G1 : G2;
+    TO* O = pass == 0 ? S1 : S2;
     if (G == nullptr) break;
     load_col<N1, N2, N3, S>(g, G, M, col, h);
     dft3<N1, N2, N3, S, -1>(g, h);
@@ -233,9 +253,10 @@ __device__ void dft3_lds(c128* x /*stride FB_T*/, c128* tmp, int n1, int n2, int
   }
 }
 
-__global__ __launch_bounds__(FB_T) void kfft_fallback_kernel(int mode, int n1, int n2, int n3, const c128* A1,
-                                                             const c128* A2, const c128* W, long w_kstride,
-                                                             c128* O1, c128* O2, long M, double scale) {
+template <typename TG, typename TW, typename TO>
+__global__ __launch_bounds__(FB_T) void kfft_fallback_kernel(int mode, int n1, int n2, int n3, const TG* A1,
+                                                             const TG* A2, const TW* W, long w_kstride, TO* O1,
+                                                             TO* O2, long M, double scale) {
   __shared__ c128 xs[FB_NK * FB_T], ys[FB_NK * FB_T], ts[FB_NK * FB_T];
   const long col = static_cast<long>(blockIdx.x) * FB_T + threadIdx.x;
   const int nk = n1 * n2 * n3;
@@ -245,44 +266,44 @@ __global__ __launch_bounds__(FB_T) void kfft_fallback_kernel(int mode, int n1, i
   if (col >= M) return;
   if (mode == 0) {  // chi0
     for (int k = 0; k < nk; ++k) {
-      x[k * FB_T] = A1[k * M + col];
-      y[k * FB_T] = A2[k * M + col];
+      x[k * FB_T] = cvt<c128>(A1[k * M + col]);
+      y[k * FB_T] = cvt<c128>(A2[k * M + col]);
     }
     dft3_lds(x, t, n1, n2, n3, -1);
     dft3_lds(y, t, n1, n2, n3, -1);
     for (int k = 0; k < nk; ++k) x[k * FB_T] = cscale(cmulconj(x[k * FB_T], y[k * FB_T]), scale);
     dft3_lds(x, t, n1, n2, n3, +1);
-    for (int k = 0; k < nk; ++k) O1[k * M + col] = x[k * FB_T];
+    for (int k = 0; k < nk; ++k) O1[k * M + col] = cvt<TO>(x[k * FB_T]);
   } else {  // Sigma
-    for (int k = 0; k < nk; ++k) y[k * FB_T] = W[k * w_kstride + col];
+    for (int k = 0; k < nk; ++k) y[k * FB_T] = cvt<c128>(W[k * w_kstride + col]);
     dft3_lds(y, t, n1, n2, n3, -1);
     for (int pass = 0; pass < 2; ++pass) {
-      const c128* G = pass == 0 ? A1 : A2;
-      c128* O = pass == 0 ? O1 : O2;
+      const TG* G = pass == 0 ? A1 : A2;
+      TO* O = pass == 0 ? O1 : O2;
       if (G == nullptr) break;
-      for (int k = 0; k < nk; ++k) x[k * FB_T] = G[k * M + col];
+      for (int k = 0; k < nk; ++k) x[k * FB_T] = cvt<c128>(G[k * M + col]);
       dft3_lds(x, t, n1, n2, n3, -1);
       for (int k = 0; k < nk; ++k) x[k * FB_T] = cscale(cmul(x[k * FB_T], y[k * FB_T]), scale);
       dft3_lds(x, t, n1, n2, n3, +1);
-      for (int k = 0; k < nk; ++k) O[k * M + col] = x[k * FB_T];
+      for (int k = 0; k < nk; ++k) O[k * M + col] = cvt<TO>(x[k * FB_T]);
     }
   }
 }
 
-template <int N1, int N2, int N3, int S>
-void launch_pi(hipStream_t s, const c128* Gp, const c128* Gn, c128* Pi, long M, double scale) {
+template <int N1, int N2, int N3, int S, typename RC, typename TG, typename TO>
+void launch_pi(hipStream_t s, const TG* Gp, const TG* Gn, TO* Pi, long M, double scale) {
   const long threads = M * S;
   const unsigned grid = static_cast<unsigned>((threads + KT - 1) / KT);
-  hipLaunchKernelGGL((pi_kfft_kernel<N1, N2, N3, S>), dim3(grid), dim3(KT), 0, s, Gp, Gn, Pi, M, scale);
+  hipLaunchKernelGGL((pi_kfft_kernel<N1, N2, N3, S, RC, TG, TO>), dim3(grid), dim3(KT), 0, s, Gp, Gn, Pi, M, scale);
 }
 
-template <int N1, int N2, int N3, int S>
-void launch_sigma(hipStream_t s, const c128* G1, const c128* G2, const c128* W, long wks, c128* S1, c128* S2,
-                  long M, double scale) {
+template <int N1, int N2, int N3, int S, typename RC, typename TG, typename TW, typename TO>
+void launch_sigma(hipStream_t s, const TG* G1, const TG* G2, const TW* W, long wks, TO* S1, TO* S2, long M,
+                  double scale) {
   const long threads = M * S;
   const unsigned grid = static_cast<unsigned>((threads + KT - 1) / KT);
-  hipLaunchKernelGGL((sigma_kfft_kernel<N1, N2, N3, S>), dim3(grid), dim3(KT), 0, s, G1, G2, W, wks, S1, S2, M,
-                     scale);
+  hipLaunchKernelGGL((sigma_kfft_kernel<N1, N2, N3, S, RC, TG, TW, TO>), dim3(grid), dim3(KT), 0, s, G1, G2, W, wks,
+                     S1, S2, M, scale);
 }
 
 int grid_code(const int* kg) {
@@ -297,42 +318,69 @@ bool kfft_supported(const int* kgrid) {
   return grid_code(kgrid) != 0 || (nk >= 1 && nk <= FB_NK);
 }
 
-void pi_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* Gp, const c128* Gn, c128* Pi,
-             long M, double scale) {
+namespace {
+
+template <typename RC, typename TG, typename TO>
+void pi_kfft_t(hipStream_t s, const int* kgrid, bool force_fallback, const TG* Gp, const TG* Gn, TO* Pi, long M,
+               double scale) {
   const int code = force_fallback ? 0 : grid_code(kgrid);
   switch (code) {
-    case 1: launch_pi<1, 1, 1, 1>(s, Gp, Gn, Pi, M, scale); break;
-    case 2: launch_pi<2, 2, 2, 1>(s, Gp, Gn, Pi, M, scale); break;
-    case 3: launch_pi<3, 3, 3, 1>(s, Gp, Gn, Pi, M, scale); break;
-    case 4: launch_pi<4, 4, 4, 4>(s, Gp, Gn, Pi, M, scale); break;
+    case 1: launch_pi<1, 1, 1, 1, RC>(s, Gp, Gn, Pi, M, scale); break;
+    case 2: launch_pi<2, 2, 2, 1, RC>(s, Gp, Gn, Pi, M, scale); break;
+    case 3: launch_pi<3, 3, 3, 1, RC>(s, Gp, Gn, Pi, M, scale); break;
+    case 4: launch_pi<4, 4, 4, 4, RC>(s, Gp, Gn, Pi, M, scale); break;
     default: {
       const int nk = kgrid[0] * kgrid[1] * kgrid[2];
       THCGW_REQUIRE(nk <= FB_NK, "k-grid not supported by the fused transform kernels");
       const unsigned grid = static_cast<unsigned>((M + FB_T - 1) / FB_T);
-      hipLaunchKernelGGL(kfft_fallback_kernel, dim3(grid), dim3(FB_T), 0, s, 0, kgrid[0], kgrid[1], kgrid[2], Gp,
-                         Gn, static_cast<const c128*>(nullptr), 0L, Pi, static_cast<c128*>(nullptr), M, scale);
+      hipLaunchKernelGGL((kfft_fallback_kernel<TG, TG, TO>), dim3(grid), dim3(FB_T), 0, s, 0, kgrid[0], kgrid[1],
+                         kgrid[2], Gp, Gn, static_cast<const TG*>(nullptr), 0L, Pi, static_cast<TO*>(nullptr), M,
+                         scale);
     }
   }
   THCGW_HIP(hipGetLastError());
 }
 
-void sigma_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G1, const c128* G2,
-                const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale) {
+template <typename RC, typename TG, typename TW, typename TO>
+void sigma_kfft_t(hipStream_t s, const int* kgrid, bool force_fallback, const TG* G1, const TG* G2, const TW* W,
+                  long w_kstride, TO* S1, TO* S2, long M, double scale) {
   const int code = force_fallback ? 0 : grid_code(kgrid);
   switch (code) {
-    case 1: launch_sigma<1, 1, 1, 1>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
-    case 2: launch_sigma<2, 2, 2, 1>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
-    case 3: launch_sigma<3, 3, 3, 1>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
-    case 4: launch_sigma<4, 4, 4, 4>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
+    case 1: launch_sigma<1, 1, 1, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
+    case 2: launch_sigma<2, 2, 2, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
+    case 3: launch_sigma<3, 3, 3, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
+    case 4: launch_sigma<4, 4, 4, 4, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
     default: {
       const int nk = kgrid[0] * kgrid[1] * kgrid[2];
       THCGW_REQUIRE(nk <= FB_NK, "k-grid not supported by the fused transform kernels");
       const unsigned grid = static_cast<unsigned>((M + FB_T - 1) / FB_T);
-      hipLaunchKernelGGL(kfft_fallback_kernel, dim3(grid), dim3(FB_T), 0, s, 1, kgrid[0], kgrid[1], kgrid[2], G1,
-                         G2, W, w_kstride, S1, S2, M, scale);
+      hipLaunchKernelGGL((kfft_fallback_kernel<TG, TW, TO>), dim3(grid), dim3(FB_T), 0, s, 1, kgrid[0], kgrid[1],
+                         kgrid[2], G1, G2, W, w_kstride, S1, S2, M, scale);
     }
   }
   THCGW_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void pi_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* Gp, const c128* Gn, c128* Pi,
+             long M, double scale) {
+  pi_kfft_t<c128>(s, kgrid, force_fallback, Gp, Gn, Pi, M, scale);
+}
+
+void sigma_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G1, const c128* G2,
+                const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale) {
+  sigma_kfft_t<c128>(s, kgrid, force_fallback, G1, G2, W, w_kstride, S1, S2, M, scale);
+}
+
+void pi_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* Gp, const c64* Gn, c128* Pi,
+                 long M, double scale) {
+  pi_kfft_t<c64>(s, kgrid, force_fallback, Gp, Gn, Pi, M, scale);
+}
+
+void sigma_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* G1, const c64* G2,
+                    const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale) {
+  sigma_kfft_t<c64>(s, kgrid, force_fallback, G1, G2, W, w_kstride, S1, S2, M, scale);
 }
 
 }  // namespace thcgw

@@ -126,13 +126,20 @@ __global__ __launch_bounds__(T256) void dyson_finish_kernel(const c128* Z, long 
   }
 }
 
-__global__ __launch_bounds__(T256) void copy_batch_kernel(const c128* src, long sS, int smod, c128* dst, long sD,
-                                                          long per, int b0) {
+template <typename T>
+__global__ __launch_bounds__(T256) void copy_batch_kernel(const T* src, long sS, int smod, T* dst, long sD, long per,
+                                                          int b0) {
   const int b = blockIdx.y;
-  const c128* s = src + static_cast<long>((b0 + b) % smod) * sS;
-  c128* d = dst + static_cast<long>(b) * sD;
+  const T* s = src + static_cast<long>((b0 + b) % smod) * sS;
+  T* d = dst + static_cast<long>(b) * sD;
   for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < per; e += static_cast<long>(gridDim.x) * T256)
     d[e] = s[e];
+}
+
+// Round complex<double> to complex<float> (round to nearest).
+__global__ __launch_bounds__(T256) void to_c64_kernel(const c128* src, c64* dst, long n) {
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < n; e += static_cast<long>(gridDim.x) * T256)
+    dst[e] = c64{static_cast<float>(src[e].x), static_cast<float>(src[e].y)};
 }
 
 // out[b][a] = sum_i conj(C[k][i][a]) T1[b][i][a], k = b % nk: the diagonal of C^H T1.
@@ -226,7 +233,20 @@ void dyson_finish(hipStream_t s, const c128* Z, long sZ, const c128* V, long sV,
 
 void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, long sD, long per, int b0, int batch) {
   const unsigned gx = static_cast<unsigned>(std::min<long>((per + T256 - 1) / T256, 64));
-  hipLaunchKernelGGL(copy_batch_kernel, dim3(gx, batch), dim3(T256), 0, s, src, sS, smod, dst, sD, per, b0);
+  hipLaunchKernelGGL(copy_batch_kernel<c128>, dim3(gx, batch), dim3(T256), 0, s, src, sS, smod, dst, sD, per, b0);
+  THCGW_HIP(hipGetLastError());
+}
+
+void copy_batch(hipStream_t s, const c64* src, long sS, int smod, c64* dst, long sD, long per, int b0, int batch) {
+  const unsigned gx = static_cast<unsigned>(std::min<long>((per + T256 - 1) / T256, 64));
+  hipLaunchKernelGGL(copy_batch_kernel<c64>, dim3(gx, batch), dim3(T256), 0, s, src, sS, smod, dst, sD, per, b0);
+  THCGW_HIP(hipGetLastError());
+}
+
+void to_c64(hipStream_t s, const c128* src, c64* dst, long n) {
+  if (n <= 0) return;
+  const unsigned g = static_cast<unsigned>(std::min<long>((n + T256 - 1) / T256, 1L << 20));
+  hipLaunchKernelGGL(to_c64_kernel, dim3(g), dim3(T256), 0, s, src, dst, n);
   THCGW_HIP(hipGetLastError());
 }
 

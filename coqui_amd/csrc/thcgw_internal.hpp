@@ -13,6 +13,11 @@ namespace thcgw {
 
 // Device complex<double>: (re, im), 16-byte aligned, same layout as std::complex<double>.
 using c128 = double2;
+// Device complex<float>: (re, im), 8-byte aligned, same layout as std::complex<float>.
+using c64 = float2;
+
+// Storage type of a complex operand.
+enum Dt : int { DT_C128 = 0, DT_C64 = 1 };
 
 struct Cplx {
   double re, im;
@@ -39,9 +44,10 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::thcgw::Error(THCGW_ERR_ARG, (msg));                             \
   } while (0)
 
-// Pointer of batch member b: p + (((b + off) / div) % mod) * stride.
+// Pointer of batch member b: p + (((b + off) / div) % mod) * stride, in elements of the
+// operand's storage type.
 struct BatchPtr {
-  const c128* p;
+  const void* p;
   long stride;
   int mod;
   int div;
@@ -51,11 +57,33 @@ struct BatchPtr {
 inline BatchPtr bp(const c128* p, long stride, int mod = 1 << 30, int div = 1, int off = 0) {
   return BatchPtr{p, stride, mod, div, off};
 }
+inline BatchPtr bp(const c64* p, long stride, int mod = 1 << 30, int div = 1, int off = 0) {
+  return BatchPtr{p, stride, mod, div, off};
+}
+
+// Triangular structure flags of zgemm (tile granular, 64 x 64 tiles):
+//   GEMM_TRI_LOWER     only tiles holding some element on or below the diagonal are computed;
+//                      elements above the diagonal inside those tiles are unspecified.
+//   GEMM_TRI_KFROM_ROW the K sum of a tile starts at its first row (op(A) is upper triangular,
+//                      e.g. A = L^H with L lower); the skipped operand entries must be zero.
+//   GEMM_TRI_KFROM_COL the K sum of a tile starts at its first column (op(B) lower triangular).
+constexpr int GEMM_TRI_LOWER = 1;
+constexpr int GEMM_TRI_KFROM_ROW = 2;
+constexpr int GEMM_TRI_KFROM_COL = 4;
+// cgemm only: accumulate the FP32 MFMA partial sums of every 16-long K chunk in FP64.
+constexpr int GEMM_FOLD64 = 8;
 
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I   (row-major, M x N)
 void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
            BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch,
-           Cplx gamma = Cplx{0.0, 0.0});
+           Cplx gamma = Cplx{0.0, 0.0}, int tri = 0);
+
+// The same product in FP32 arithmetic (complex<float> MFMA, FP32 accumulation) with each
+// operand and the result stored as complex<double> or complex<float> (ta, tb, tc); alpha,
+// beta and gamma are applied in double.  Ops N/T/C for A and B in {N, C} x {N, C}.
+void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, Dt ta, long lda,
+           BatchPtr B, Dt tb, long ldb, Cplx beta, void* C, Dt tc, long strideC, long ldc, int batch,
+           Cplx gamma = Cplx{0.0, 0.0}, int tri = 0);
 
 // Number of FP64 GEMM launches and their modelled flops since the last reset (diagnostics).
 struct GemmStats {
@@ -66,6 +94,7 @@ GemmStats gemm_stats();
 void gemm_stats_reset();
 void gemm_profile_enable(bool on);
 void gemm_set_tile_mode(int mode);  // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
+constexpr int GEMM_PROFILE_CLASSES = 13;
 void gemm_profile_read(double* ms, double* flops, long* launches);
 
 // kfft.hip
@@ -74,6 +103,11 @@ void pi_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G
              long M, double scale);
 void sigma_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G1, const c128* G2,
                 const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale);
+// FP32 arithmetic (the generic-grid fallback computes in FP64), complex<float> G and Sigma.
+void pi_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* Gp, const c64* Gn, c128* Pi,
+                 long M, double scale);
+void sigma_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* G1, const c64* G2,
+                    const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale);
 
 // lu.hip
 void getrf_batched(hipStream_t s, c128* A, long sA, int lda, int n, int* ipiv, int* info, int batch);
@@ -84,6 +118,16 @@ void trsm_left_lower(hipStream_t s, const c128* L, long sL, int ldl, c128* B, lo
 void trsm_left_upper_unit(hipStream_t s, const c128* U, long sU, int ldu, c128* B, long sB, int ldb, int n, int ncol,
                           int batch);
 
+// chol.hip (lower Cholesky; only lower triangles of factors are defined)
+void potrf_batched(hipStream_t s, c128* A, long sA, int lda, int n, int* info, int batch);
+void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* T, long sT, int ldt, int m, int nb,
+                           int batch);
+void trsm_right_lower_conj_f32(hipStream_t s, const c128* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
+                               int nb, int batch);
+void zero_upper(hipStream_t s, c128* A, long sA, int n, int batch);
+void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch);
+void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch);
+
 // misc.hip
 void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, double mu, double beta, int nt, int nk,
               int no, c128* CG);
@@ -93,6 +137,8 @@ void log_re_det(hipStream_t s, const c128* A, long sA, int n, const int* ipiv, d
 void dyson_finish(hipStream_t s, const c128* Z, long sZ, const c128* V, long sV, int vmod, const int* ipiv, int n,
                   c128* W, long sW, int b0, int batch);
 void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, long sD, long per, int b0, int batch);
+void copy_batch(hipStream_t s, const c64* src, long sS, int smod, c64* dst, long sD, long per, int b0, int batch);
+void to_c64(hipStream_t s, const c128* src, c64* dst, long n);
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb);
 void trace_prod(hipStream_t s, const c128* A, const c128* B, int nw, int nk, int no, double scale, c128* out);
 

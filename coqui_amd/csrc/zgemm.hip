@@ -1,4 +1,7 @@
-// Batched complex<double> GEMM on the CDNA4 FP64 matrix cores.
+// Batched complex GEMM on the CDNA4 matrix cores, FP64 (v_mfma_f64_16x16x4_f64) or FP32
+// (v_mfma_f32_16x16x4_f32) arithmetic, with operands and result stored as complex<double>
+// or complex<float> independently of the arithmetic (conversion happens on the way
+// through LDS and in the epilogue).
 //
 //   C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I,   op in {N, T, C}
 //
@@ -24,31 +27,57 @@ namespace thcgw {
 namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BK = 8;
 constexpr int NT = 256;
 constexpr int NXCD = 8;
 
 struct Args {
-  int M, N, K, batch, tiles_m, tiles_n;
+  int M, N, K, batch, tiles_m, tiles_n, tri;
   double ar, ai, br, bi, gr, gi;
   BatchPtr A, B;
   long lda, ldb;
-  c128* C;
+  void* C;
   long strideC, ldc;
 };
 
-__device__ __forceinline__ const c128* batch_ptr(const BatchPtr& p, int b) {
-  return p.p + static_cast<long>(((b + p.off) / p.div) % p.mod) * p.stride;
+// Arithmetic traits: complex register type, accumulator, MFMA and accumulator layout.
+template <typename R>
+struct Arith;
+template <>
+struct Arith<double> {
+  using cplx = c128;
+  using acc = f64x4;
+  static constexpr int BK = 8;     // K chunk staged per LDS buffer
+  static constexpr int WAVES = 3;  // waves per SIMD the register budget is held to
+  __device__ static acc mfma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  // accumulator register v of lane l holds row l/16 + 4v, column l%16 (checked on gfx950)
+  __device__ static int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Arith<float> {
+  using cplx = c64;
+  using acc = f32x4;
+  static constexpr int BK = 16;
+  static constexpr int WAVES = 4;
+  __device__ static acc mfma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  // accumulator register v of lane l holds row 4(l/16) + v, column l%16 (checked on gfx950)
+  __device__ static int row(int lk, int v) { return 4 * lk + v; }
+};
+
+template <typename T>
+__device__ __forceinline__ const T* batch_ptr(const BatchPtr& p, int b) {
+  return static_cast<const T*>(p.p) + static_cast<long>(((b + p.off) / p.div) % p.mod) * p.stride;
 }
 
-__device__ __forceinline__ c128 ld_elem(const c128* X, long ld, int r, int c, int R, int Cn, bool conj) {
+template <typename RC, typename T>
+__device__ __forceinline__ RC ld_elem(const T* X, long ld, int r, int c, int R, int Cn, bool conj) {
+  using S = decltype(RC{}.x);
   if (r < R && c < Cn) {
-    c128 v = X[static_cast<long>(r) * ld + c];
-    if (conj) v.y = -v.y;
-    return v;
+    const T v = X[static_cast<long>(r) * ld + c];
+    return RC{static_cast<S>(v.x), static_cast<S>(conj ? -v.y : v.y)};
   }
-  return c128{0.0, 0.0};
+  return RC{0, 0};
 }
 
 // Tile of an operand: BX elements along its free dimension x (m for op(A), n for op(B))
@@ -56,7 +85,7 @@ __device__ __forceinline__ c128 ld_elem(const c128* X, long ld, int r, int c, in
 // (op(A) = A, op(B) = B^T or B^H); otherwise at X[k][x] (op(A) = A^T or A^H, op(B) = B).
 // CONJ conjugates on load.  Pairs of adjacent elements along the contiguous direction
 // are loaded together; each thread loads P = BX*BK/(2*NT) pairs.
-template <bool ROWS, bool CONJ, int BX>
+template <typename RC, typename T, bool ROWS, bool CONJ, int BX, int BK>
 struct TileIO {
   static constexpr int P = BX * BK / (2 * NT);
   __device__ static void coords(int t, int p, int& x, int& k) {
@@ -69,21 +98,21 @@ struct TileIO {
       x = (q % (BX / 2)) * 2;
     }
   }
-  __device__ static void load(const c128* X, long ld, int nx, int nk, int x0, int k0, c128 (&r)[2 * P]) {
+  __device__ static void load(const T* X, long ld, int nx, int nk, int x0, int k0, RC (&r)[2 * P]) {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       int x, k;
       coords(threadIdx.x, p, x, k);
       if (ROWS) {
-        r[2 * p] = ld_elem(X, ld, x0 + x, k0 + k, nx, nk, CONJ);
-        r[2 * p + 1] = ld_elem(X, ld, x0 + x, k0 + k + 1, nx, nk, CONJ);
+        r[2 * p] = ld_elem<RC>(X, ld, x0 + x, k0 + k, nx, nk, CONJ);
+        r[2 * p + 1] = ld_elem<RC>(X, ld, x0 + x, k0 + k + 1, nx, nk, CONJ);
       } else {
-        r[2 * p] = ld_elem(X, ld, k0 + k, x0 + x, nk, nx, CONJ);
-        r[2 * p + 1] = ld_elem(X, ld, k0 + k, x0 + x + 1, nk, nx, CONJ);
+        r[2 * p] = ld_elem<RC>(X, ld, k0 + k, x0 + x, nk, nx, CONJ);
+        r[2 * p + 1] = ld_elem<RC>(X, ld, k0 + k, x0 + x + 1, nk, nx, CONJ);
       }
     }
   }
-  __device__ static void store(c128 (*S)[BX], const c128 (&r)[2 * P]) {
+  __device__ static void store(RC (*S)[BX], const RC (&r)[2 * P]) {
 #pragma unroll
     for (int p = 0; p < P; ++p) {
       int x, k;
@@ -99,13 +128,20 @@ struct TileIO {
   }
 };
 
-template <int OPA, int OPB, int BM, int BN>
-__global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
+// FOLD (FP32 arithmetic only): after every K chunk the FP32 accumulators are added into
+// FP64 accumulators and cleared, so rounding grows with the chunk length (16) instead of
+// with K.
+template <typename R, typename TA, typename TB, typename TC, int OPA, int OPB, int BM, int BN, bool FOLD = false>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : 1)))
+void gemm_kernel(Args p) {
+  using AR = Arith<R>;
+  using RC = typename AR::cplx;
+  constexpr int BK = AR::BK;
   constexpr int TM = BM / 32, TN = BN / 32;  // 16x16 MFMA blocks per wave (waves are 2 x 2)
-  using IOA = TileIO<OPA == OP_N, OPA == OP_C, BM>;  // op(A)(m,k): A[m][k] or A[k][m]
-  using IOB = TileIO<OPB != OP_N, OPB == OP_C, BN>;  // op(B)(k,n): B[k][n] or B[n][k]
-  __shared__ c128 As[2][BK][BM];
-  __shared__ c128 Bs[2][BK][BN];
+  using IOA = TileIO<RC, TA, OPA == OP_N, OPA == OP_C, BM, BK>;  // op(A)(m,k): A[m][k] or A[k][m]
+  using IOB = TileIO<RC, TB, OPB != OP_N, OPB == OP_C, BN, BK>;  // op(B)(k,n): B[k][n] or B[n][k]
+  __shared__ RC As[2][BK][BM];
+  __shared__ RC Bs[2][BK][BN];
 
   // XCD-aware remap of the hardware block id to a logical (batch, tile) index.
   const int nblk = gridDim.x;
@@ -120,52 +156,72 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
   const int m0 = (tile / p.tiles_n) * BM;
   const int n0 = (tile % p.tiles_n) * BN;
 
-  const c128* A = batch_ptr(p.A, b);
-  const c128* B = batch_ptr(p.B, b);
-  c128* C = p.C + static_cast<long>(b) * p.strideC;
+  // Triangular structure (see GEMM_TRI_*): skip tiles strictly above the diagonal and
+  // start the K loop where a triangular operand's band begins.
+  if ((p.tri & GEMM_TRI_LOWER) && m0 + BM - 1 < n0) return;
+  int k_lo = 0;
+  if (p.tri & GEMM_TRI_KFROM_ROW) k_lo = m0;
+  if (p.tri & GEMM_TRI_KFROM_COL) k_lo = n0 > k_lo ? n0 : k_lo;
+  k_lo = (k_lo / BK) * BK;
+
+  const TA* A = batch_ptr<TA>(p.A, b);
+  const TB* B = batch_ptr<TB>(p.B, b);
+  TC* C = static_cast<TC*>(p.C) + static_cast<long>(b) * p.strideC;
 
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  f64x4 cr[TM][TN], ci[TM][TN];
+  typename AR::acc cr[TM][TN], ci[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      cr[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-      ci[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+      cr[i][j] = typename AR::acc{0, 0, 0, 0};
+      ci[i][j] = typename AR::acc{0, 0, 0, 0};
     }
+  constexpr int FT = FOLD ? TM : 1, FN = FOLD ? TN : 1;
+  f64x4 fr[FT][FN], fi[FT][FN];
+  if constexpr (FOLD) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        fr[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        fi[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+      }
+  }
 
   const int nk = (p.K + BK - 1) / BK;
-  c128 ra[2 * IOA::P], rb[2 * IOB::P];
-  if (nk > 0) {
-    IOA::load(A, p.lda, p.M, p.K, m0, 0, ra);
-    IOB::load(B, p.ldb, p.N, p.K, n0, 0, rb);
+  const int kt0 = k_lo / BK;
+  RC ra[2 * IOA::P], rb[2 * IOB::P];
+  if (kt0 < nk) {
+    IOA::load(A, p.lda, p.M, p.K, m0, k_lo, ra);
+    IOB::load(B, p.ldb, p.N, p.K, n0, k_lo, rb);
     IOA::store(As[0], ra);
     IOB::store(Bs[0], rb);
   }
   __syncthreads();
 
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
+  for (int kt = kt0; kt < nk; ++kt) {
+    const int cur = (kt - kt0) & 1;
     if (kt + 1 < nk) {
       IOA::load(A, p.lda, p.M, p.K, m0, (kt + 1) * BK, ra);
       IOB::load(B, p.ldb, p.N, p.K, n0, (kt + 1) * BK, rb);
     }
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
-      double a_r[TM], a_i[TM], a_n[TM], b_r[TN], b_i[TN];
+      R a_r[TM], a_i[TM], a_n[TM], b_r[TN], b_i[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        const c128 v = As[cur][ks * 4 + lk][wm * (BM / 2) + i * 16 + lr];
+        const RC v = As[cur][ks * 4 + lk][wm * (BM / 2) + i * 16 + lr];
         a_r[i] = v.x;
         a_i[i] = v.y;
         a_n[i] = -v.y;
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const c128 v = Bs[cur][ks * 4 + lk][wn * (BN / 2) + j * 16 + lr];
+        const RC v = Bs[cur][ks * 4 + lk][wn * (BN / 2) + j * 16 + lr];
         b_r[j] = v.x;
         b_i[j] = v.y;
       }
@@ -173,16 +229,29 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], cr[i][j], 0, 0, 0);
-          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_i[j], ci[i][j], 0, 0, 0);
+          cr[i][j] = AR::mfma(a_r[i], b_r[j], cr[i][j]);
+          ci[i][j] = AR::mfma(a_r[i], b_i[j], ci[i][j]);
         }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
-          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[i], b_i[j], cr[i][j], 0, 0, 0);
-          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], ci[i][j], 0, 0, 0);
+          cr[i][j] = AR::mfma(a_n[i], b_i[j], cr[i][j]);
+          ci[i][j] = AR::mfma(a_i[i], b_r[j], ci[i][j]);
         }
+    }
+    if constexpr (FOLD) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            fr[i][j][v] += static_cast<double>(cr[i][j][v]);
+            fi[i][j][v] += static_cast<double>(ci[i][j][v]);
+            cr[i][j][v] = 0;
+            ci[i][j][v] = 0;
+          }
     }
     if (kt + 1 < nk) {
       IOA::store(As[cur ^ 1], ra);
@@ -191,8 +260,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
     __syncthreads();
   }
 
-  // Epilogue: for v_mfma_f64_16x16x4_f64 accumulator register v of lane l holds row
-  // l/16 + 4v and column l%16 of its 16 x 16 block (checked on gfx950).
+  // Epilogue (in double whatever the arithmetic).
   const bool beta0 = (p.br == 0.0 && p.bi == 0.0);
   const bool gamma0 = (p.gr == 0.0 && p.gi == 0.0);
 #pragma unroll
@@ -201,23 +269,29 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(Args p) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = m0 + wm * (BM / 2) + i * 16 + lk + 4 * v;
+        const int r = m0 + wm * (BM / 2) + i * 16 + AR::row(lk, v);
         const int c = n0 + wn * (BN / 2) + j * 16 + lr;
         if (r < p.M && c < p.N) {
-          const double xr = cr[i][j][v], xi = ci[i][j][v];
+          double xr = cr[i][j][v], xi = ci[i][j][v];
+          if constexpr (FOLD) {
+            xr += fr[i][j][v];
+            xi += fi[i][j][v];
+          }
           double outr = p.ar * xr - p.ai * xi;
           double outi = p.ar * xi + p.ai * xr;
-          c128* dst = C + static_cast<long>(r) * p.ldc + c;
+          TC* dst = C + static_cast<long>(r) * p.ldc + c;
           if (!beta0) {
-            const c128 old = *dst;
-            outr += p.br * old.x - p.bi * old.y;
-            outi += p.br * old.y + p.bi * old.x;
+            const TC old = *dst;
+            const double o_r = old.x, o_i = old.y;
+            outr += p.br * o_r - p.bi * o_i;
+            outi += p.br * o_i + p.bi * o_r;
           }
           if (!gamma0 && r == c) {
             outr += p.gr;
             outi += p.gi;
           }
-          *dst = c128{outr, outi};
+          using S = decltype(TC{}.x);
+          *dst = TC{static_cast<S>(outr), static_cast<S>(outi)};
         }
       }
 }
@@ -247,7 +321,7 @@ hipEvent_t pool_event() {
   return e;
 }
 
-// Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128.
+// Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128 (complex<double> only).
 template <int OPA, int OPB>
 void launch(hipStream_t s, Args a, int shape) {
   const int bm = shape == 0 ? 64 : 128, bn = shape == 2 ? 128 : 64;
@@ -257,11 +331,25 @@ void launch(hipStream_t s, Args a, int shape) {
   THCGW_REQUIRE(nblocks < (1L << 31), "zgemm: grid too large");
   const dim3 g(static_cast<unsigned>(nblocks)), t(NT);
   if (shape == 0)
-    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, 64, 64>), g, t, 0, s, a);
+    hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 64, 64>), g, t, 0, s, a);
   else if (shape == 1)
-    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, 128, 64>), g, t, 0, s, a);
+    hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 128, 64>), g, t, 0, s, a);
   else
-    hipLaunchKernelGGL((zgemm_kernel<OPA, OPB, 128, 128>), g, t, 0, s, a);
+    hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 128, 128>), g, t, 0, s, a);
+}
+
+template <typename TA, typename TB, typename TC, int OPA, int OPB>
+void launch_f32(hipStream_t s, Args a) {
+  a.tiles_m = (a.M + 63) / 64;
+  a.tiles_n = (a.N + 63) / 64;
+  const long nblocks = static_cast<long>(a.batch) * a.tiles_m * a.tiles_n;
+  THCGW_REQUIRE(nblocks < (1L << 31), "cgemm: grid too large");
+  if (a.tri & GEMM_FOLD64)
+    hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, true>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(NT), 0, s, a);
+  else
+    hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(NT), 0, s, a);
 }
 
 std::atomic<int> g_tile_mode{0};  // 0 = automatic, else shape + 1
@@ -275,16 +363,16 @@ int tile_shape(int, int, int) {
 
 }  // namespace
 
-void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
-           BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch, Cplx gamma) {
-  THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "zgemm: negative size");
-  THCGW_REQUIRE(opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2, "zgemm: bad op");
-  if (M == 0 || N == 0 || batch == 0) return;
+namespace {
+
+Args make_args(int M, int N, int K, Cplx alpha, BatchPtr A, long lda, BatchPtr B, long ldb, Cplx beta, void* C,
+               long strideC, long ldc, int batch, Cplx gamma, int tri) {
   Args a;
   a.M = M;
   a.N = N;
   a.K = K;
   a.batch = batch;
+  a.tri = tri;
   a.ar = alpha.re;
   a.ai = alpha.im;
   a.br = beta.re;
@@ -298,27 +386,31 @@ void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
   a.C = C;
   a.strideC = strideC;
   a.ldc = ldc;
-  const int nb = tile_shape(M, N, K);
-  const double flops = 8.0 * double(M) * N * K * batch;
+  return a;
+}
+
+// Modelled flops; triangular structure (square operands) halves the work per flag.
+double model_flops(int M, int N, int K, int batch, int tri) {
+  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL;
+  double flops = 8.0 * double(M) * N * K * batch;
+  if ((tri & GEMM_TRI_LOWER) && (tri & (GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL))) flops /= 6.0;
+  else if (tri) flops /= 2.0;
+  return flops;
+}
+
+// Launch with optional event timing; cls indexes the profile classes (opA*3 + opB for
+// FP64, 9 + ... for FP32).
+template <class F>
+void timed(hipStream_t s, double flops, int cls, F&& launch_fn) {
   std::unique_lock<std::mutex> lk(g_prof_mu, std::defer_lock);
-  ProfRec rec{nullptr, nullptr, flops, opA * 3 + opB};
+  ProfRec rec{nullptr, nullptr, flops, cls};
   if (g_prof_on) {
     lk.lock();
     rec.a = pool_event();
     rec.b = pool_event();
     THCGW_HIP(hipEventRecord(rec.a, s));
   }
-  switch (opA * 3 + opB) {
-    case 0: launch<OP_N, OP_N>(s, a, nb); break;
-    case 1: launch<OP_N, OP_T>(s, a, nb); break;
-    case 2: launch<OP_N, OP_C>(s, a, nb); break;
-    case 3: launch<OP_T, OP_N>(s, a, nb); break;
-    case 4: launch<OP_T, OP_T>(s, a, nb); break;
-    case 5: launch<OP_T, OP_C>(s, a, nb); break;
-    case 6: launch<OP_C, OP_N>(s, a, nb); break;
-    case 7: launch<OP_C, OP_T>(s, a, nb); break;
-    default: launch<OP_C, OP_C>(s, a, nb); break;
-  }
+  launch_fn();
   THCGW_HIP(hipGetLastError());
   if (g_prof_on) {
     THCGW_HIP(hipEventRecord(rec.b, s));
@@ -326,6 +418,60 @@ void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
   }
   g_launches.fetch_add(1);
   g_flops_milli.fetch_add(static_cast<long long>(flops / 1000.0));
+}
+
+template <typename TA, typename TB, typename TC>
+void dispatch_f32_ops(hipStream_t s, int opA, int opB, const Args& a) {
+  if (opA == OP_N && opB == OP_N) launch_f32<TA, TB, TC, OP_N, OP_N>(s, a);
+  else if (opA == OP_N && opB == OP_C) launch_f32<TA, TB, TC, OP_N, OP_C>(s, a);
+  else if (opA == OP_C && opB == OP_N) launch_f32<TA, TB, TC, OP_C, OP_N>(s, a);
+  else if (opA == OP_C && opB == OP_C) launch_f32<TA, TB, TC, OP_C, OP_C>(s, a);
+  else throw Error(THCGW_ERR_UNSUPPORTED, "cgemm: transpose without conjugation is not built");
+}
+
+template <typename TA, typename TB>
+void dispatch_f32_c(hipStream_t s, int opA, int opB, Dt tc, const Args& a) {
+  if (tc == DT_C128) dispatch_f32_ops<TA, TB, c128>(s, opA, opB, a);
+  else dispatch_f32_ops<TA, TB, c64>(s, opA, opB, a);
+}
+
+}  // namespace
+
+void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
+           BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch, Cplx gamma, int tri) {
+  THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "zgemm: negative size");
+  THCGW_REQUIRE(opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2, "zgemm: bad op");
+  if (M == 0 || N == 0 || batch == 0) return;
+  const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri);
+  const int nb = tile_shape(M, N, K);
+  timed(s, model_flops(M, N, K, batch, tri), opA * 3 + opB, [&] {
+    switch (opA * 3 + opB) {
+      case 0: launch<OP_N, OP_N>(s, a, nb); break;
+      case 1: launch<OP_N, OP_T>(s, a, nb); break;
+      case 2: launch<OP_N, OP_C>(s, a, nb); break;
+      case 3: launch<OP_T, OP_N>(s, a, nb); break;
+      case 4: launch<OP_T, OP_T>(s, a, nb); break;
+      case 5: launch<OP_T, OP_C>(s, a, nb); break;
+      case 6: launch<OP_C, OP_N>(s, a, nb); break;
+      case 7: launch<OP_C, OP_T>(s, a, nb); break;
+      default: launch<OP_C, OP_C>(s, a, nb); break;
+    }
+  });
+}
+
+void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, Dt ta, long lda,
+           BatchPtr B, Dt tb, long ldb, Cplx beta, void* C, Dt tc, long strideC, long ldc, int batch, Cplx gamma,
+           int tri) {
+  THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm: negative size");
+  if (M == 0 || N == 0 || batch == 0) return;
+  const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri);
+  const int cls = 9 + (opA == OP_C ? 2 : 0) + (opB == OP_C ? 1 : 0);
+  timed(s, model_flops(M, N, K, batch, tri), cls, [&] {
+    if (ta == DT_C128 && tb == DT_C128) dispatch_f32_c<c128, c128>(s, opA, opB, tc, a);
+    else if (ta == DT_C128) dispatch_f32_c<c128, c64>(s, opA, opB, tc, a);
+    else if (tb == DT_C128) dispatch_f32_c<c64, c128>(s, opA, opB, tc, a);
+    else dispatch_f32_c<c64, c64>(s, opA, opB, tc, a);
+  });
 }
 
 GemmStats gemm_stats() { return GemmStats{g_launches.load(), 1000.0 * double(g_flops_milli.load())}; }
@@ -341,11 +487,12 @@ void gemm_profile_enable(bool on) {
   g_prof_on = on;
 }
 
-// Per class (opA*3 + opB, 9 classes): summed launch durations (ms), modelled flops and
-// launch counts of the recorded launches; the records are then dropped.
+// Per class (FP64: opA*3 + opB, 0..8; FP32: 9 + 2*(opA == C) + (opB == C), 9..12): summed
+// launch durations (ms), modelled flops and launch counts of the recorded launches; the
+// records are then dropped.
 void gemm_profile_read(double* ms, double* flops, long* launches) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
-  for (int c = 0; c < 9; ++c) {
+  for (int c = 0; c < GEMM_PROFILE_CLASSES; ++c) {
     ms[c] = 0.0;
     flops[c] = 0.0;
     launches[c] = 0;

@@ -166,13 +166,14 @@ class ThcGW:
 
     def profile_read(self):
         """{class: (ms, flops, launches)} of the GEMM launches recorded since the last read
-        (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B)."""
-        ms = (ctypes.c_double * 9)()
-        fl = (ctypes.c_double * 9)()
-        n = (ctypes.c_long * 9)()
+        (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B) for FP64 products and
+        "NN32", "NC32", "CN32", "CC32" for FP32 products."""
+        ms = (ctypes.c_double * 13)()
+        fl = (ctypes.c_double * 13)()
+        n = (ctypes.c_long * 13)()
         self._check(self.lib.thcgw_profile_read(self.h, ms, fl, n))
-        names = [a + b for a in "NTC" for b in "NTC"]
-        return {names[c]: (ms[c], fl[c], n[c]) for c in range(9) if n[c]}
+        names = [a + b for a in "NTC" for b in "NTC"] + ["NN32", "NC32", "CN32", "CC32"]
+        return {names[c]: (ms[c], fl[c], n[c]) for c in range(13) if n[c]}
 
     def workspace_bytes(self):
         return int(self.lib.thcgw_workspace_bytes(self.h))

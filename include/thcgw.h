@@ -165,11 +165,21 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_t
  *   flag 2: tau pairs per projection batch (0 = automatic);
  *   flag 3: matrices per Dyson batch (0 = automatic);
  *   flag 4: record HIP events around every complex-FP64 GEMM launch (process-wide);
- *   flag 5: GEMM tile shape (process-wide): 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128.
- * thcgw_profile_read fills, for each of the 9 GEMM classes c = opA*3 + opB (op 0 = N,
- * 1 = T, 2 = C), the summed kernel time ms[c] of the launches recorded since the last read,
- * their modelled flops[c] (8 M N K per complex product) and launches[c], then drops the
- * records.  It waits for the recorded launches to finish. */
+ *   flag 5: GEMM tile shape (process-wide): 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128;
+ *   flag 6: Dyson algorithm: 0 = LU with partial pivoting (the reference's algorithm,
+ *           default), 1 = Hermitian form V = L L^H, M = I - L^H Pi L = L_M L_M^H (two
+ *           Cholesky factorisations, triangular products; valid for Hermitian positive
+ *           definite V and Hermitian negative semidefinite Pi, with per-chunk fallback to LU);
+ *   flag 7: precision: 0 = FP64 throughout (default); 1 = mixed: the projections X G X^H and
+ *           X^H Sigma X, the k-grid transforms with the Hadamard products, and the products
+ *           of the Hermitian Dyson form run in FP32 (complex<float> intermediates), while
+ *           the tau <-> iW transforms, the Cholesky factorisations and ln det stay FP64.
+ *           Inputs and outputs keep their complex<double> layouts.
+ * thcgw_profile_read fills, for each of the 13 GEMM classes, the summed kernel time ms[c]
+ * of the launches recorded since the last read, their modelled flops[c] (8 M N K per
+ * complex product, reduced for triangular products) and launches[c], then drops the
+ * records.  Classes 0..8 are the FP64 products c = opA*3 + opB (op 0 = N, 1 = T, 2 = C);
+ * 9..12 the FP32 products NN, NC, CN, CC.  It waits for the recorded launches to finish. */
 long thcgw_workspace_bytes(const thcgw_ctx* ctx);
 int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value);
 int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, long* launches);
