@@ -9,7 +9,8 @@ Forms (--forms, the first is the headline value):
   plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm);
   hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
-             transforms, Cholesky factorisations and ln det.
+             transforms, Cholesky factorisations and ln det;
+  mixed_bf16 as mixed, with the FP32 products done as split-BF16 MFMA products.
 Every further form is reported with its speedup over the first and its errors against it
 (W at q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
 
@@ -44,7 +45,7 @@ CONFIGS = {
 
 
 # form: (flag 6 Dyson algorithm, flag 7 precision)
-FORMS = {"plain": (0, 0), "hermitian": (1, 0), "mixed": (1, 1)}
+FORMS = {"plain": (0, 0), "hermitian": (1, 0), "mixed": (1, 1), "mixed_bf16": (1, 2)}
 HA_PER_MEV = 1.0 / 27211.386245988
 
 
@@ -215,8 +216,8 @@ def main():
     ms_step, wall, ec, prof = run_form(forms[0], args.steps, args.warmup)
     log("bench: %s %.1f ms per step" % (forms[0], ms_step))
     g_ms, g_fl, g_n = prof.get("NN", (0.0, 0.0, 0))  # Dyson, LU and triangular-solve GEMMs
-    a_ms = sum(v[0] for k, v in prof.items() if not k.endswith("32"))
-    a_fl = sum(v[1] for k, v in prof.items() if not k.endswith("32"))
+    a_ms = sum(v[0] for k, v in prof.items() if len(k) == 2)
+    a_fl = sum(v[1] for k, v in prof.items() if len(k) == 2)
     achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     gemm_classes = {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
                         "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}

@@ -44,7 +44,9 @@ struct thcgw_ctx {
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
   int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky)
-  int precision = 0;    // 0 = FP64 throughout, 1 = mixed (FP32 products, FP64 factorisations and IR)
+  int precision = 0;    // 0 = FP64 throughout, 1 = mixed (FP32 products, FP64 factorisations and IR),
+                        // 2 = mixed with split-BF16 products in place of FP32 MFMA
+  int pf() const { return precision == 2 ? thcgw::GEMM_SPLIT_BF16 : 0; }  // cgemm precision flags
 
   template <class T>
   T* buf(const std::string& name, size_t count) {
@@ -152,9 +154,10 @@ void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* 
   THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
                thcgw::bp(Oin, static_cast<long>(no) * no), DT_C128, no, kZero, Abuf, DT_C64, sX, no,
-               static_cast<int>(nbk));
+               static_cast<int>(nbk), kZero, ctx->pf());
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, nmu, nmu, no, kOne, thcgw::bp(Abuf, sX), DT_C64, no,
-               thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C64, ctx->M, nmu, static_cast<int>(nbk));
+               thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C64, ctx->M, nmu, static_cast<int>(nbk), kZero,
+               ctx->pf());
 }
 
 // O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>.
@@ -163,10 +166,11 @@ void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c64* O
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
   thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
-               thcgw::bp(Oin, ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu, static_cast<int>(nbk));
+               thcgw::bp(Oin, ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu, static_cast<int>(nbk), kZero,
+               ctx->pf());
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), DT_C64, nmu,
                thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C128, static_cast<long>(no) * no, no,
-               static_cast<int>(nbk));
+               static_cast<int>(nbk), kZero, ctx->pf());
 }
 
 int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
@@ -198,7 +202,7 @@ void run_pi_f32(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStrea
 }
 
 void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
-  if (ctx->precision == 1) return run_pi_f32(ctx, G, X, Pi, s);
+  if (ctx->precision != 0) return run_pi_f32(ctx, G, X, Pi, s);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
   const long gs = static_cast<long>(nk) * no * no;  // one tau slice of G
@@ -324,19 +328,20 @@ bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv
   c128* P = PiW + static_cast<long>(b0) * M;
   // M feeds ln det: both products keep FP64 sums across K chunks
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(P, M), DT_C128, n, thcgw::bp(Lv64, M, nq, 1, b0),
-               DT_C64, n, kZero, Z, DT_C64, M, n, cb, kZero, thcgw::GEMM_TRI_KFROM_COL | thcgw::GEMM_FOLD64);
+               DT_C64, n, kZero, Z, DT_C64, M, n, cb, kZero,
+               thcgw::GEMM_TRI_KFROM_COL | thcgw::GEMM_FOLD64 | ctx->pf());
   thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(Lv64, M, nq, 1, b0), DT_C64, n,
                thcgw::bp(Z, M), DT_C64, n, kZero, Mb, DT_C128, M, n, cb, kOne,
-               thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW | thcgw::GEMM_FOLD64);
+               thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW | thcgw::GEMM_FOLD64 | ctx->pf());
   if (d_trace) thcgw::one_minus_trace(s, Mb, M, n, d_trace + b0, cb);
   thcgw::potrf_batched(s, Mb, M, n, n, info, cb);
   if (any_nonzero(info, cb, s)) return false;
   if (d_ldet) thcgw::chol_logdet(s, Mb, M, n, d_ldet + b0, cb);
   if (solve) {
     thcgw::copy_batch(s, Lv64, M, nq, Z, M, M, b0, cb);
-    thcgw::trsm_right_lower_conj_f32(s, Mb, M, n, Z, M, n, n, n, cb);
+    thcgw::trsm_right_lower_conj_f32(s, Mb, M, n, Z, M, n, n, n, cb, ctx->pf());
     thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), DT_C64, n, thcgw::bp(Z, M), DT_C64, n,
-                 kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER);
+                 kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER | ctx->pf());
     thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
   }
   return true;
@@ -353,7 +358,7 @@ void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* 
   const long M = ctx->M;
   const int total = ctx->nw_half * nq;
   const int CH = choose_dyson_chunk(ctx, total);
-  const bool mixed = ctx->precision == 1;
+  const bool mixed = ctx->precision != 0;
   const bool want_hpd = mixed || ctx->dyson_algo == 1;
   c128* LU = ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M);
   int* info_all = ctx->buf<int>("dy_info", static_cast<size_t>(total));
@@ -453,7 +458,7 @@ void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X
 }
 
 void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm, hipStream_t s) {
-  if (ctx->precision == 1) return run_sigma_f32(ctx, G, Wqt, X, Sig, herm, s);
+  if (ctx->precision != 0) return run_sigma_f32(ctx, G, Wqt, X, Sig, herm, s);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
   const long gs = static_cast<long>(nk) * no * no;
@@ -575,7 +580,7 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
       THCGW_REQUIRE(value == 0 || value == 1, "flag 6 (Dyson algorithm) takes 0 (LU) or 1 (Hermitian)");
       ctx->dyson_algo = static_cast<int>(value);
     } else if (flag == 7) {
-      THCGW_REQUIRE(value == 0 || value == 1, "flag 7 (precision) takes 0 (FP64) or 1 (mixed)");
+      THCGW_REQUIRE(value >= 0 && value <= 2, "flag 7 (precision) takes 0 (FP64), 1 (mixed) or 2 (mixed, split BF16)");
       ctx->precision = static_cast<int>(value);
     }
     else THCGW_REQUIRE(false, "unknown flag");
@@ -749,6 +754,32 @@ int thcgw_solve_qp(thcgw_ctx* ctx, const double* Sigma, const double* Cmo, const
     THCGW_HIP(hipStreamSynchronize(s));
     thcgw::host::solve_qp_diag(ctx->tab, nk, no, hd.data(), hd.data() + static_cast<size_t>(nt) * nk * no, E0, mu,
                                nfit, eta, tol, E_out);
+  });
+}
+
+int thcgw_hf(thcgw_ctx* ctx, const double* Dm, const double* X, const double* V, int hartree, int exchange, double* F,
+             void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(Dm && X && V && F, "thcgw_hf: null pointer");
+    hipStream_t s = S(stream);
+    const int nk = ctx->nk, nmu = ctx->d.nmu, no = ctx->d.no;
+    const long M = ctx->M;
+    c128* DP = ctx->buf<c128>("hf_dp", static_cast<size_t>(nk) * M);
+    c128* FP = ctx->buf<c128>("hf_fp", static_cast<size_t>(nk) * M);
+    c128* Ab = ctx->buf<c128>("hf_ab", static_cast<size_t>(nk) * nmu * no);
+    project_p2a(ctx, s, C(X), C(Dm), DP, nk, Ab);  // thc_hf.icc:162-163
+    // exchange: F^P(k) = sum_R e^{ikR} [-Dm(R) o U(R)] = -(1/N_k) sum_q Dm^P(k-q) o V(q)  (thc_hf.icc:211-232)
+    if (exchange)
+      thcgw::sigma_kfft(s, ctx->d.kgrid, ctx->force_fallback, DP, nullptr, C(V), M, FP, nullptr, M,
+                        -1.0 / (double(nk) * nk));
+    else
+      thcgw::zero_c128(s, FP, static_cast<long>(nk) * M);
+    if (hartree) {  // thc_hf.icc:182-205
+      c128* d = ctx->buf<c128>("hf_d", static_cast<size_t>(nmu));
+      thcgw::hf_hartree(s, DP, C(V), d, FP, nk, nmu);
+    }
+    project_a2p(ctx, s, C(X), FP, Cm(F), nk, Ab);  // thc_hf.icc:236-237
   });
 }
 

@@ -136,6 +136,43 @@ __global__ __launch_bounds__(T256) void copy_batch_kernel(const T* src, long sS,
     d[e] = s[e];
 }
 
+// d[Q] = (2 / nk) sum_k DP[k][Q][Q]: twice the R = 0 diagonal of the projected density.
+__global__ __launch_bounds__(T256) void hf_diag_kernel(const c128* DP, int nk, int nmu, c128* d) {
+  const int Q = blockIdx.x * T256 + threadIdx.x;
+  if (Q >= nmu) return;
+  const long M = static_cast<long>(nmu) * nmu;
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < nk; ++k) {
+    const c128 v = DP[k * M + static_cast<long>(Q) * nmu + Q];
+    re += v.x;
+    im += v.y;
+  }
+  d[Q] = c128{2.0 * re / nk, 2.0 * im / nk};
+}
+
+// J_P = sum_Q V0[P][Q] d[Q];  FP[k][P][P] += J_P for every k.
+__global__ __launch_bounds__(T256) void hf_hartree_kernel(const c128* V0, const c128* d, c128* FP, int nk, int nmu) {
+  const int P = blockIdx.x * T256 + threadIdx.x;
+  if (P >= nmu) return;
+  const long M = static_cast<long>(nmu) * nmu;
+  double re = 0.0, im = 0.0;
+  const c128* row = V0 + static_cast<long>(P) * nmu;
+  for (int Q = 0; Q < nmu; ++Q) {
+    const c128 v = row[Q], x = d[Q];
+    re += v.x * x.x - v.y * x.y;
+    im += v.x * x.y + v.y * x.x;
+  }
+  for (int k = 0; k < nk; ++k) {
+    c128* f = FP + k * M + static_cast<long>(P) * nmu + P;
+    *f = c128{f->x + re, f->y + im};
+  }
+}
+
+__global__ __launch_bounds__(T256) void zero_c128_kernel(c128* p, long n) {
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < n; e += static_cast<long>(gridDim.x) * T256)
+    p[e] = c128{0.0, 0.0};
+}
+
 // Round complex<double> to complex<float> (round to nearest).
 __global__ __launch_bounds__(T256) void to_c64_kernel(const c128* src, c64* dst, long n) {
   for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < n; e += static_cast<long>(gridDim.x) * T256)
@@ -240,6 +277,21 @@ void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, lo
 void copy_batch(hipStream_t s, const c64* src, long sS, int smod, c64* dst, long sD, long per, int b0, int batch) {
   const unsigned gx = static_cast<unsigned>(std::min<long>((per + T256 - 1) / T256, 64));
   hipLaunchKernelGGL(copy_batch_kernel<c64>, dim3(gx, batch), dim3(T256), 0, s, src, sS, smod, dst, sD, per, b0);
+  THCGW_HIP(hipGetLastError());
+}
+
+void hf_hartree(hipStream_t s, const c128* DP, const c128* V0, c128* d, c128* FP, int nk, int nmu) {
+  const unsigned g = static_cast<unsigned>((nmu + T256 - 1) / T256);
+  hipLaunchKernelGGL(hf_diag_kernel, dim3(g), dim3(T256), 0, s, DP, nk, nmu, d);
+  THCGW_HIP(hipGetLastError());
+  hipLaunchKernelGGL(hf_hartree_kernel, dim3(g), dim3(T256), 0, s, V0, static_cast<const c128*>(d), FP, nk, nmu);
+  THCGW_HIP(hipGetLastError());
+}
+
+void zero_c128(hipStream_t s, c128* p, long n) {
+  if (n <= 0) return;
+  const unsigned g = static_cast<unsigned>(std::min<long>((n + T256 - 1) / T256, 1L << 20));
+  hipLaunchKernelGGL(zero_c128_kernel, dim3(g), dim3(T256), 0, s, p, n);
   THCGW_HIP(hipGetLastError());
 }
 

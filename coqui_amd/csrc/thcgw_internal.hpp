@@ -70,8 +70,11 @@ inline BatchPtr bp(const c64* p, long stride, int mod = 1 << 30, int div = 1, in
 constexpr int GEMM_TRI_LOWER = 1;
 constexpr int GEMM_TRI_KFROM_ROW = 2;
 constexpr int GEMM_TRI_KFROM_COL = 4;
-// cgemm only: accumulate the FP32 MFMA partial sums of every 16-long K chunk in FP64.
+// cgemm only: accumulate the FP32 MFMA partial sums of every K chunk in FP64.
 constexpr int GEMM_FOLD64 = 8;
+// cgemm only: split-BF16 products (three bf16 parts per value, six part products, FP32
+// accumulation) instead of FP32 MFMA.
+constexpr int GEMM_SPLIT_BF16 = 16;
 
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I   (row-major, M x N)
 void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
@@ -94,7 +97,7 @@ GemmStats gemm_stats();
 void gemm_stats_reset();
 void gemm_profile_enable(bool on);
 void gemm_set_tile_mode(int mode);  // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
-constexpr int GEMM_PROFILE_CLASSES = 13;
+constexpr int GEMM_PROFILE_CLASSES = 17;
 void gemm_profile_read(double* ms, double* flops, long* launches);
 
 // kfft.hip
@@ -123,7 +126,7 @@ void potrf_batched(hipStream_t s, c128* A, long sA, int lda, int n, int* info, i
 void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* T, long sT, int ldt, int m, int nb,
                            int batch);
 void trsm_right_lower_conj_f32(hipStream_t s, const c128* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
-                               int nb, int batch);
+                               int nb, int batch, int gemm_flags = 0);
 void zero_upper(hipStream_t s, c128* A, long sA, int n, int batch);
 void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch);
 void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch);
@@ -139,6 +142,9 @@ void dyson_finish(hipStream_t s, const c128* Z, long sZ, const c128* V, long sV,
 void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, long sD, long per, int b0, int batch);
 void copy_batch(hipStream_t s, const c64* src, long sS, int smod, c64* dst, long sD, long per, int b0, int batch);
 void to_c64(hipStream_t s, const c128* src, c64* dst, long n);
+void zero_c128(hipStream_t s, c128* p, long n);
+// THC Hartree term: d = twice the R = 0 diagonal of DP (nk, nmu, nmu); FP_kPP += (V0 d)_P.
+void hf_hartree(hipStream_t s, const c128* DP, const c128* V0, c128* d, c128* FP, int nk, int nmu);
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb);
 void trace_prod(hipStream_t s, const c128* A, const c128* B, int nw, int nk, int no, double scale, c128* out);
 

@@ -296,6 +296,265 @@ void gemm_kernel(Args p) {
       }
 }
 
+// ---------------------------------------------------------------------------------
+// Split-BF16 product with FP32 accumulation (v_mfma_f32_16x16x32_bf16).
+//
+// Every real operand value x is split into three bf16 parts x = x0 + x1 + x2 (x0 = bf16(x),
+// x1 = bf16(x - x0), x2 = bf16(x - x0 - x1); the differences are exact in FP32), which
+// carry 24 significant bits between them.  A real product keeps the six part products
+// with s + t <= 2, a_s b_t, so the dropped terms are O(2^-24) relative: the result has
+// FP32-class accuracy while the MFMA work is six bf16 products per real product, each
+// 8 times the K depth of the FP32 instruction.
+//
+// Tile 64 x 64 (complex), four waves as 2 x 2, K chunks of 32 complex values.  The LDS
+// holds, per operand, part and real/imaginary component, a 64 x 32 bf16 panel with the K
+// index contiguous (rows padded to 40 values so that the 16-byte fragment reads of 16
+// lanes fall into distinct banks).  Lane l of an MFMA reads row l%16, k = 8(l/16) .. +7.
+// The next chunk's global loads are issued before the current chunk's MFMAs.
+// ---------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int XBK = 32;  // K chunk, complex elements
+constexpr int XLD = 40;  // LDS row stride, bf16 values
+
+template <typename T>
+__device__ __forceinline__ float2 ld_f2(const T* X, long ld, int r, int c, int R, int Cn, bool conj) {
+  if (r < R && c < Cn) {
+    const T v = X[static_cast<long>(r) * ld + c];
+    return float2{static_cast<float>(v.x), static_cast<float>(conj ? -v.y : v.y)};
+  }
+  return float2{0.0f, 0.0f};
+}
+
+__device__ __forceinline__ void split3(float x, __bf16& h0, __bf16& h1, __bf16& h2) {
+  h0 = static_cast<__bf16>(x);
+  const float r1 = x - static_cast<float>(h0);
+  h1 = static_cast<__bf16>(r1);
+  const float r2 = r1 - static_cast<float>(h1);
+  h2 = static_cast<__bf16>(r2);
+}
+
+// One operand's 64 x 32 chunk: 8 complex values per thread.  ROWS: element (x, k) at
+// X[x][k] (8 consecutive k per thread); otherwise at X[k][x] (4 consecutive x for each
+// of two consecutive k per thread).
+template <typename T, bool ROWS, bool CONJ>
+struct SplitIO {
+  __device__ static void load(const T* X, long ld, int nx, int nk, int x0, int k0, float2 (&v)[8]) {
+    const int t = threadIdx.x;
+    if (ROWS) {
+      const int x = t >> 2, k = (t & 3) * 8;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = ld_f2(X, ld, x0 + x, k0 + k + e, nx, nk, CONJ);
+    } else {
+      const int k = (t >> 4) * 2, x = (t & 15) * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[e] = ld_f2(X, ld, k0 + k, x0 + x + e, nk, nx, CONJ);
+        v[4 + e] = ld_f2(X, ld, k0 + k + 1, x0 + x + e, nk, nx, CONJ);
+      }
+    }
+  }
+  // S[part][re/im][x][k]
+  __device__ static void store(__bf16 (*S)[2][64][XLD], const float2 (&v)[8]) {
+    const int t = threadIdx.x;
+    if (ROWS) {
+      const int x = t >> 2, k = (t & 3) * 8;
+      bf16x8 p[3][2];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 a0, a1, a2, b0, b1, b2;
+        split3(v[e].x, a0, a1, a2);
+        split3(v[e].y, b0, b1, b2);
+        p[0][0][e] = a0;
+        p[1][0][e] = a1;
+        p[2][0][e] = a2;
+        p[0][1][e] = b0;
+        p[1][1][e] = b1;
+        p[2][1][e] = b2;
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) *reinterpret_cast<bf16x8*>(&S[q][c][x][k]) = p[q][c];
+    } else {
+      const int k = (t >> 4) * 2, x = (t & 15) * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        __bf16 a[2][3], b[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          split3(v[4 * h + e].x, a[h][0], a[h][1], a[h][2]);
+          split3(v[4 * h + e].y, b[h][0], b[h][1], b[h][2]);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          S[q][0][x + e][k] = a[0][q];
+          S[q][0][x + e][k + 1] = a[1][q];
+          S[q][1][x + e][k] = b[0][q];
+          S[q][1][x + e][k + 1] = b[1][q];
+        }
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ bf16x8 bneg(bf16x8 v) {
+  u32x4 u = __builtin_bit_cast(u32x4, v);
+  u ^= u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+template <typename TA, typename TB, typename TC, int OPA, int OPB, bool FOLD>
+__global__ __launch_bounds__(NT) void gemm_bf16x6_kernel(Args p) {
+  using IOA = SplitIO<TA, OPA == OP_N, OPA == OP_C>;  // op(A)(m,k)
+  using IOB = SplitIO<TB, OPB != OP_N, OPB == OP_C>;  // op(B)(k,n) stored as (n,k)
+  __shared__ __attribute__((aligned(16))) __bf16 As[3][2][64][XLD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][2][64][XLD];
+
+  const int nblk = gridDim.x;
+  const int hw = blockIdx.x;
+  const int xcd = hw % NXCD, loc = hw / NXCD;
+  const int base = nblk / NXCD, rem = nblk % NXCD;
+  const int logical = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const int tiles = p.tiles_m * p.tiles_n;
+  const int b = logical / tiles;
+  const int tile = logical % tiles;
+  const int m0 = (tile / p.tiles_n) * 64;
+  const int n0 = (tile % p.tiles_n) * 64;
+  if ((p.tri & GEMM_TRI_LOWER) && m0 + 63 < n0) return;
+  int k_lo = 0;
+  if (p.tri & GEMM_TRI_KFROM_ROW) k_lo = m0;
+  if (p.tri & GEMM_TRI_KFROM_COL) k_lo = n0 > k_lo ? n0 : k_lo;
+  k_lo = (k_lo / XBK) * XBK;
+
+  const TA* A = batch_ptr<TA>(p.A, b);
+  const TB* B = batch_ptr<TB>(p.B, b);
+  TC* C = static_cast<TC*>(p.C) + static_cast<long>(b) * p.strideC;
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  f32x4 cr[2][2], ci[2][2];
+  f64x4 fr[FOLD ? 2 : 1][FOLD ? 2 : 1], fi[FOLD ? 2 : 1][FOLD ? 2 : 1];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cr[i][j] = f32x4{0, 0, 0, 0};
+      ci[i][j] = f32x4{0, 0, 0, 0};
+      if constexpr (FOLD) {
+        fr[i][j] = f64x4{0, 0, 0, 0};
+        fi[i][j] = f64x4{0, 0, 0, 0};
+      }
+    }
+
+  const int nk = (p.K + XBK - 1) / XBK;
+  const int kt0 = k_lo / XBK;
+  float2 va[8], vb[8];
+  if (kt0 < nk) {
+    IOA::load(A, p.lda, p.M, p.K, m0, k_lo, va);
+    IOB::load(B, p.ldb, p.N, p.K, n0, k_lo, vb);
+  }
+  for (int kt = kt0; kt < nk; ++kt) {
+    IOA::store(As, va);
+    IOB::store(Bs, vb);
+    __syncthreads();
+    if (kt + 1 < nk) {
+      IOA::load(A, p.lda, p.M, p.K, m0, (kt + 1) * XBK, va);
+      IOB::load(B, p.ldb, p.N, p.K, n0, (kt + 1) * XBK, vb);
+    }
+    bf16x8 fa[2][3][3], fb[2][3][2];  // [block][part][re, im, -im] and [block][part][re, im]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int row = wm * 32 + i * 16 + lr;
+        fa[i][q][0] = *reinterpret_cast<const bf16x8*>(&As[q][0][row][lk * 8]);
+        fa[i][q][1] = *reinterpret_cast<const bf16x8*>(&As[q][1][row][lk * 8]);
+        fa[i][q][2] = bneg(fa[i][q][1]);
+      }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int col = wn * 32 + j * 16 + lr;
+        fb[j][q][0] = *reinterpret_cast<const bf16x8*>(&Bs[q][0][col][lk * 8]);
+        fb[j][q][1] = *reinterpret_cast<const bf16x8*>(&Bs[q][1][col][lk * 8]);
+      }
+    // part pairs (s, t) with s + t <= 2, smallest first
+#pragma unroll
+    for (int pp = 5; pp >= 0; --pp) {
+      const int s_ = pp == 0 ? 0 : pp == 1 ? 0 : pp == 2 ? 1 : pp == 3 ? 0 : pp == 4 ? 1 : 2;
+      const int t_ = pp == 0 ? 0 : pp == 1 ? 1 : pp == 2 ? 0 : pp == 3 ? 2 : pp == 4 ? 1 : 0;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][s_][0], fb[j][t_][0], cr[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][s_][0], fb[j][t_][1], ci[i][j], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][s_][2], fb[j][t_][1], cr[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i][s_][1], fb[j][t_][0], ci[i][j], 0, 0, 0);
+        }
+    }
+    if constexpr (FOLD) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            fr[i][j][v] += static_cast<double>(cr[i][j][v]);
+            fi[i][j][v] += static_cast<double>(ci[i][j][v]);
+            cr[i][j][v] = 0.0f;
+            ci[i][j][v] = 0.0f;
+          }
+    }
+    __syncthreads();
+  }
+
+  const bool beta0 = (p.br == 0.0 && p.bi == 0.0);
+  const bool gamma0 = (p.gr == 0.0 && p.gi == 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // accumulator register v of lane l holds row 4(l/16) + v, column l%16 (checked on gfx950)
+        const int r = m0 + wm * 32 + i * 16 + 4 * lk + v;
+        const int c = n0 + wn * 32 + j * 16 + lr;
+        if (r < p.M && c < p.N) {
+          double xr = cr[i][j][v], xi = ci[i][j][v];
+          if constexpr (FOLD) {
+            xr += fr[i][j][v];
+            xi += fi[i][j][v];
+          }
+          double outr = p.ar * xr - p.ai * xi;
+          double outi = p.ar * xi + p.ai * xr;
+          TC* dst = C + static_cast<long>(r) * p.ldc + c;
+          if (!beta0) {
+            const TC old = *dst;
+            const double o_r = old.x, o_i = old.y;
+            outr += p.br * o_r - p.bi * o_i;
+            outi += p.br * o_i + p.bi * o_r;
+          }
+          if (!gamma0 && r == c) {
+            outr += p.gr;
+            outi += p.gi;
+          }
+          using S = decltype(TC{}.x);
+          *dst = TC{static_cast<S>(outr), static_cast<S>(outi)};
+        }
+      }
+}
+
 std::atomic<long> g_launches{0};
 std::atomic<long long> g_flops_milli{0};
 
@@ -344,7 +603,14 @@ void launch_f32(hipStream_t s, Args a) {
   a.tiles_n = (a.N + 63) / 64;
   const long nblocks = static_cast<long>(a.batch) * a.tiles_m * a.tiles_n;
   THCGW_REQUIRE(nblocks < (1L << 31), "cgemm: grid too large");
-  if (a.tri & GEMM_FOLD64)
+  if (a.tri & GEMM_SPLIT_BF16) {
+    if (a.tri & GEMM_FOLD64)
+      hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, OPA, OPB, true>), dim3(static_cast<unsigned>(nblocks)),
+                         dim3(NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, OPA, OPB, false>), dim3(static_cast<unsigned>(nblocks)),
+                         dim3(NT), 0, s, a);
+  } else if (a.tri & GEMM_FOLD64)
     hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, true>), dim3(static_cast<unsigned>(nblocks)),
                        dim3(NT), 0, s, a);
   else
@@ -391,7 +657,7 @@ Args make_args(int M, int N, int K, Cplx alpha, BatchPtr A, long lda, BatchPtr B
 
 // Modelled flops; triangular structure (square operands) halves the work per flag.
 double model_flops(int M, int N, int K, int batch, int tri) {
-  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL;
+  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL;  // the precision flags do not change the work
   double flops = 8.0 * double(M) * N * K * batch;
   if ((tri & GEMM_TRI_LOWER) && (tri & (GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL))) flops /= 6.0;
   else if (tri) flops /= 2.0;
@@ -465,7 +731,7 @@ void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
   THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm: negative size");
   if (M == 0 || N == 0 || batch == 0) return;
   const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri);
-  const int cls = 9 + (opA == OP_C ? 2 : 0) + (opB == OP_C ? 1 : 0);
+  const int cls = ((tri & GEMM_SPLIT_BF16) ? 13 : 9) + (opA == OP_C ? 2 : 0) + (opB == OP_C ? 1 : 0);
   timed(s, model_flops(M, N, K, batch, tri), cls, [&] {
     if (ta == DT_C128 && tb == DT_C128) dispatch_f32_c<c128, c128>(s, opA, opB, tc, a);
     else if (ta == DT_C128) dispatch_f32_c<c128, c64>(s, opA, opB, tc, a);
@@ -487,7 +753,8 @@ void gemm_profile_enable(bool on) {
   g_prof_on = on;
 }
 
-// Per class (FP64: opA*3 + opB, 0..8; FP32: 9 + 2*(opA == C) + (opB == C), 9..12): summed
+// Per class (FP64: opA*3 + opB, 0..8; FP32: 9 + 2*(opA == C) + (opB == C), 9..12; split
+// BF16: 13 + 2*(opA == C) + (opB == C), 13..16): summed
 // launch durations (ms), modelled flops and launch counts of the recorded launches; the
 // records are then dropped.
 void gemm_profile_read(double* ms, double* flops, long* launches) {

@@ -20,7 +20,7 @@ EXPORTS = (
     "thcgw_create", "thcgw_destroy", "thcgw_last_error", "thcgw_update_G", "thcgw_primary_to_aux",
     "thcgw_aux_to_primary", "thcgw_pi", "thcgw_tau_to_w", "thcgw_dyson_w", "thcgw_w_to_tau", "thcgw_sigma",
     "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
-    "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read",
+    "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -72,6 +72,7 @@ def load_library(path=LIB_PATH):
         "thcgw_gw_iteration": (i, [p, p, p, p, p, p, p, i, p]),
         "thcgw_solve_qp": (i, [p, p, p, p, p, d, i, d, d, p, p]),
         "thcgw_gm_energy": (i, [p, p, p, p, p]),
+        "thcgw_hf": (i, [p, p, p, p, i, i, p, p]),
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
         "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
@@ -166,14 +167,16 @@ class ThcGW:
 
     def profile_read(self):
         """{class: (ms, flops, launches)} of the GEMM launches recorded since the last read
-        (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B) for FP64 products and
-        "NN32", "NC32", "CN32", "CC32" for FP32 products."""
-        ms = (ctypes.c_double * 13)()
-        fl = (ctypes.c_double * 13)()
-        n = (ctypes.c_long * 13)()
+        (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B) for FP64 products,
+        "NN32", "NC32", "CN32", "CC32" for FP32 products and "NNbf" ... "CCbf" for split-BF16
+        products."""
+        ms = (ctypes.c_double * 17)()
+        fl = (ctypes.c_double * 17)()
+        n = (ctypes.c_long * 17)()
         self._check(self.lib.thcgw_profile_read(self.h, ms, fl, n))
-        names = [a + b for a in "NTC" for b in "NTC"] + ["NN32", "NC32", "CN32", "CC32"]
-        return {names[c]: (ms[c], fl[c], n[c]) for c in range(13) if n[c]}
+        names = ([a + b for a in "NTC" for b in "NTC"] + ["NN32", "NC32", "CN32", "CC32"]
+                 + ["NNbf", "NCbf", "CNbf", "CCbf"])
+        return {names[c]: (ms[c], fl[c], n[c]) for c in range(17) if n[c]}
 
     def workspace_bytes(self):
         return int(self.lib.thcgw_workspace_bytes(self.h))
@@ -266,6 +269,14 @@ class ThcGW:
                                             int(nfit), ctypes.c_double(eta), ctypes.c_double(tol), _np_ptr(E),
                                             self._stream(stream)))
         return E
+
+    def hf(self, Dm, X, V, hartree=True, exchange=True, out=None, stream=None):
+        """THC Hartree-Fock potential F_skij (1, nk, no, no) of the density matrix Dm_skij."""
+        Dm, X, V = self._dev(Dm), self._dev(X), self._dev(V)
+        F = out if out is not None else self._empty(1, self.nk, self.no, self.no)
+        self._check(self.lib.thcgw_hf(self.h, _ptr(Dm), _ptr(X), _ptr(V), int(bool(hartree)), int(bool(exchange)),
+                                      _ptr(F), self._stream(stream)))
+        return F
 
     def gm_energy(self, G, Sigma, stream=None):
         G, Sigma = self._dev(G), self._dev(Sigma)

@@ -159,6 +159,17 @@ int thcgw_solve_qp(thcgw_ctx* ctx, const double* Sigma_tskij, const double* C, c
 int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_tskij, double* e_corr,
                     void* stream);
 
+/* THC Hartree-Fock potential F_skij (1,nk,no,no) of a density matrix Dm_skij (1,nk,no,no)
+ * with collocation X_skPa and THC Coulomb V_qPQ (the same V as the GW calls): Hartree
+ * J_P = sum_Q V(q=0)_PQ 2 Dm^P_{R=0,QQ} on the PQ diagonal and exchange
+ * -(1/N_k) sum_q Dm^P(k-q) o V(q), projected back by X^H . X.  Either part can be
+ * switched off.  Replaces hf_t::evaluate -> thc_hf_Xqindep
+ * (src/methods/HF/thc_hf.cpp:20-60, src/methods/HF/thc_hf.icc:18-300) for ns = npol = 1,
+ * no symmetry and ignore_g0 (HF_K_correction returns at once, hf_t.cpp:49-52).
+ * Enqueues only. */
+int thcgw_hf(thcgw_ctx* ctx, const double* Dm_skij, const double* X_skPa, const double* V_qPQ, int hartree,
+             int exchange, double* F_skij, void* stream);
+
 /* Diagnostics (no counterpart in the reference): bytes of device workspace held by ctx,
  * control flags and GEMM launch timing.
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;
@@ -173,13 +184,16 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_t
  *   flag 7: precision: 0 = FP64 throughout (default); 1 = mixed: the projections X G X^H and
  *           X^H Sigma X, the k-grid transforms with the Hadamard products, and the products
  *           of the Hermitian Dyson form run in FP32 (complex<float> intermediates), while
- *           the tau <-> iW transforms, the Cholesky factorisations and ln det stay FP64.
+ *           the tau <-> iW transforms, the Cholesky factorisations and ln det stay FP64;
+ *           2 = as 1 with every FP32 product done as split-BF16 MFMA products (three bf16
+ *           parts per value, six part products, FP32 accumulation).
  *           Inputs and outputs keep their complex<double> layouts.
- * thcgw_profile_read fills, for each of the 13 GEMM classes, the summed kernel time ms[c]
+ * thcgw_profile_read fills, for each of the 17 GEMM classes, the summed kernel time ms[c]
  * of the launches recorded since the last read, their modelled flops[c] (8 M N K per
  * complex product, reduced for triangular products) and launches[c], then drops the
  * records.  Classes 0..8 are the FP64 products c = opA*3 + opB (op 0 = N, 1 = T, 2 = C);
- * 9..12 the FP32 products NN, NC, CN, CC.  It waits for the recorded launches to finish. */
+ * 9..12 the FP32 products NN, NC, CN, CC; 13..16 the split-BF16 products NN, NC, CN, CC.
+ * It waits for the recorded launches to finish. */
 long thcgw_workspace_bytes(const thcgw_ctx* ctx);
 int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value);
 int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, long* launches);

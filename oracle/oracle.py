@@ -34,6 +34,10 @@ Parity pins (what checks each oracle function; see tests/test_oracle_pins.py):
   pade_eval       (pade_t.hpp:34-168)                  exact reproduction of a rational function
   solve_qp        (qp_scf_common.cpp:150-311,679-728)  known root of a single-pole self-energy
   gm_energy       (scf_common.cpp:87-136)              closed form for diagonal G and Sigma
+  thc_hf          (thc_hf.icc:18-300)                  N_k = 1 closed form X^H[diag(J) - D^P o V]X;
+                                                        brute-force k - q convolution of the
+                                                        exchange; Hermiticity; negative exchange
+                                                        energy
 
 No function of this oracle is left "parity unpinned"; the reference's own end-to-end
 values (test_thc_gw.cpp) need the ISDF/QE/PySCF stack and cannot be used as pins.
@@ -70,7 +74,7 @@ def lib():
         _lib.orc_num_threads.restype = i
         for f in ("orc_update_G", "orc_primary_to_aux", "orc_aux_to_primary", "orc_pi_rspace", "orc_pi_kspace",
                   "orc_tau_to_w_phsym", "orc_w_to_tau_phsym", "orc_transform", "orc_sigma_rspace",
-                  "orc_sigma_kspace", "orc_hermitize", "orc_pade_eval", "orc_solve_qp", "orc_kR_coefficients",
+                  "orc_sigma_kspace", "orc_thc_hf", "orc_hermitize", "orc_pade_eval", "orc_solve_qp", "orc_kR_coefficients",
                   "orc_zgemm", "orc_set_num_threads"):
             getattr(_lib, f).restype = None
         del d, i, l, p
@@ -279,6 +283,16 @@ def sigma_kspace(G, X, Wqt, kgrid):
     S = np.zeros_like(G)
     lib().orc_sigma_kspace(nt, _kg(kgrid), nmu, no, _p(G), _p(X), _p(Wqt), _p(S))
     return S
+
+
+def thc_hf(Dm, X, V, kgrid, hartree=True, exchange=True):
+    """THC Hartree-Fock potential F_kij for a density matrix Dm (nk, no, no) (thc_hf.icc:18-300)."""
+    Dm, X, V = _c(Dm), _c(X), _c(V)
+    nk, no = Dm.shape[-3], Dm.shape[-1]
+    nmu = X.shape[-2]
+    F = np.zeros((nk, no, no), np.complex128)
+    lib().orc_thc_hf(_kg(kgrid), nmu, no, _p(Dm), _p(X), _p(V), int(bool(hartree)), int(bool(exchange)), _p(F))
+    return F
 
 
 def hermitize(A):

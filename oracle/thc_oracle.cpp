@@ -420,6 +420,52 @@ void orc_sigma_rspace(int nt, const int* kgrid, int nmu, int no, const cd* G, co
   }
 }
 
+// THC Hartree-Fock potential, hf_t::thc_hf_Xqindep (thc_hf.icc:18-300) for ns = npol = 1,
+// no symmetry and ignore_g0 (HF_K_correction returns at once, hf_t.cpp:49-52):
+//   Dm_kPQ = X_k Dm_k X_k^H                                  (primary_to_aux, :162-163)
+//   Dm_RPQ = f_Rk Dm_kPQ                                      (:170-179)
+//   Hartree: J_P = sum_Q U(q=0)_PQ * 2 Dm_{R=0,QQ}            (:75, :182-205, factor 2 for ns=npol=1)
+//            F_kPP += J_P for every k
+//   exchange: U_RPQ = f_Rk U_qPQ                              (:143-151)
+//             F_kPQ += f_kR [ -Dm_RPQ o U_RPQ ]               (:211-232)
+//   F_kij = X_k^H F_kPQ X_k                                   (aux_to_primary, :236-237)
+// Dm and F are (nk, no, no); U = V is (nk, nmu, nmu).
+void orc_thc_hf(const int* kgrid, int nmu, int no, const cd* Dm, const cd* X, const cd* V, int hartree, int exchange,
+                cd* F) {
+  long nk = long(kgrid[0]) * kgrid[1] * kgrid[2];
+  long M = long(nmu) * nmu;
+  std::vector<cd> f_Rk(nk * nk), f_kR(nk * nk);
+  orc_kR_coefficients(kgrid, f_Rk.data(), f_kR.data());
+  std::vector<cd> D(nk * M), U(V, V + nk * M), buf(nk * M), FP(nk * M, cd(0.0));
+  orc_primary_to_aux(1, int(nk), nmu, no, X, Dm, D.data());
+  if (nk != 1) {
+    zgemm('N', 'N', nk, M, nk, 1.0, f_Rk.data(), nk, D.data(), M, 0.0, buf.data(), M);
+    D.swap(buf);
+  }
+  if (hartree) {
+    std::vector<cd> dQ(nmu), J(nmu, cd(0.0));
+    for (int Q = 0; Q < nmu; ++Q) dQ[Q] = 2.0 * D[long(Q) * nmu + Q];  // R = 0 is the first R index
+    for (int P = 0; P < nmu; ++P)
+      for (int Q = 0; Q < nmu; ++Q) J[P] += V[long(P) * nmu + Q] * dQ[Q];  // U(q = 0)
+    for (long k = 0; k < nk; ++k)
+      for (int P = 0; P < nmu; ++P) FP[k * M + long(P) * nmu + P] += J[P];
+  }
+  if (exchange) {
+    if (nk != 1) {
+      zgemm('N', 'N', nk, M, nk, 1.0, f_Rk.data(), nk, U.data(), M, 0.0, buf.data(), M);
+      U.swap(buf);
+    }
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < nk * M; ++x) D[x] = -1.0 * cmul(D[x], U[x]);
+    if (nk != 1) {
+      zgemm('N', 'N', nk, M, nk, 1.0, f_kR.data(), nk, D.data(), M, 1.0, FP.data(), M);
+    } else {
+      for (long x = 0; x < M; ++x) FP[x] += D[x];
+    }
+  }
+  orc_aux_to_primary(1, int(nk), nmu, no, X, FP.data(), F);
+}
+
 // k-space self-energy, thc_gw.icc:342-443 with isym = 0 and no time reversal:
 //   Sigma_PQ(tau, k) = -(1/N_k) sum_q G_PQ(tau, k - q) o W_PQ(tau, q)
 void orc_sigma_kspace(int nt, const int* kgrid, int nmu, int no, const cd* G, const cd* X,

@@ -179,6 +179,52 @@ def test_pi_brute_force_convolution(orc):
     assert rel(PR, ref) < 1e-13
 
 
+# ----------------------------------------------------------------- THC Hartree-Fock
+def density_matrix(s, beta=1000.0, mu=0.0):
+    f = 1.0 / (1.0 + np.exp(np.clip(beta * (s.E - mu), -700, 700)))
+    return np.einsum("kia,ka,kja->kij", s.C, f, s.C.conj())
+
+
+def test_thc_hf_gamma_closed_form(orc):
+    """N_k = 1: F = X^H [diag(J) - D^P o V] X with D^P = X Dm X^H and J = 2 V diag(D^P)."""
+    s = make_system((1, 1, 1), 6, 11, system_id=21)
+    Dm = density_matrix(s)
+    DP = np.einsum("pa,ab,qb->pq", s.X[0], Dm[0], s.X[0].conj())
+    J = 2.0 * s.V[0] @ np.diag(DP)
+    FP = np.diag(J) - DP * s.V[0]
+    ref = np.einsum("pa,pq,qb->ab", s.X[0].conj(), FP, s.X[0])
+    F = orc.thc_hf(Dm, s.X, s.V, (1, 1, 1))
+    assert rel(F[0], ref) < 1e-13
+    Fx = orc.thc_hf(Dm, s.X, s.V, (1, 1, 1), hartree=False)
+    assert rel(Fx[0], np.einsum("pa,pq,qb->ab", s.X[0].conj(), -DP * s.V[0], s.X[0])) < 1e-13
+
+
+@pytest.mark.parametrize("kgrid", [(2, 2, 1), (3, 1, 2)])
+def test_thc_hf_brute_force_convolution(orc, kgrid):
+    """Exchange F^P(k) = -(1/N_k) sum_q D^P(k - q) o V(q); Hartree J = 2 V(0) mean_k diag D^P(k)."""
+    s = make_system(kgrid, 5, 7, system_id=22)
+    Dm = density_matrix(s)
+    nk = s.nk
+    DP = np.einsum("kpa,kab,kqb->kpq", s.X, Dm, s.X.conj())
+    n = np.array(kgrid)
+    FP = np.zeros_like(DP)
+    for k in range(nk):
+        for q in range(nk):
+            kmq = np.ravel_multi_index(tuple((np.array(np.unravel_index(k, kgrid)) - np.array(np.unravel_index(q, kgrid))) % n), kgrid)
+            FP[k] += -DP[kmq] * s.V[q] / nk
+    J = 2.0 * s.V[0] @ np.mean(np.diagonal(DP, axis1=1, axis2=2), axis=0)
+    FP += np.diag(J)[None]
+    ref = np.einsum("kpa,kpq,kqb->kab", s.X.conj(), FP, s.X)
+    F = orc.thc_hf(Dm, s.X, s.V, kgrid)
+    assert rel(F, ref) < 1e-12
+    # Hermitian Dm and V_q give a Hermitian exchange potential (a sum of Hadamard products of
+    # Hermitian matrices) with a negative exchange energy; the Hartree part is Hermitian only
+    # for a real V(q = 0), which the synthetic V is not
+    Fx = orc.thc_hf(Dm, s.X, s.V, kgrid, hartree=False)
+    assert rel(Fx, np.conj(np.swapaxes(Fx, -1, -2))) < 1e-12
+    assert np.einsum("kij,kji->", Dm, Fx).real < 0.0
+
+
 # ----------------------------------------------------------------- Dyson
 def test_dyson_closed_form_and_lapack(orc):
     rng = np.random.default_rng(5)
