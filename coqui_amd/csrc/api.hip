@@ -38,6 +38,9 @@ struct thcgw_ctx {
   c128* d_Tplus = nullptr;  // nw_half x nt_half (IAFT.icc:62-70)
   c128* d_Tminus = nullptr; // nt_half x nw_half (IAFT.icc:117-125)
   c128* d_Twt_ff = nullptr; // nw_f x nt
+  c128* d_Ttw_ff = nullptr; // nt x nw_f
+  c128* d_Tbeta = nullptr;  // 1 x nt
+  long* d_wn_f = nullptr;   // nw_f fermionic Matsubara indices
   std::map<std::string, DevBuf> ws;
   std::string err;
   bool force_fallback = false;
@@ -74,8 +77,12 @@ struct thcgw_ctx {
     if (d_Tplus) (void)hipFree(d_Tplus);
     if (d_Tminus) (void)hipFree(d_Tminus);
     if (d_Twt_ff) (void)hipFree(d_Twt_ff);
+    if (d_Ttw_ff) (void)hipFree(d_Ttw_ff);
+    if (d_Tbeta) (void)hipFree(d_Tbeta);
+    if (d_wn_f) (void)hipFree(d_wn_f);
     d_tau = nullptr;
-    d_Tplus = d_Tminus = d_Twt_ff = nullptr;
+    d_Tplus = d_Tminus = d_Twt_ff = d_Ttw_ff = d_Tbeta = nullptr;
+    d_wn_f = nullptr;
   }
 };
 
@@ -489,6 +496,73 @@ void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c1
   if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
 }
 
+// ---- Dyson equation for G on the Matsubara axis ------------------------------------
+
+// Sw(iw_n) = tau_to_w(Sigma(tau), fermi) over all (k, i, j)   (simple_dyson.cpp:39-40)
+c128* sigma_to_w(thcgw_ctx* ctx, const c128* Sigma, hipStream_t s) {
+  const int nt = ctx->d.nt, nw = ctx->d.nw_f;
+  const long dim = static_cast<long>(ctx->nk) * ctx->d.no * ctx->d.no;
+  THCGW_REQUIRE(dim < (1L << 31), "Dyson for G: N_k N_o^2 too large for one GEMM");
+  c128* Sw = ctx->buf<c128>("dg_sw", static_cast<size_t>(nw) * dim);
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nw, static_cast<int>(dim), nt, kOne, thcgw::bp(ctx->d_Twt_ff, 0), nt,
+               thcgw::bp(Sigma, 0), dim, kZero, Sw, 0, dim, 1);
+  return Sw;
+}
+
+// Out[n,k] = [(iw_n + mu) S - H0 - F - Sw]^-1 Rhs, Rhs = S_k (with_s) or I, for all (n, k) by
+// batched LU (A = Lt Ut Pt), two triangular solves and the row gather that undoes the swaps.
+void matsubara_solve(thcgw_ctx* ctx, const c128* H0, const c128* F, const c128* S, const c128* Sw, double mu,
+                     bool rhs_s, c128* Out, hipStream_t s) {
+  const int nw = ctx->d.nw_f, nk = ctx->nk, no = ctx->d.no;
+  const long m = static_cast<long>(no) * no;
+  const long count = static_cast<long>(nw) * nk;
+  THCGW_REQUIRE(count < (1L << 30), "Dyson for G: too many matrices");
+  c128* X = ctx->buf<c128>("dg_x", static_cast<size_t>(count) * m);
+  c128* Y = ctx->buf<c128>("dg_y", static_cast<size_t>(count) * m);
+  int* ipiv = ctx->buf<int>("dg_ipiv", static_cast<size_t>(count) * no);
+  int* info = ctx->buf<int>("dg_info", static_cast<size_t>(count));
+  thcgw::matsubara_dyson(s, H0, F, S, Sw, ctx->d_wn_f, ctx->d.beta, mu, nw, nk, no, X);
+  thcgw::getrf_batched(s, X, m, no, no, ipiv, info, static_cast<int>(count));
+  thcgw::rhs_batch(s, rhs_s ? S : nullptr, nk, no, count, Y);
+  thcgw::trsm_left_lower(s, X, m, no, Y, m, no, no, no, static_cast<int>(count));
+  thcgw::trsm_left_upper_unit(s, X, m, no, Y, m, no, no, no, static_cast<int>(count));
+  thcgw::dyson_finish(s, Y, m, nullptr, 0, 1, ipiv, no, Out, m, 0, static_cast<int>(count));
+  std::vector<int> h(count);
+  THCGW_HIP(hipMemcpyAsync(h.data(), info, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  for (long b = 0; b < count; ++b)
+    if (h[b] != 0)
+      throw thcgw::Error(THCGW_ERR_SINGULAR, "Dyson for G: (iw_n + mu) S - H0 - F - Sigma is singular at (n, k) = (" +
+                                                 std::to_string(b / nk) + ", " + std::to_string(b % nk) + ")");
+}
+
+// Electron number at mu (compute_Nelec, scf_common.cpp:10-42) with the spectral sum of
+// compute_eigenspectra (simple_dyson.cpp:130-172) taken as the resolvent trace
+// sum_b 1/(iw_n + mu - lambda_b) = Tr[((iw_n + mu) S - H0 - F - Sigma)^-1 S].
+double nelec_at(thcgw_ctx* ctx, const c128* H0, const c128* F, const c128* S, const c128* Sw, double mu,
+                hipStream_t s) {
+  const int nw = ctx->d.nw_f, nk = ctx->nk, no = ctx->d.no, nt = ctx->d.nt;
+  const long m = static_cast<long>(no) * no;
+  const long count = static_cast<long>(nw) * nk;
+  c128* R = ctx->buf<c128>("dg_r", static_cast<size_t>(count) * m);
+  c128* tr = ctx->buf<c128>("dg_tr", static_cast<size_t>(count));
+  matsubara_solve(ctx, H0, F, S, Sw, mu, true, R, s);
+  thcgw::batched_trace(s, R, m, no, tr, static_cast<int>(count));
+  std::vector<std::complex<double>> h(count);
+  THCGW_HIP(hipMemcpyAsync(h.data(), tr, sizeof(c128) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  std::vector<std::complex<double>> Xw(nw, 0.0);
+  for (int n = 0; n < nw; ++n)
+    for (int k = 0; k < nk; ++k) Xw[n] += h[static_cast<size_t>(n) * nk + k] / double(nk);  // k weights 1/N_k
+  std::complex<double> nel = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    std::complex<double> xt = 0.0;
+    for (int n = 0; n < nw; ++n) xt += ctx->tab.Ttw_ff[static_cast<size_t>(t) * nw + n] * Xw[n];
+    nel += ctx->tab.T_beta[t] * xt;
+  }
+  return -2.0 * nel.real();  // spin factor 2 for ns = npol = 1
+}
+
 }  // namespace
 
 extern "C" {
@@ -531,6 +605,15 @@ int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ct
     THCGW_HIP(hipMemcpy(ctx->d_Tminus, Tm.data(), sizeof(c128) * Tm.size(), hipMemcpyHostToDevice));
     THCGW_HIP(hipMalloc(&ctx->d_Twt_ff, sizeof(c128) * ctx->tab.Twt_ff.size()));
     THCGW_HIP(hipMemcpy(ctx->d_Twt_ff, ctx->tab.Twt_ff.data(), sizeof(c128) * ctx->tab.Twt_ff.size(),
+                        hipMemcpyHostToDevice));
+    THCGW_HIP(hipMalloc(&ctx->d_Ttw_ff, sizeof(c128) * ctx->tab.Ttw_ff.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_Ttw_ff, ctx->tab.Ttw_ff.data(), sizeof(c128) * ctx->tab.Ttw_ff.size(),
+                        hipMemcpyHostToDevice));
+    THCGW_HIP(hipMalloc(&ctx->d_Tbeta, sizeof(c128) * ctx->tab.T_beta.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_Tbeta, ctx->tab.T_beta.data(), sizeof(c128) * ctx->tab.T_beta.size(),
+                        hipMemcpyHostToDevice));
+    THCGW_HIP(hipMalloc(&ctx->d_wn_f, sizeof(long) * ctx->tab.wn_f.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_wn_f, ctx->tab.wn_f.data(), sizeof(long) * ctx->tab.wn_f.size(),
                         hipMemcpyHostToDevice));
   } catch (const thcgw::Error& e) {
     ctx->err = e.what();
@@ -780,6 +863,78 @@ int thcgw_hf(thcgw_ctx* ctx, const double* Dm, const double* X, const double* V,
       thcgw::hf_hartree(s, DP, C(V), d, FP, nk, nmu);
     }
     project_a2p(ctx, s, C(X), FP, Cm(F), nk, Ab);  // thc_hf.icc:236-237
+  });
+}
+
+int thcgw_dyson_g(thcgw_ctx* ctx, const double* H0, const double* F, const double* Sov, const double* Sigma, double mu,
+                  double* G, double* Dm, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(F && Sigma && G, "thcgw_dyson_g: null pointer");
+    hipStream_t s = S(stream);
+    const int nt = ctx->d.nt, nw = ctx->d.nw_f;
+    const long dim = static_cast<long>(ctx->nk) * ctx->d.no * ctx->d.no;
+    c128* Sw = sigma_to_w(ctx, C(Sigma), s);
+    c128* Gw = ctx->buf<c128>("dg_r", static_cast<size_t>(nw) * dim);
+    matsubara_solve(ctx, C(H0), C(F), C(Sov), Sw, mu, false, Gw, s);  // simple_dyson.cpp:65-78
+    // G(tau) = w_to_tau(G(iw), fermi)  (simple_dyson.cpp:97-98)
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nt, static_cast<int>(dim), nw, kOne, thcgw::bp(ctx->d_Ttw_ff, 0), nw,
+                 thcgw::bp(Gw, 0), dim, kZero, Cm(G), 0, dim, 1);
+    if (Dm)  // Dm = -tau_to_beta(G)  (simple_dyson.cpp:121-126)
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, 1, static_cast<int>(dim), nt, kMinusOne, thcgw::bp(ctx->d_Tbeta, 0),
+                   nt, thcgw::bp(C(G), 0), dim, kZero, Cm(Dm), 0, dim, 1);
+  });
+}
+
+int thcgw_nelec(thcgw_ctx* ctx, const double* H0, const double* F, const double* Sov, const double* Sigma, double mu,
+                double* nel, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(F && Sigma && nel, "thcgw_nelec: null pointer");
+    hipStream_t s = S(stream);
+    c128* Sw = sigma_to_w(ctx, C(Sigma), s);
+    *nel = nelec_at(ctx, C(H0), C(F), C(Sov), Sw, mu, s);
+  });
+}
+
+int thcgw_update_mu(thcgw_ctx* ctx, const double* H0, const double* F, const double* Sov, const double* Sigma,
+                    double nel_target, double old_mu, double mu_tol, double* mu_out, double* nel_out, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(F && Sigma && mu_out, "thcgw_update_mu: null pointer");
+    THCGW_REQUIRE(mu_tol > 0.0, "thcgw_update_mu: mu_tol must be positive");
+    hipStream_t s = S(stream);
+    c128* Sw = sigma_to_w(ctx, C(Sigma), s);
+    auto N = [&](double mu) { return nelec_at(ctx, C(H0), C(F), C(Sov), Sw, mu, s); };
+    // scf_common.cpp:147-202: bracket in steps of 0.2, then bisect to |nel - target| < mu_tol
+    const double delta = 0.2;
+    double nel = N(old_mu), mu = old_mu;
+    if (std::abs(nel - nel_target) >= mu_tol) {
+      double mu1, mu2;
+      if (nel >= nel_target) {
+        mu2 = old_mu;
+        mu1 = old_mu - delta;
+        int it = 0;
+        for (; it < 1000 && N(mu1) > nel_target; ++it) mu1 -= delta;
+        THCGW_REQUIRE(it < 1000, "thcgw_update_mu: no lower bracket within 200 Ha");
+      } else {
+        mu1 = old_mu;
+        mu2 = old_mu + delta;
+        int it = 0;
+        for (; it < 1000 && N(mu2) < nel_target; ++it) mu2 += delta;
+        THCGW_REQUIRE(it < 1000, "thcgw_update_mu: no upper bracket within 200 Ha");
+      }
+      mu = 0.5 * (mu1 + mu2);
+      nel = N(mu);
+      for (int it = 0; it < 200 && std::abs(nel - nel_target) >= mu_tol; ++it) {
+        if (nel >= nel_target) mu2 = mu;
+        else mu1 = mu;
+        mu = 0.5 * (mu1 + mu2);
+        nel = N(mu);
+      }
+    }
+    *mu_out = mu;
+    if (nel_out) *nel_out = nel;
   });
 }
 

@@ -115,13 +115,13 @@ __global__ __launch_bounds__(T256) void dyson_finish_kernel(const c128* Z, long 
   }
   __syncthreads();
   const c128* Zb = Z + static_cast<long>(b) * sZ;
-  const c128* Vb = V + static_cast<long>((b0 + b) % vmod) * sV;
+  const c128* Vb = V ? V + static_cast<long>((b0 + b) % vmod) * sV : nullptr;
   c128* Wb = W + static_cast<long>(b) * sW;
   const long per = static_cast<long>(n) * n;
   for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < per; e += static_cast<long>(gridDim.x) * T256) {
     const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
     const c128 z = Zb[static_cast<long>(idx[r]) * n + c];
-    const c128 v = Vb[e];
+    const c128 v = Vb ? Vb[e] : c128{0.0, 0.0};
     Wb[e] = c128{z.x - v.x, z.y - v.y};
   }
 }
@@ -134,6 +134,50 @@ __global__ __launch_bounds__(T256) void copy_batch_kernel(const T* src, long sS,
   T* d = dst + static_cast<long>(b) * sD;
   for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < per; e += static_cast<long>(gridDim.x) * T256)
     d[e] = s[e];
+}
+
+// X[n][k] = (iw_n + mu) S_k - H0_k - F_k - Sw[n][k], iw_n = i wn[n] pi / beta; S = NULL is the
+// identity and H0 = NULL zero.  One thread per element of the (nw, nk, no, no) array.
+__global__ __launch_bounds__(T256) void matsubara_dyson_kernel(const c128* H0, const c128* F, const c128* S,
+                                                               const c128* Sw, const long* wn, double beta, double mu,
+                                                               int nw, int nk, int no, c128* X) {
+  const long m = static_cast<long>(no) * no, dim = nk * m;
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= nw * dim) return;
+  const int n = static_cast<int>(e / dim);
+  const long ke = e % dim;
+  const int i = static_cast<int>((ke % m) / no), j = static_cast<int>(ke % no);
+  const double wi = double(wn[n]) * M_PI / beta;
+  const c128 sij = S ? S[ke] : c128{i == j ? 1.0 : 0.0, 0.0};
+  const c128 h = H0 ? H0[ke] : c128{0.0, 0.0};
+  const c128 f = F[ke], sw = Sw[e];
+  // (mu + i wi) * s
+  X[e] = c128{mu * sij.x - wi * sij.y - h.x - f.x - sw.x, mu * sij.y + wi * sij.x - h.y - f.y - sw.y};
+}
+
+// RHS of the inverse: S_k (or the identity) for every (n, k).
+__global__ __launch_bounds__(T256) void rhs_batch_kernel(const c128* S, int nk, int no, long count, c128* Y) {
+  const long m = static_cast<long>(no) * no;
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= count * m) return;
+  const long b = e / m, ij = e % m;
+  const int i = static_cast<int>(ij / no), j = static_cast<int>(ij % no);
+  Y[e] = S ? S[(b % nk) * m + ij] : c128{i == j ? 1.0 : 0.0, 0.0};
+}
+
+// tr[b] = sum_j A[b][j][j]
+__global__ __launch_bounds__(T256) void batched_trace_kernel(const c128* A, long sA, int n, c128* tr) {
+  __shared__ double sh[T256 / 64];
+  const c128* Ab = A + static_cast<long>(blockIdx.x) * sA;
+  double re = 0.0, im = 0.0;
+  for (int c = threadIdx.x; c < n; c += T256) {
+    const c128 d = Ab[static_cast<long>(c) * n + c];
+    re += d.x;
+    im += d.y;
+  }
+  re = block_sum(re, sh);
+  im = block_sum(im, sh);
+  if (threadIdx.x == 0) tr[blockIdx.x] = c128{re, im};
 }
 
 // d[Q] = (2 / nk) sum_k DP[k][Q][Q]: twice the R = 0 diagonal of the projected density.
@@ -277,6 +321,26 @@ void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, lo
 void copy_batch(hipStream_t s, const c64* src, long sS, int smod, c64* dst, long sD, long per, int b0, int batch) {
   const unsigned gx = static_cast<unsigned>(std::min<long>((per + T256 - 1) / T256, 64));
   hipLaunchKernelGGL(copy_batch_kernel<c64>, dim3(gx, batch), dim3(T256), 0, s, src, sS, smod, dst, sD, per, b0);
+  THCGW_HIP(hipGetLastError());
+}
+
+void matsubara_dyson(hipStream_t s, const c128* H0, const c128* F, const c128* S, const c128* Sw, const long* wn,
+                     double beta, double mu, int nw, int nk, int no, c128* X) {
+  const long total = static_cast<long>(nw) * nk * no * no;
+  hipLaunchKernelGGL(matsubara_dyson_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s,
+                     H0, F, S, Sw, wn, beta, mu, nw, nk, no, X);
+  THCGW_HIP(hipGetLastError());
+}
+
+void rhs_batch(hipStream_t s, const c128* S, int nk, int no, long count, c128* Y) {
+  const long total = count * no * no;
+  hipLaunchKernelGGL(rhs_batch_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, S,
+                     nk, no, count, Y);
+  THCGW_HIP(hipGetLastError());
+}
+
+void batched_trace(hipStream_t s, const c128* A, long sA, int n, c128* tr, int batch) {
+  hipLaunchKernelGGL(batched_trace_kernel, dim3(batch), dim3(T256), 0, s, A, sA, n, tr);
   THCGW_HIP(hipGetLastError());
 }
 

@@ -137,12 +137,18 @@ void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, 
 void hermitize(hipStream_t s, c128* A, int n, long nb);
 void one_minus_trace(hipStream_t s, const c128* A, long sA, int n, c128* trace, int batch);
 void log_re_det(hipStream_t s, const c128* A, long sA, int n, const int* ipiv, double* ldet, int batch);
+// W = P Z - V: rows of Z in the order that undoes the LU column swaps, minus V (nullable).
 void dyson_finish(hipStream_t s, const c128* Z, long sZ, const c128* V, long sV, int vmod, const int* ipiv, int n,
                   c128* W, long sW, int b0, int batch);
 void copy_batch(hipStream_t s, const c128* src, long sS, int smod, c128* dst, long sD, long per, int b0, int batch);
 void copy_batch(hipStream_t s, const c64* src, long sS, int smod, c64* dst, long sD, long per, int b0, int batch);
 void to_c64(hipStream_t s, const c128* src, c64* dst, long n);
 void zero_c128(hipStream_t s, c128* p, long n);
+// X = (iw_n + mu) S - H0 - F - Sw for all (n, k) (S, H0 nullable); Y = S_k or I per (n, k).
+void matsubara_dyson(hipStream_t s, const c128* H0, const c128* F, const c128* S, const c128* Sw, const long* wn,
+                     double beta, double mu, int nw, int nk, int no, c128* X);
+void rhs_batch(hipStream_t s, const c128* S, int nk, int no, long count, c128* Y);
+void batched_trace(hipStream_t s, const c128* A, long sA, int n, c128* tr, int batch);
 // THC Hartree term: d = twice the R = 0 diagonal of DP (nk, nmu, nmu); FP_kPP += (V0 d)_P.
 void hf_hartree(hipStream_t s, const c128* DP, const c128* V0, c128* d, c128* FP, int nk, int nmu);
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb);

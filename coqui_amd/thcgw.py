@@ -20,7 +20,8 @@ EXPORTS = (
     "thcgw_create", "thcgw_destroy", "thcgw_last_error", "thcgw_update_G", "thcgw_primary_to_aux",
     "thcgw_aux_to_primary", "thcgw_pi", "thcgw_tau_to_w", "thcgw_dyson_w", "thcgw_w_to_tau", "thcgw_sigma",
     "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
-    "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf",
+    "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf", "thcgw_dyson_g",
+    "thcgw_nelec", "thcgw_update_mu",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -73,6 +74,9 @@ def load_library(path=LIB_PATH):
         "thcgw_solve_qp": (i, [p, p, p, p, p, d, i, d, d, p, p]),
         "thcgw_gm_energy": (i, [p, p, p, p, p]),
         "thcgw_hf": (i, [p, p, p, p, i, i, p, p]),
+        "thcgw_dyson_g": (i, [p, p, p, p, p, d, p, p, p]),
+        "thcgw_nelec": (i, [p, p, p, p, p, d, ctypes.POINTER(d), p]),
+        "thcgw_update_mu": (i, [p, p, p, p, p, d, d, d, ctypes.POINTER(d), ctypes.POINTER(d), p]),
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
         "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
@@ -86,7 +90,7 @@ def load_library(path=LIB_PATH):
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(None if t is None else t.data_ptr())
 
 
 def _np_ptr(a):
@@ -277,6 +281,35 @@ class ThcGW:
         self._check(self.lib.thcgw_hf(self.h, _ptr(Dm), _ptr(X), _ptr(V), int(bool(hartree)), int(bool(exchange)),
                                       _ptr(F), self._stream(stream)))
         return F
+
+    def _opt_dev(self, a):
+        return None if a is None else self._dev(a)
+
+    def dyson_g(self, F, Sigma, mu, H0=None, S=None, want_dm=True, out=None, stream=None):
+        """Matsubara Dyson equation for G: returns (G_tskij, Dm_skij or None)."""
+        F, Sigma, H0, S = self._dev(F), self._dev(Sigma), self._opt_dev(H0), self._opt_dev(S)
+        G = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        Dm = self._empty(1, self.nk, self.no, self.no) if want_dm else None
+        self._check(self.lib.thcgw_dyson_g(self.h, _ptr(H0), _ptr(F), _ptr(S), _ptr(Sigma), ctypes.c_double(mu),
+                                           _ptr(G), _ptr(Dm), self._stream(stream)))
+        return G, Dm
+
+    def nelec(self, F, Sigma, mu, H0=None, S=None, stream=None):
+        F, Sigma, H0, S = self._dev(F), self._dev(Sigma), self._opt_dev(H0), self._opt_dev(S)
+        n = ctypes.c_double()
+        self._check(self.lib.thcgw_nelec(self.h, _ptr(H0), _ptr(F), _ptr(S), _ptr(Sigma), ctypes.c_double(mu),
+                                         ctypes.byref(n), self._stream(stream)))
+        return n.value
+
+    def update_mu(self, F, Sigma, nel_target, old_mu, mu_tol=1e-9, H0=None, S=None, stream=None):
+        """Chemical potential for nel_target: returns (mu, nelec)."""
+        F, Sigma, H0, S = self._dev(F), self._dev(Sigma), self._opt_dev(H0), self._opt_dev(S)
+        mu, n = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.thcgw_update_mu(self.h, _ptr(H0), _ptr(F), _ptr(S), _ptr(Sigma),
+                                             ctypes.c_double(nel_target), ctypes.c_double(old_mu),
+                                             ctypes.c_double(mu_tol), ctypes.byref(mu), ctypes.byref(n),
+                                             self._stream(stream)))
+        return mu.value, n.value
 
     def gm_energy(self, G, Sigma, stream=None):
         G, Sigma = self._dev(G), self._dev(Sigma)

@@ -170,6 +170,30 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G_tskij, const double* Sigma_t
 int thcgw_hf(thcgw_ctx* ctx, const double* Dm_skij, const double* X_skPa, const double* V_qPQ, int hartree,
              int exchange, double* F_skij, void* stream);
 
+/* Dyson equation for G on the Matsubara axis: Sigma_tskij (nt,1,nk,no,no) -> Sigma(iw_n)
+ * (fermionic), G(iw_n,k) = [(iw_n + mu) S_k - H0_k - F_k - Sigma(iw_n,k)]^-1,
+ * G_tskij = w_to_tau(G(iw)), and, if Dm_skij is not NULL, Dm = -G(beta^-).  H0, F, S are
+ * (1,nk,no,no); H0 = NULL means zero and S = NULL the identity.  Replaces
+ * simple_dyson::solve_dyson (src/methods/SCF/simple_dyson.cpp:11-127) for ns = 1.
+ * Returns THCGW_ERR_SINGULAR for a singular matrix.  Synchronises the stream. */
+int thcgw_dyson_g(thcgw_ctx* ctx, const double* H0_skij, const double* F_skij, const double* S_skij,
+                  const double* Sigma_tskij, double mu, double* G_tskij, double* Dm_skij, void* stream);
+
+/* Electron number per cell at chemical potential mu for the same Dyson equation:
+ * -2 tau_to_beta(w_to_tau(sum_k w_k Tr[G(iw_n,k) S_k])).  Replaces compute_Nelec
+ * (src/methods/SCF/scf_common.cpp:10-42) with the spectral sum of compute_eigenspectra
+ * (simple_dyson.cpp:130-172), sum_b 1/(iw_n + mu - lambda_b), taken as that resolvent
+ * trace.  Synchronises the stream. */
+int thcgw_nelec(thcgw_ctx* ctx, const double* H0_skij, const double* F_skij, const double* S_skij,
+                const double* Sigma_tskij, double mu, double* nelec, void* stream);
+
+/* Chemical potential for nelec_target: bracket from old_mu in steps of 0.2 Ha, then bisect
+ * until |nelec - target| < mu_tol (at most 200 halvings).  Replaces update_mu
+ * (src/methods/SCF/scf_common.cpp:147-202).  nelec_out may be NULL.  Synchronises. */
+int thcgw_update_mu(thcgw_ctx* ctx, const double* H0_skij, const double* F_skij, const double* S_skij,
+                    const double* Sigma_tskij, double nelec_target, double old_mu, double mu_tol, double* mu_out,
+                    double* nelec_out, void* stream);
+
 /* Diagnostics (no counterpart in the reference): bytes of device workspace held by ctx,
  * control flags and GEMM launch timing.
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;

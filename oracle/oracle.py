@@ -34,6 +34,12 @@ Parity pins (what checks each oracle function; see tests/test_oracle_pins.py):
   pade_eval       (pade_t.hpp:34-168)                  exact reproduction of a rational function
   solve_qp        (qp_scf_common.cpp:150-311,679-728)  known root of a single-pole self-energy
   gm_energy       (scf_common.cpp:87-136)              closed form for diagonal G and Sigma
+  dyson_g         (simple_dyson.cpp:11-127)            Sigma = 0, F = H(k): G equals update_G's
+                                                        closed form and Dm = C f C^H
+  nelec / update_mu (scf_common.cpp:10-42,147-202)     numpy eigenvalues of H + Sigma(iw_n) (the
+                                                        reference's compute_eigenspectra);
+                                                        Sigma = 0: 2 sum_k w_k sum_b f(e_b - mu)
+                                                        and the mu that solves it
   thc_hf          (thc_hf.icc:18-300)                  N_k = 1 closed form X^H[diag(J) - D^P o V]X;
                                                         brute-force k - q convolution of the
                                                         exchange; Hermiticity; negative exchange
@@ -70,6 +76,9 @@ def lib():
         _lib.orc_rpa_ec.restype = d
         _lib.orc_gm_energy.restype = d
         _lib.orc_dyson.restype = i
+        _lib.orc_dyson_g.restype = i
+        _lib.orc_nelec.restype = d
+        _lib.orc_update_mu.restype = d
         _lib.orc_zinverse.restype = i
         _lib.orc_num_threads.restype = i
         for f in ("orc_update_G", "orc_primary_to_aux", "orc_aux_to_primary", "orc_pi_rspace", "orc_pi_kspace",
@@ -293,6 +302,52 @@ def thc_hf(Dm, X, V, kgrid, hartree=True, exchange=True):
     F = np.zeros((nk, no, no), np.complex128)
     lib().orc_thc_hf(_kg(kgrid), nmu, no, _p(Dm), _p(X), _p(V), int(bool(hartree)), int(bool(exchange)), _p(F))
     return F
+
+
+def _opt(a):
+    return None if a is None else _p(_c(a))
+
+
+def dyson_g(tab, F, Sigma, mu, H0=None, S=None):
+    """G(tau) (nt, 1, nk, no, no) and Dm (nk, no, no) from the Matsubara Dyson equation
+    (simple_dyson.cpp:11-127)."""
+    F, Sigma = _c(F), _c(Sigma)
+    H0 = None if H0 is None else _c(H0)
+    S = None if S is None else _c(S)
+    nt, nk, no = tab["nt"], F.shape[-3], F.shape[-1]
+    G = np.zeros((nt, 1, nk, no, no), np.complex128)
+    Dm = np.zeros((nk, no, no), np.complex128)
+    wn = np.ascontiguousarray(tab["wn_f"], np.int64)
+    bad = lib().orc_dyson_g(nt, tab["nw_f"], _p(_c(tab["Twt_ff"])), _p(_c(tab["Ttw_ff"])), _p(_c(tab["T_beta"])),
+                            _p(wn), ctypes.c_double(tab["beta"]), nk, no, _opt(H0), _p(F), _opt(S), _p(Sigma),
+                            ctypes.c_double(mu), _p(G), _p(Dm))
+    if bad:
+        raise np.linalg.LinAlgError("oracle dyson_g: singular matrix at (n, k) index %d" % (bad - 1))
+    return G, Dm
+
+
+def nelec(tab, F, Sigma, mu, H0=None, S=None):
+    """Electron number at mu (compute_Nelec, scf_common.cpp:10-42)."""
+    F, Sigma = _c(F), _c(Sigma)
+    nk, no = F.shape[-3], F.shape[-1]
+    Sw = transform(tab["Twt_ff"], Sigma)
+    wn = np.ascontiguousarray(tab["wn_f"], np.int64)
+    return lib().orc_nelec(tab["nt"], tab["nw_f"], _p(_c(tab["Ttw_ff"])), _p(_c(tab["T_beta"])), _p(wn),
+                           ctypes.c_double(tab["beta"]), nk, no, _opt(H0), _p(F), _opt(S), _p(_c(Sw)),
+                           ctypes.c_double(mu))
+
+
+def update_mu(tab, F, Sigma, nel_target, old_mu, mu_tol=1e-9, H0=None, S=None):
+    """Chemical potential by bracketing and bisection (update_mu, scf_common.cpp:147-202)."""
+    F, Sigma = _c(F), _c(Sigma)
+    nk, no = F.shape[-3], F.shape[-1]
+    wn = np.ascontiguousarray(tab["wn_f"], np.int64)
+    nel = ctypes.c_double()
+    mu = lib().orc_update_mu(tab["nt"], tab["nw_f"], _p(_c(tab["Twt_ff"])), _p(_c(tab["Ttw_ff"])),
+                             _p(_c(tab["T_beta"])), _p(wn), ctypes.c_double(tab["beta"]), nk, no, _opt(H0), _p(F),
+                             _opt(S), _p(Sigma), ctypes.c_double(nel_target), ctypes.c_double(old_mu),
+                             ctypes.c_double(mu_tol), ctypes.byref(nel))
+    return mu, nel.value
 
 
 def hermitize(A):

@@ -466,6 +466,124 @@ void orc_thc_hf(const int* kgrid, int nmu, int no, const cd* Dm, const cd* X, co
   orc_aux_to_primary(1, int(nk), nmu, no, X, FP.data(), F);
 }
 
+// Dyson equation for G on the Matsubara axis, simple_dyson::solve_dyson
+// (simple_dyson.cpp:11-127), ns = 1, one rank:
+//   Sigma(iw_n) = tau_to_w(Sigma(tau), fermi)                   (:39-40; IAFT.icc:9-26)
+//   G(iw_n, k) = [(iw_n + mu) S_k - H0_k - F_k - Sigma(iw_n, k)]^-1   (:70-77)
+//                with iw_n = i wn pi / beta                          (IAFT.hpp:55-57)
+//   G(tau) = w_to_tau(G(iw), fermi)                              (:97-98; IAFT.icc:82-98)
+//   Dm = -tau_to_beta(G(tau))                                     (:116-127; IAFT.icc:226-238)
+// H0, F, S are (nk, no, no) (S = NULL: identity, H0 = NULL: zero); Sigma, G are
+// (nt, nk, no, no); Dm (nk, no, no) may be NULL.  Returns 0 or the first singular (n, k) + 1.
+int orc_dyson_g(int nt, int nw, const cd* Twt_ff, const cd* Ttw_ff, const cd* T_beta, const long* wn, double beta,
+                int nk, int no, const cd* H0, const cd* F, const cd* S, const cd* Sigma, double mu, cd* G, cd* Dm) {
+  const long m = long(no) * no, dim = long(nk) * m;
+  std::vector<cd> Sw(size_t(nw) * dim), Gw(size_t(nw) * dim);
+  orc_transform(nw, nt, Twt_ff, dim, Sigma, Sw.data());
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic)
+  for (long nkk = 0; nkk < long(nw) * nk; ++nkk) {
+    const long n = nkk / nk, k = nkk % nk;
+    const cd z = cd(mu, double(wn[n]) * M_PI / beta);
+    cd* X = &Gw[size_t(n) * dim + size_t(k) * m];
+    for (long i = 0; i < no; ++i)
+      for (long j = 0; j < no; ++j) {
+        const long e = k * m + i * no + j;
+        const cd sij = S ? S[e] : cd(i == j ? 1.0 : 0.0, 0.0);
+        X[i * no + j] = z * sij - (H0 ? H0[e] : cd(0.0)) - F[e] - Sw[size_t(n) * dim + e];
+      }
+    if (orc_zinverse(no, X) != 0) {
+#pragma omp critical
+      if (bad == 0) bad = int(nkk) + 1;
+    }
+  }
+  orc_transform(nt, nw, Ttw_ff, dim, Gw.data(), G);
+  if (Dm) {
+    for (long e = 0; e < dim; ++e) {
+      cd acc = 0.0;
+      for (int t = 0; t < nt; ++t) acc += T_beta[t] * G[size_t(t) * dim + e];
+      Dm[e] = -acc;
+    }
+  }
+  return bad;
+}
+
+// Electron number at chemical potential mu, compute_Nelec (scf_common.cpp:10-42), with the
+// spectral sum of compute_eigenspectra (simple_dyson.cpp:130-172) taken as a resolvent trace:
+//   sum_b 1/(iw_n + mu - lambda_b) = Tr[((iw_n + mu) S - H0 - F - Sigma(iw_n))^-1 S],
+//   lambda_b the eigenvalues of S^-1 (H0 + F + Sigma(iw_n));
+//   Xw(n) = sum_k w_k (the above), w_k = 1/nk;  nelec = -2 Re tau_to_beta(w_to_tau(Xw)).
+// Sw is Sigma(iw_n) (nw, nk, no, no).
+double orc_nelec(int nt, int nw, const cd* Ttw_ff, const cd* T_beta, const long* wn, double beta, int nk, int no,
+                 const cd* H0, const cd* F, const cd* S, const cd* Sw, double mu) {
+  const long m = long(no) * no, dim = long(nk) * m;
+  std::vector<cd> Xw(nw, cd(0.0));
+#pragma omp parallel for schedule(dynamic)
+  for (int n = 0; n < nw; ++n) {
+    std::vector<cd> X(m), R(m);
+    const cd z = cd(mu, double(wn[n]) * M_PI / beta);
+    cd acc = 0.0;
+    for (long k = 0; k < nk; ++k) {
+      for (long i = 0; i < no; ++i)
+        for (long j = 0; j < no; ++j) {
+          const long e = k * m + i * no + j;
+          const cd sij = S ? S[e] : cd(i == j ? 1.0 : 0.0, 0.0);
+          X[i * no + j] = z * sij - (H0 ? H0[e] : cd(0.0)) - F[e] - Sw[size_t(n) * dim + e];
+        }
+      orc_zinverse(no, X.data());
+      cd tr = 0.0;
+      for (long i = 0; i < no; ++i)
+        for (long j = 0; j < no; ++j) tr += X[i * no + j] * (S ? S[k * m + j * no + i] : cd(i == j ? 1.0 : 0.0));
+      acc += tr / double(nk);
+    }
+    Xw[n] = acc;
+  }
+  cd nel = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    cd xt = 0.0;
+    for (int n = 0; n < nw; ++n) xt += Ttw_ff[long(t) * nw + n] * Xw[n];
+    nel += T_beta[t] * xt;
+  }
+  return -2.0 * nel.real();
+}
+
+// update_mu (scf_common.cpp:147-202): bracket in steps of 0.2 from old_mu, then bisect until
+// |nelec - target| < mu_tol (at most 200 bisection steps).
+double orc_update_mu(int nt, int nw, const cd* Twt_ff, const cd* Ttw_ff, const cd* T_beta, const long* wn,
+                     double beta, int nk, int no, const cd* H0, const cd* F, const cd* S, const cd* Sigma,
+                     double nel_target, double old_mu, double mu_tol, double* nel_out) {
+  const long dim = long(nk) * no * no;
+  std::vector<cd> Sw(size_t(nw) * dim);
+  orc_transform(nw, nt, Twt_ff, dim, Sigma, Sw.data());
+  auto N = [&](double mu) { return orc_nelec(nt, nw, Ttw_ff, T_beta, wn, beta, nk, no, H0, F, S, Sw.data(), mu); };
+  const double delta = 0.2;
+  double nel = N(old_mu);
+  if (std::abs(nel - nel_target) < mu_tol) {
+    if (nel_out) *nel_out = nel;
+    return old_mu;
+  }
+  double mu1, mu2;
+  if (nel >= nel_target) {
+    mu2 = old_mu;
+    mu1 = old_mu - delta;
+    for (int it = 0; it < 1000 && N(mu1) > nel_target; ++it) mu1 -= delta;
+  } else {
+    mu1 = old_mu;
+    mu2 = old_mu + delta;
+    for (int it = 0; it < 1000 && N(mu2) < nel_target; ++it) mu2 += delta;
+  }
+  double mid = 0.5 * (mu1 + mu2);
+  nel = N(mid);
+  for (int it = 0; it < 200 && std::abs(nel - nel_target) >= mu_tol; ++it) {
+    if (nel >= nel_target) mu2 = mid;
+    else mu1 = mid;
+    mid = 0.5 * (mu1 + mu2);
+    nel = N(mid);
+  }
+  if (nel_out) *nel_out = nel;
+  return mid;
+}
+
 // k-space self-energy, thc_gw.icc:342-443 with isym = 0 and no time reversal:
 //   Sigma_PQ(tau, k) = -(1/N_k) sum_q G_PQ(tau, k - q) o W_PQ(tau, q)
 void orc_sigma_kspace(int nt, const int* kgrid, int nmu, int no, const cd* G, const cd* X,

@@ -225,6 +225,66 @@ def test_thc_hf_brute_force_convolution(orc, kgrid):
     assert np.einsum("kij,kji->", Dm, Fx).real < 0.0
 
 
+# ----------------------------------------------------------------- Dyson for G, mu search
+def fermi_sum(E, mu, beta=1000.0):
+    """2 sum_k w_k sum_b f(e_b - mu), w_k = 1/N_k."""
+    x = np.clip(beta * (E - mu), -700, 700)
+    return 2.0 * np.sum(1.0 / (1.0 + np.exp(x))) / E.shape[0]
+
+
+def test_dyson_g_free_particle(orc):
+    """Sigma = 0, F = H(k): the Matsubara Dyson equation returns update_G's closed-form G(tau)
+    and Dm = C f C^H (lambda = 1e4 keeps every |e| <= 1 Ha inside the IR window)."""
+    s = make_system((2, 1, 1), 7, 3, system_id=31)
+    tab = orc.load_ir(1000.0, None, table="1e4.1e-15")
+    Sig = np.zeros((tab["nt"], 1, s.nk, s.no, s.no), complex)
+    for mu in (0.0, 0.13):
+        G, Dm = orc.dyson_g(tab, s.H, Sig, mu)
+        G0 = orc.update_G(tab, s.C, s.E, mu)
+        assert rel(G, G0) < 1e-9
+        f = 1.0 / (1.0 + np.exp(np.clip(1000.0 * (s.E - mu), -700, 700)))
+        assert rel(Dm, np.einsum("kia,ka,kja->kij", s.C, f, s.C.conj())) < 1e-9
+
+
+def test_nelec_free_particle_and_mu(orc):
+    """Sigma = 0: nelec(mu) = 2 sum_k w_k sum_b f(e_b - mu); update_mu finds the mu where the
+    Fermi sum reaches the target."""
+    s = make_system((2, 1, 1), 7, 3, system_id=32)
+    tab = orc.load_ir(1000.0, None, table="1e4.1e-15")
+    Sig = np.zeros((tab["nt"], 1, s.nk, s.no, s.no), complex)
+    for mu in (-0.3, 0.0, 0.41):
+        assert abs(orc.nelec(tab, s.H, Sig, mu) - fermi_sum(s.E, mu)) < 1e-8
+    # a target between two levels: any mu in the gap gives it; check the found mu reproduces it
+    target = fermi_sum(s.E, 0.5 * (np.sort(s.E.ravel())[6] + np.sort(s.E.ravel())[7]))
+    mu, nel = orc.update_mu(tab, s.H, Sig, target, 0.0, mu_tol=1e-9)
+    assert abs(nel - target) < 1e-9 and abs(fermi_sum(s.E, mu) - target) < 1e-7
+
+
+def test_nelec_matches_eigen_spectra(orc):
+    """compute_Nelec with the reference's eigenvalue spectra of H + Sigma(iw_n)
+    (compute_eigenspectra, simple_dyson.cpp:130-172) equals the oracle's resolvent trace."""
+    s = make_system((2, 1, 1), 6, 3, system_id=33)
+    tab = orc.load_ir(1000.0, None, table="1e4.1e-15")
+    nt, nw = tab["nt"], tab["nw_f"]
+    # a causal self-energy: one pole per k at energy 0.3 with a Hermitian positive weight
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((s.nk, s.no, s.no)) + 1j * rng.standard_normal((s.nk, s.no, s.no))
+    Wt = 0.02 * B @ np.conj(np.swapaxes(B, 1, 2)) / s.no
+    tau, beta, ep = tab["tau"], 1000.0, 0.3
+    g = -np.exp(-ep * tau) / (1.0 + np.exp(-ep * beta))
+    Sig = (g[:, None, None, None] * Wt[None])[:, None]
+    Sw = orc.transform(tab["Twt_ff"], Sig)[:, 0]
+    for mu in (-0.1, 0.05):
+        Xw = np.zeros(nw, complex)
+        for n in range(nw):
+            z = mu + 1j * tab["wn_f"][n] * np.pi / beta
+            for k in range(s.nk):
+                lam = np.linalg.eigvals(s.H[k] + Sw[n, k])
+                Xw[n] += np.sum(1.0 / (z - lam)) / s.nk
+        ref = -2.0 * (tab["T_beta"] @ (tab["Ttw_ff"] @ Xw)).real
+        assert abs(orc.nelec(tab, s.H, Sig, mu) - ref) < 1e-9 * max(1.0, abs(ref))
+
+
 # ----------------------------------------------------------------- Dyson
 def test_dyson_closed_form_and_lapack(orc):
     rng = np.random.default_rng(5)
