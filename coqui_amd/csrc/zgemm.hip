@@ -132,7 +132,7 @@ struct TileIO {
 // FP64 accumulators and cleared, so rounding grows with the chunk length (16) instead of
 // with K.
 template <typename R, typename TA, typename TB, typename TC, int OPA, int OPB, int BM, int BN, bool FOLD = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : 1)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : (BN == 64 ? 2 : 1))))
 void gemm_kernel(Args p) {
   using AR = Arith<R>;
   using RC = typename AR::cplx;
