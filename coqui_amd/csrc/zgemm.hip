@@ -81,10 +81,16 @@ __device__ __forceinline__ RC ld_elem(const T* X, long ld, int r, int c, int R, 
 }
 
 // Tile of an operand: BX elements along its free dimension x (m for op(A), n for op(B))
-// times BK along k, staged in LDS as S[k][x].  ROWS: element (x, k) is stored at X[x][k]
+// times BK along k, staged in LDS as S[k][x ^ swz(k)].  ROWS: element (x, k) is stored at X[x][k]
 // (op(A) = A, op(B) = B^T or B^H); otherwise at X[k][x] (op(A) = A^T or A^H, op(B) = B).
 // CONJ conjugates on load.  Pairs of adjacent elements along the contiguous direction
 // are loaded together; each thread loads P = BX*BK/(2*NT) pairs.
+// Column swizzle of LDS row k: the K-contiguous stores of a thread group write one column
+// of several rows 1-2 KB apart, which would fall into one bank; XOR-ing the column with
+// (k/2) mod 8 spreads them.  The MFMA reads (16 consecutive columns of one row) stay a
+// permutation of the same aligned 16 columns.
+__device__ __forceinline__ int swz(int k) { return (k >> 1) & 7; }
+
 template <typename RC, typename T, bool ROWS, bool CONJ, int BX, int BK>
 struct TileIO {
   static constexpr int P = BX * BK / (2 * NT);
@@ -118,11 +124,11 @@ struct TileIO {
       int x, k;
       coords(threadIdx.x, p, x, k);
       if (ROWS) {
-        S[k][x] = r[2 * p];
-        S[k + 1][x] = r[2 * p + 1];
+        S[k][x ^ swz(k)] = r[2 * p];
+        S[k + 1][x ^ swz(k + 1)] = r[2 * p + 1];
       } else {
-        S[k][x] = r[2 * p];
-        S[k][x + 1] = r[2 * p + 1];
+        S[k][x ^ swz(k)] = r[2 * p];
+        S[k][(x + 1) ^ swz(k)] = r[2 * p + 1];
       }
     }
   }
@@ -214,14 +220,16 @@ void gemm_kernel(Args p) {
       R a_r[TM], a_i[TM], a_n[TM], b_r[TN], b_i[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
-        const RC v = As[cur][ks * 4 + lk][wm * (BM / 2) + i * 16 + lr];
+        const int kk = ks * 4 + lk;
+        const RC v = As[cur][kk][(wm * (BM / 2) + i * 16 + lr) ^ swz(kk)];
         a_r[i] = v.x;
         a_i[i] = v.y;
         a_n[i] = -v.y;
       }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
-        const RC v = Bs[cur][ks * 4 + lk][wn * (BN / 2) + j * 16 + lr];
+        const int kk = ks * 4 + lk;
+        const RC v = Bs[cur][kk][(wn * (BN / 2) + j * 16 + lr) ^ swz(kk)];
         b_r[j] = v.x;
         b_i[j] = v.y;
       }
