@@ -41,7 +41,23 @@ CONFIGS = {
     "cfg2": ((4, 4, 4), 52, 800, "1e3.1e-15"),
     "cfg1": ((2, 2, 2), 26, 200, "1e2.1e-15"),
     "cfg3": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
+    "cfg4": None,  # batched: see batch_systems()
 }
+BATCH_TABLE = "1e5.1e-06"  # 61 tau points, the nearest shipped table to N_tau = 60
+
+
+def batch_systems(n=24):
+    """The batched workload: n systems with N_orb ~ U{16..96}, N_mu = 8 N_orb and a k-grid of
+    2^3, 3^3 or 4^3 points, drawn from the generator's base seed; system i uses seed id i."""
+    from coqui_amd.system import BASE_SEED
+
+    rng = np.random.default_rng(BASE_SEED)
+    out = []
+    for i in range(n):
+        no = int(rng.integers(16, 97))
+        n1 = int(rng.choice([2, 3, 4]))
+        out.append(((n1, n1, n1), no, 8 * no, i))
+    return out
 
 
 # form: (flag 6 Dyson algorithm, flag 7 precision)
@@ -145,6 +161,7 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--gemm-tile", type=int, default=0, help="GEMM tile shape: 0 auto, 1 64x64, 2 128x64, 3 128x128")
+    ap.add_argument("--gemm-tile32", type=int, default=0, help="FP32 GEMM tile shape: 0 64x64, 1 128x64, 2 128x128")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
     ap.add_argument("--forms", default="plain,mixed",
@@ -166,6 +183,8 @@ def main():
 
     from coqui_amd import IAFT, ThcGW, make_system
 
+    if args.config == "cfg4":
+        return run_batch(args)
     kgrid, no, nmu, table = CONFIGS[args.config]
     beta = 1000.0
     log("bench: config %s kgrid=%s no=%d nmu=%d table=%s" % (args.config, kgrid, no, nmu, table))
@@ -175,6 +194,7 @@ def main():
     ia = IAFT.from_table(beta, table)
     ctx = ThcGW(kgrid, no, nmu, ia)
     ctx.set_flag(5, args.gemm_tile)
+    ctx.set_flag(8, args.gemm_tile32)
     Zd = torch.from_numpy(s.Z).cuda()
     Vd = torch.matmul(Zd, Zd.conj().transpose(1, 2)) / nmu  # input preparation, V = Z Z^H / N_mu
     del Zd
@@ -334,6 +354,112 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def run_batch(args):
+    """Batched form: the 24 systems of batch_systems() in mixed precision, one iteration of
+    all of them per step, replayed from one captured HIP graph; also timed one system after
+    another without the graph, and checked against a plain FP64 pass of the same batch."""
+    import torch
+
+    from coqui_amd import IAFT, ThcBatch, ThcGW, make_system
+
+    torch.cuda.set_device(0)
+    beta = 1000.0
+    ia = IAFT.from_table(beta, BATCH_TABLE)
+    specs = batch_systems()
+    need = sum(int(np.prod(k)) * (ia.nt_half + 2) * nmu * nmu * 16 for k, no, nmu, _ in specs)
+    log("bench: cfg4 batch of %d systems, %.1f GB of per-system arrays" % (len(specs), need / 1e9))
+    if need > 200e9:
+        print("bench.py: the cfg4 draw needs %.0f GB of per-system arrays" % (need / 1e9), file=sys.stderr)
+        sys.exit(2)
+    sysd = []
+    t0 = time.perf_counter()
+    for kgrid, no, nmu, sid in specs:
+        s = make_system(kgrid, no, nmu, system_id=sid, with_v=False)
+        ctx = ThcGW(kgrid, no, nmu, ia)
+        Zd = torch.from_numpy(s.Z).cuda()
+        Vd = torch.matmul(Zd, Zd.conj().transpose(1, 2)) / nmu
+        del Zd
+        d = dict(s=s, ctx=ctx, V=Vd, X=torch.from_numpy(s.X).cuda(), G=ctx.update_G(s.C, s.E, s.mu),
+                 Sig=torch.empty((ia.nt, 1, s.nk, no, no), dtype=torch.complex128, device="cuda"),
+                 W=torch.empty((s.nk, ia.nt_half, nmu, nmu), dtype=torch.complex128, device="cuda"))
+        sysd.append(d)
+    torch.cuda.synchronize()
+    log("bench: batch inputs ready in %.1f s" % (time.perf_counter() - t0))
+    lists = [[d[k] for d in sysd] for k in ("G", "X", "V", "Sig", "W")]
+    b = ThcBatch([d["ctx"] for d in sysd], use_graph=True, tune=True)  # from here on the scratch is shared
+
+    # plain FP64 reference pass, one system after another
+    ref = []
+    for d in sysd:
+        d["ctx"].set_flag(6, 0)
+        d["ctx"].set_flag(7, 0)
+        S, W, ec = d["ctx"].gw_iteration(d["G"], d["X"], d["V"], with_ec=True, hermitize=True, Sigma=d["Sig"],
+                                         W_qt=d["W"])
+        H = torch.from_numpy(d["s"].H).cuda()
+        Cd = torch.from_numpy(d["s"].C).cuda()
+        E = d["ctx"].solve_qp(S, Cd, H, d["s"].E, mu=d["s"].mu)
+        ref.append((S.clone(), W[0].clone(), ec, E))
+    for d in sysd:
+        d["ctx"].set_flag(7, 1)  # mixed precision
+
+    stream = torch.cuda.current_stream()
+
+    def timed(steps, graph):
+        b_set = b.lib.thcgw_batch_set(b.h, int(graph), 1)
+        assert b_set == 0
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ec = None
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(steps):
+            ec = b.gw_iteration(*lists)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / steps, ec
+
+    t0 = time.perf_counter()
+    b.gw_iteration(*lists)  # eager pass, tile tuning, capture and first replay
+    log("bench: batch warm-up (tuning + capture) %.1f s" % (time.perf_counter() - t0))
+    for _ in range(max(0, args.warmup - 1)):
+        b.gw_iteration(*lists)
+    ms_eager, _ = timed(max(1, args.steps), False)
+    ms_graph, ec = timed(args.steps, True)
+    errs = {"W_q0_rel": 0.0, "Sigma_rel": 0.0, "Ec_Ha": 0.0, "qp_meV": 0.0}
+    for d, (S0, W00, ec0, E0), e in zip(sysd, ref, ec):
+        H = torch.from_numpy(d["s"].H).cuda()
+        Cd = torch.from_numpy(d["s"].C).cuda()
+        E = d["ctx"].solve_qp(d["Sig"], Cd, H, d["s"].E, mu=d["s"].mu)
+        rel = lambda a, c: float((a - c).abs().max() / c.abs().max())
+        errs["W_q0_rel"] = max(errs["W_q0_rel"], rel(d["W"][0], W00))
+        errs["Sigma_rel"] = max(errs["Sigma_rel"], rel(d["Sig"], S0))
+        errs["Ec_Ha"] = max(errs["Ec_Ha"], abs(e - ec0))
+        errs["qp_meV"] = max(errs["qp_meV"], float(np.max(np.abs(E - E0)) / HA_PER_MEV))
+    out = {
+        "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
+        "value": round(ms_graph / 1000.0, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_graph, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 in/out, FP32 products, FP64 IR transforms and factorisations",
+        "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
+        "config": {"workload": "cfg4 batched mixed-precision G0W0 iteration of %d systems" % len(specs),
+                   "systems": [{"kgrid": list(k), "n_orb": no, "n_mu": nmu} for k, no, nmu, _ in specs],
+                   "ir_table": BATCH_TABLE, "n_tau": ia.nt, "beta": beta, "parallelism": "single GPU"},
+        "ms_per_step_one_by_one": round(ms_eager, 3),
+        "graph_speedup": round(ms_eager / ms_graph, 3),
+        "max_errors_vs_plain_fp64": errs,
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+    for d in sysd:
+        d["ctx"].close()
 
 
 if __name__ == "__main__":

@@ -6,7 +6,7 @@ include/thcgw.h).  This package holds the Python binding (thcgw.ThcGW), the IR t
 """
 from .iaft import IAFT, available_tables, determine_lambda  # noqa: F401
 from .system import System, make_system  # noqa: F401
-from .thcgw import EXPORTS, LIB_PATH, ThcGW, ThcgwError, load_library  # noqa: F401
+from .thcgw import EXPORTS, LIB_PATH, ThcBatch, ThcGW, ThcgwError, load_library  # noqa: F401
 
-__all__ = ["IAFT", "available_tables", "determine_lambda", "System", "make_system", "ThcGW", "ThcgwError",
+__all__ = ["IAFT", "available_tables", "determine_lambda", "System", "make_system", "ThcGW", "ThcBatch", "ThcgwError",
            "load_library", "EXPORTS", "LIB_PATH"]

@@ -49,11 +49,21 @@ struct thcgw_ctx {
   int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky)
   int precision = 0;    // 0 = FP64 throughout, 1 = mixed (FP32 products, FP64 factorisations and IR),
                         // 2 = mixed with split-BF16 products in place of FP32 MFMA
-  int pf() const { return precision == 2 ? thcgw::GEMM_SPLIT_BF16 : 0; }  // cgemm precision flags
+  int tile32 = -1;      // FP32 tile shape chosen for this context (-1: process default)
+  bool deferred = false;  // enqueue without host synchronisation (stream capture): checks run later
+  std::map<std::string, DevBuf>* pool = nullptr;  // scratch shared with the other systems of a batch
+  // cgemm flags of this context: precision and FP32 tile shape
+  int pf() const {
+    return (precision == 2 ? thcgw::GEMM_SPLIT_BF16 : 0) | (tile32 >= 0 ? (tile32 + 1) << thcgw::GEMM_TILE32_SHIFT : 0);
+  }
+  // Arrays read back after the stream completes stay private to the context.
+  static bool private_buf(const std::string& name) {
+    return name == "dy_info" || name == "dy_vinfo" || name == "ec_tr" || name == "ec_ld";
+  }
 
   template <class T>
   T* buf(const std::string& name, size_t count) {
-    DevBuf& b = ws[name];
+    DevBuf& b = (pool && !private_buf(name)) ? (*pool)[name] : ws[name];
     const size_t need = count * sizeof(T);
     if (b.bytes < need) {
       if (b.p) THCGW_HIP(hipFree(b.p));
@@ -83,6 +93,32 @@ struct thcgw_ctx {
     d_tau = nullptr;
     d_Tplus = d_Tminus = d_Twt_ff = d_Ttw_ff = d_Tbeta = nullptr;
     d_wn_f = nullptr;
+  }
+};
+
+// A batch of independent systems (uneven sizes), each with its own context.  The systems
+// run one after another on one stream and share one scratch pool; after a first eager
+// iteration (which allocates the scratch, tunes each mixed-precision system's FP32 tile
+// shape and keeps the chunk sizes) the whole batch iteration is captured into one HIP
+// graph and replayed.
+struct thcgw_batch {
+  std::vector<thcgw_ctx*> ctxs;
+  std::map<std::string, DevBuf> pool;
+  bool warmed = false;
+  bool use_graph = true;
+  bool tune = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> sig;  // pointers the graph was captured with
+  int herm = -1;
+  int device = 0;
+  std::string err;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    sig.clear();
   }
 };
 
@@ -297,6 +333,7 @@ bool factor_v(thcgw_ctx* ctx, const c128* V, c128* Lv, hipStream_t s) {
   thcgw::copy_batch(s, V, M, nq, Lv, M, M, 0, nq);
   thcgw::potrf_batched(s, Lv, M, n, n, info, nq);
   thcgw::zero_upper(s, Lv, M, n, nq);
+  if (ctx->deferred) return true;  // checked after the stream completes
   return !any_nonzero(info, nq, s);
 }
 
@@ -314,7 +351,7 @@ bool dyson_chunk_hpd(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Lv, i
                kZero, Mb, M, n, cb, kOne, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW);
   if (d_trace) thcgw::one_minus_trace(s, Mb, M, n, d_trace + b0, cb);
   thcgw::potrf_batched(s, Mb, M, n, n, info, cb);
-  if (any_nonzero(info, cb, s)) return false;
+  if (!ctx->deferred && any_nonzero(info, cb, s)) return false;
   if (d_ldet) thcgw::chol_logdet(s, Mb, M, n, d_ldet + b0, cb);
   if (solve) {
     thcgw::copy_batch(s, Lv, M, nq, Z, M, M, b0, cb);
@@ -329,24 +366,28 @@ bool dyson_chunk_hpd(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Lv, i
 // The Hermitian form with the products in FP32 (L, C and K stored as complex<float>) and
 // M, its Cholesky factor and ln det in FP64.
 bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv64, int b0, int cb, c128* d_trace,
-                         double* d_ldet, bool solve, c128* Mb, c64* Z, int* info, hipStream_t s) {
+                         double* d_ldet, bool solve, c128* Mb, c64* Z, c64* Mb64, int* info, hipStream_t s) {
   const int n = ctx->d.nmu, nq = ctx->nk;
   const long M = ctx->M;
   c128* P = PiW + static_cast<long>(b0) * M;
-  // M feeds ln det: both products keep FP64 sums across K chunks
-  thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(P, M), DT_C128, n, thcgw::bp(Lv64, M, nq, 1, b0),
-               DT_C64, n, kZero, Z, DT_C64, M, n, cb, kZero,
+  // Pi rounded once to complex<float> (the FP32 product rounds it on load anyway), which
+  // halves the bytes of its repeated tile reads; M feeds ln det, so both products keep
+  // FP64 sums across K chunks.
+  thcgw::to_c64(s, P, Mb64, static_cast<long>(cb) * M);
+  thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(static_cast<const c64*>(Mb64), M), DT_C64, n,
+               thcgw::bp(Lv64, M, nq, 1, b0), DT_C64, n, kZero, Z, DT_C64, M, n, cb, kZero,
                thcgw::GEMM_TRI_KFROM_COL | thcgw::GEMM_FOLD64 | ctx->pf());
   thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(Lv64, M, nq, 1, b0), DT_C64, n,
                thcgw::bp(Z, M), DT_C64, n, kZero, Mb, DT_C128, M, n, cb, kOne,
                thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW | thcgw::GEMM_FOLD64 | ctx->pf());
   if (d_trace) thcgw::one_minus_trace(s, Mb, M, n, d_trace + b0, cb);
   thcgw::potrf_batched(s, Mb, M, n, n, info, cb);
-  if (any_nonzero(info, cb, s)) return false;
+  if (!ctx->deferred && any_nonzero(info, cb, s)) return false;
   if (d_ldet) thcgw::chol_logdet(s, Mb, M, n, d_ldet + b0, cb);
   if (solve) {
     thcgw::copy_batch(s, Lv64, M, nq, Z, M, M, b0, cb);
-    thcgw::trsm_right_lower_conj_f32(s, Mb, M, n, Z, M, n, n, n, cb, ctx->pf());
+    thcgw::to_c64(s, Mb, Mb64, static_cast<long>(cb) * M);  // L_M rounded once for the FP32 solve
+    thcgw::trsm_right_lower_conj_f32(s, static_cast<const c64*>(Mb64), M, n, Z, M, n, n, n, cb, ctx->pf());
     thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), DT_C64, n, thcgw::bp(Z, M), DT_C64, n,
                  kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER | ctx->pf());
     thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
@@ -382,12 +423,15 @@ void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* 
   }
   c128* Z = nullptr;
   c64* Z64 = nullptr;
+  c64* Mb64 = nullptr;
   int* ipiv = nullptr;
   for (int b0 = 0; b0 < total; b0 += CH) {
     const int cb = std::min(CH, total - b0);
     if (hpd && mixed) {
       if (!Z64) Z64 = ctx->buf<c64>("dy_z64", static_cast<size_t>(CH) * M);
-      if (dyson_chunk_hpd_f32(ctx, PiW, V, Lv64, b0, cb, d_trace, d_ldet, solve, LU, Z64, info_all + b0, s)) continue;
+      if (!Mb64) Mb64 = ctx->buf<c64>("dy_m64", static_cast<size_t>(CH) * M);
+      if (dyson_chunk_hpd_f32(ctx, PiW, V, Lv64, b0, cb, d_trace, d_ldet, solve, LU, Z64, Mb64, info_all + b0, s))
+        continue;
     } else if (hpd) {
       if (!Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
       if (dyson_chunk_hpd(ctx, PiW, V, Lv, b0, cb, d_trace, d_ldet, solve, LU, Z, info_all + b0, s)) continue;
@@ -396,6 +440,7 @@ void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* 
     if (solve && !Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
     dyson_chunk_lu(ctx, PiW, V, b0, cb, d_trace, d_ldet, solve, pi_first, LU, Z, ipiv, info_all + b0, s);
   }
+  if (ctx->deferred) return;  // check_deferred() reads the flags after the stream completes
   std::vector<int> info(total);
   THCGW_HIP(hipMemcpyAsync(info.data(), info_all, sizeof(int) * total, hipMemcpyDeviceToHost, s));
   THCGW_HIP(hipStreamSynchronize(s));
@@ -494,6 +539,54 @@ void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c1
     if (c2 > 0) project_a2p(ctx, s, X, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
   }
   if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
+}
+
+// ---- one iteration as enqueue + finish (the batch captures the enqueue part) -----------
+
+void enqueue_iteration(thcgw_ctx* ctx, const c128* G, const c128* X, const c128* V, c128* Sig, c128* Wqt, bool want_ec,
+                       bool herm, hipStream_t s) {
+  const long M = ctx->M;
+  const int total = ctx->nw_half * ctx->nk;
+  c128* Pit = Wqt;  // W_qtPQ doubles as Pi_tqPQ scratch (same size)
+  c128* Piw = ctx->buf<c128>("it_piw", static_cast<size_t>(total) * M);
+  c128* dtr = want_ec ? ctx->buf<c128>("ec_tr", total) : nullptr;
+  double* dld = want_ec ? ctx->buf<double>("ec_ld", total) : nullptr;
+  run_pi(ctx, G, X, Pit, s);
+  run_tau_to_w(ctx, Pit, Piw, s);
+  run_dyson(ctx, Piw, V, dtr, dld, true, false, s);
+  run_w_to_tau(ctx, Piw, Wqt, s);
+  run_sigma(ctx, G, Wqt, X, Sig, herm, s);
+}
+
+// After a deferred enqueue has completed: Cholesky / LU flags of the Dyson step.
+void check_deferred(thcgw_ctx* ctx, hipStream_t s) {
+  const int total = ctx->nw_half * ctx->nk, nq = ctx->nk;
+  if (ctx->precision != 0 || ctx->dyson_algo == 1) {
+    std::vector<int> vi(nq);
+    THCGW_HIP(hipMemcpyAsync(vi.data(), ctx->buf<int>("dy_vinfo", static_cast<size_t>(nq)), sizeof(int) * nq,
+                             hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < nq; ++q)
+      if (vi[q] != 0)
+        throw thcgw::Error(THCGW_ERR_SINGULAR, "batch: V_q is not positive definite at q = " + std::to_string(q) +
+                                                   " (the captured Hermitian form has no LU fallback)");
+  }
+  std::vector<int> info(total);
+  THCGW_HIP(hipMemcpyAsync(info.data(), ctx->buf<int>("dy_info", static_cast<size_t>(total)), sizeof(int) * total,
+                           hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < total; ++b)
+    if (info[b] != 0)
+      throw thcgw::Error(THCGW_ERR_SINGULAR, "batch: Dyson factorisation failed at (n, q) = (" + std::to_string(b / nq) +
+                                                 ", " + std::to_string(b % nq) + ")");
+}
+
+double finish_ec(thcgw_ctx* ctx, hipStream_t s) {
+  const int total = ctx->nw_half * ctx->nk;
+  std::vector<std::complex<double>> tr;
+  std::vector<double> ld;
+  fetch_ec_terms(ctx, ctx->buf<c128>("ec_tr", total), ctx->buf<double>("ec_ld", total), tr, ld, s);
+  return assemble_ec(ctx, tr, ld);
 }
 
 // ---- Dyson equation for G on the Matsubara axis ------------------------------------
@@ -659,7 +752,10 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     else if (flag == 3) ctx->dyson_chunk = static_cast<int>(value);
     else if (flag == 4) thcgw::gemm_profile_enable(value != 0);
     else if (flag == 5) thcgw::gemm_set_tile_mode(static_cast<int>(value));
-    else if (flag == 6) {
+    else if (flag == 8) {
+      THCGW_REQUIRE(value >= 0 && value <= 2, "flag 8 (FP32 tile) takes 0, 1 or 2");
+      thcgw::gemm_set_tile_mode32(static_cast<int>(value));
+    } else if (flag == 6) {
       THCGW_REQUIRE(value == 0 || value == 1, "flag 6 (Dyson algorithm) takes 0 (LU) or 1 (Hermitian)");
       ctx->dyson_algo = static_cast<int>(value);
     } else if (flag == 7) {
@@ -794,23 +890,8 @@ int thcgw_gw_iteration(thcgw_ctx* ctx, const double* G, const double* X, const d
   return guarded(ctx, [&] {
     THCGW_REQUIRE(G && X && V && Sig && Wqt, "thcgw_gw_iteration: null pointer");
     hipStream_t s = S(stream);
-    const long M = ctx->M;
-    const int total = ctx->nw_half * ctx->nk;
-    c128* Pit = Cm(Wqt);  // W_qtPQ doubles as Pi_tqPQ scratch (same size)
-    c128* Piw = ctx->buf<c128>("it_piw", static_cast<size_t>(total) * M);
-    c128* dtr = Ec ? ctx->buf<c128>("ec_tr", total) : nullptr;
-    double* dld = Ec ? ctx->buf<double>("ec_ld", total) : nullptr;
-    run_pi(ctx, C(G), C(X), Pit, s);
-    run_tau_to_w(ctx, Pit, Piw, s);
-    run_dyson(ctx, Piw, C(V), dtr, dld, true, false, s);
-    run_w_to_tau(ctx, Piw, Cm(Wqt), s);
-    run_sigma(ctx, C(G), Cm(Wqt), C(X), Cm(Sig), herm != 0, s);
-    if (Ec) {
-      std::vector<std::complex<double>> tr;
-      std::vector<double> ld;
-      fetch_ec_terms(ctx, dtr, dld, tr, ld, s);
-      *Ec = assemble_ec(ctx, tr, ld);
-    }
+    enqueue_iteration(ctx, C(G), C(X), C(V), Cm(Sig), Cm(Wqt), Ec != nullptr, herm != 0, s);
+    if (Ec) *Ec = finish_ec(ctx, s);
     THCGW_HIP(hipStreamSynchronize(s));
   });
 }
@@ -936,6 +1017,169 @@ int thcgw_update_mu(thcgw_ctx* ctx, const double* H0, const double* F, const dou
     *mu_out = mu;
     if (nel_out) *nel_out = nel;
   });
+}
+
+int thcgw_batch_create(int nsys, thcgw_ctx* const* ctxs, thcgw_batch** out) {
+  g_create_error.clear();
+  if (nsys < 1 || !ctxs || !out) {
+    g_create_error = "thcgw_batch_create: bad argument";
+    return THCGW_ERR_ARG;
+  }
+  *out = nullptr;
+  thcgw_batch* b = new (std::nothrow) thcgw_batch();
+  if (!b) return THCGW_ERR_ALLOC;
+  for (int i = 0; i < nsys; ++i) {
+    if (!ctxs[i] || (i > 0 && ctxs[i]->device != ctxs[0]->device)) {
+      g_create_error = "thcgw_batch_create: null context or contexts on different devices";
+      delete b;
+      return THCGW_ERR_ARG;
+    }
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) {
+        g_create_error = "thcgw_batch_create: a context appears twice";
+        delete b;
+        return THCGW_ERR_ARG;
+      }
+  }
+  b->ctxs.assign(ctxs, ctxs + nsys);
+  b->device = ctxs[0]->device;
+  for (thcgw_ctx* c : b->ctxs) {  // scratch moves to the shared pool
+    for (auto& kv : c->ws)
+      if (!thcgw_ctx::private_buf(kv.first) && kv.second.p) (void)hipFree(kv.second.p);
+    for (auto it = c->ws.begin(); it != c->ws.end();)
+      it = thcgw_ctx::private_buf(it->first) ? std::next(it) : c->ws.erase(it);
+    c->pool = &b->pool;
+  }
+  *out = b;
+  return THCGW_OK;
+}
+
+void thcgw_batch_destroy(thcgw_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  b->drop_graph();
+  for (thcgw_ctx* c : b->ctxs) c->pool = nullptr;
+  for (auto& kv : b->pool)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  delete b;
+}
+
+const char* thcgw_batch_last_error(const thcgw_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int thcgw_batch_set(thcgw_batch* b, int use_graph, int tune) {
+  if (!b) return THCGW_ERR_ARG;
+  b->use_graph = use_graph != 0;
+  b->tune = tune != 0;
+  if (!b->use_graph) b->drop_graph();
+  return THCGW_OK;
+}
+
+int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const double* const* X, const double* const* V,
+                             double* const* Sig, double* const* Wqt, double* Ec, int herm, void* stream) {
+  if (!b) return THCGW_ERR_ARG;
+  const int n = static_cast<int>(b->ctxs.size());
+  try {
+    THCGW_HIP(hipSetDevice(b->device));
+    THCGW_REQUIRE(G && X && V && Sig && Wqt, "thcgw_batch_gw_iteration: null pointer array");
+    for (int i = 0; i < n; ++i)
+      THCGW_REQUIRE(G[i] && X[i] && V[i] && Sig[i] && Wqt[i], "thcgw_batch_gw_iteration: null pointer");
+    hipStream_t s = S(stream);
+    const bool want_ec = Ec != nullptr;
+    if (!b->warmed) {
+      // eager pass: allocates the shared scratch at its largest size, fixes the chunk sizes
+      for (int i = 0; i < n; ++i) {
+        thcgw_ctx* c = b->ctxs[i];
+        c->pair_chunk = c->pair_chunk > 0 ? c->pair_chunk : choose_pair_chunk(c, 4);
+        c->dyson_chunk = c->dyson_chunk > 0 ? c->dyson_chunk : choose_dyson_chunk(c, c->nw_half * c->nk);
+        enqueue_iteration(c, C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), true, herm != 0, s);
+        finish_ec(c, s);
+      }
+      // tile tuning: the FP32 tile shape per mixed-precision system, timed on a full iteration
+      if (b->tune) {
+        hipEvent_t e0, e1;
+        THCGW_HIP(hipEventCreate(&e0));
+        THCGW_HIP(hipEventCreate(&e1));
+        for (int i = 0; i < n; ++i) {
+          thcgw_ctx* c = b->ctxs[i];
+          if (c->precision == 0) continue;
+          float best = 1e30f;
+          int pick = 0;
+          for (int t = 0; t < 3; ++t) {
+            c->tile32 = t;
+            THCGW_HIP(hipEventRecord(e0, s));
+            enqueue_iteration(c, C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), true, herm != 0, s);
+            THCGW_HIP(hipEventRecord(e1, s));
+            THCGW_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            THCGW_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+              best = ms;
+              pick = t;
+            }
+          }
+          c->tile32 = pick;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+      }
+      b->warmed = true;
+    }
+    std::vector<const void*> sig;
+    for (int i = 0; i < n; ++i) {
+      sig.push_back(G[i]);
+      sig.push_back(X[i]);
+      sig.push_back(V[i]);
+      sig.push_back(Sig[i]);
+      sig.push_back(Wqt[i]);
+    }
+    sig.push_back(stream);
+    sig.push_back(reinterpret_cast<const void*>(static_cast<long>(want_ec)));
+    if (b->use_graph) {
+      if (!b->exec || sig != b->sig || b->herm != herm) {
+        b->drop_graph();
+        for (thcgw_ctx* c : b->ctxs) c->deferred = true;
+        hipError_t ce = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        if (ce != hipSuccess) {
+          for (thcgw_ctx* c : b->ctxs) c->deferred = false;
+          THCGW_HIP(ce);
+        }
+        try {
+          for (int i = 0; i < n; ++i)
+            enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0, s);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          for (thcgw_ctx* c : b->ctxs) c->deferred = false;
+          throw;
+        }
+        for (thcgw_ctx* c : b->ctxs) c->deferred = false;
+        THCGW_HIP(hipStreamEndCapture(s, &b->graph));
+        THCGW_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->sig = sig;
+        b->herm = herm;
+      }
+      THCGW_HIP(hipGraphLaunch(b->exec, s));
+      THCGW_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i < n; ++i) {
+        check_deferred(b->ctxs[i], s);
+        if (want_ec) Ec[i] = finish_ec(b->ctxs[i], s);
+      }
+    } else {
+      for (int i = 0; i < n; ++i) {
+        enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0, s);
+        if (want_ec) Ec[i] = finish_ec(b->ctxs[i], s);
+      }
+      THCGW_HIP(hipStreamSynchronize(s));
+    }
+    return THCGW_OK;
+  } catch (const thcgw::Error& e) {
+    b->err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return THCGW_ERR_ARG;
+  }
 }
 
 int thcgw_gm_energy(thcgw_ctx* ctx, const double* G, const double* Sigma, double* e_corr, void* stream) {

@@ -241,23 +241,35 @@ void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128*
   trsm_right_lower_conj(s, L + static_cast<long>(n1) * ldl + n1, sL, ldl, T + n1, sT, ldt, m, n2, batch);
 }
 
-// The same in FP32 arithmetic with T stored as complex<float> and L as complex<double>;
-// gemm_flags selects the product of the updates (0: FP32 MFMA, GEMM_SPLIT_BF16).
-void trsm_right_lower_conj_f32(hipStream_t s, const c128* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
-                               int nb, int batch, int gemm_flags) {
+// The same in FP32 arithmetic with T stored as complex<float> and L as complex<double> or
+// complex<float>; gemm_flags selects the product of the updates (0: FP32 MFMA,
+// GEMM_SPLIT_BF16).
+template <typename TL>
+void trsm_rlc_f32(hipStream_t s, const TL* L, long sL, int ldl, c64* T, long sT, int ldt, int m, int nb, int batch,
+                  int gemm_flags) {
   if (m <= 0 || nb <= 0 || batch <= 0) return;
   if (nb <= CB) {
-    hipLaunchKernelGGL((trsm_rlc_leaf_kernel<c64, c64, c128>), dim3((m + 255) / 256, batch), dim3(256), 0, s, L, sL,
+    hipLaunchKernelGGL((trsm_rlc_leaf_kernel<c64, c64, TL>), dim3((m + 255) / 256, batch), dim3(256), 0, s, L, sL,
                        ldl, T, sT, ldt, m, nb);
     THCGW_HIP(hipGetLastError());
     return;
   }
   const int n1 = split_point(nb, CB), n2 = nb - n1;
-  trsm_right_lower_conj_f32(s, L, sL, ldl, T, sT, ldt, m, n1, batch, gemm_flags);
+  trsm_rlc_f32(s, L, sL, ldl, T, sT, ldt, m, n1, batch, gemm_flags);
   cgemm(s, OP_N, OP_C, m, n2, n1, kMinusOne, bp(T, sT), DT_C64, ldt, bp(L + static_cast<long>(n1) * ldl, sL),
-        DT_C128, ldl, kOne, T + n1, DT_C64, sT, ldt, batch, Cplx{0.0, 0.0}, gemm_flags);
-  trsm_right_lower_conj_f32(s, L + static_cast<long>(n1) * ldl + n1, sL, ldl, T + n1, sT, ldt, m, n2, batch,
-                            gemm_flags);
+        sizeof(TL) == sizeof(c128) ? DT_C128 : DT_C64, ldl, kOne, T + n1, DT_C64, sT, ldt, batch, Cplx{0.0, 0.0},
+        gemm_flags);
+  trsm_rlc_f32(s, L + static_cast<long>(n1) * ldl + n1, sL, ldl, T + n1, sT, ldt, m, n2, batch, gemm_flags);
+}
+
+void trsm_right_lower_conj_f32(hipStream_t s, const c128* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
+                               int nb, int batch, int gemm_flags) {
+  trsm_rlc_f32(s, L, sL, ldl, T, sT, ldt, m, nb, batch, gemm_flags);
+}
+
+void trsm_right_lower_conj_f32(hipStream_t s, const c64* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
+                               int nb, int batch, int gemm_flags) {
+  trsm_rlc_f32(s, L, sL, ldl, T, sT, ldt, m, nb, batch, gemm_flags);
 }
 
 // In-place lower Cholesky A = L L^H of `batch` Hermitian n x n matrices (lower triangle read).

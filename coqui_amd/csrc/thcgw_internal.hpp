@@ -75,6 +75,9 @@ constexpr int GEMM_FOLD64 = 8;
 // cgemm only: split-BF16 products (three bf16 parts per value, six part products, FP32
 // accumulation) instead of FP32 MFMA.
 constexpr int GEMM_SPLIT_BF16 = 16;
+// cgemm only: bits 6-7 hold (FP32 tile shape + 1) for products without FP64 folding
+// (0 = the process default, see gemm_set_tile_mode32).
+constexpr int GEMM_TILE32_SHIFT = 6;
 
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I   (row-major, M x N)
 void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
@@ -96,7 +99,8 @@ struct GemmStats {
 GemmStats gemm_stats();
 void gemm_stats_reset();
 void gemm_profile_enable(bool on);
-void gemm_set_tile_mode(int mode);  // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
+void gemm_set_tile_mode(int mode);    // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
+void gemm_set_tile_mode32(int mode);  // FP32 products without FP64 folding: 0 = 64x64, 1 = 128x64, 2 = 128x128
 constexpr int GEMM_PROFILE_CLASSES = 17;
 void gemm_profile_read(double* ms, double* flops, long* launches);
 
@@ -126,6 +130,8 @@ void potrf_batched(hipStream_t s, c128* A, long sA, int lda, int n, int* info, i
 void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* T, long sT, int ldt, int m, int nb,
                            int batch);
 void trsm_right_lower_conj_f32(hipStream_t s, const c128* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
+                               int nb, int batch, int gemm_flags = 0);
+void trsm_right_lower_conj_f32(hipStream_t s, const c64* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
                                int nb, int batch, int gemm_flags = 0);
 void zero_upper(hipStream_t s, c128* A, long sA, int n, int batch);
 void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch);

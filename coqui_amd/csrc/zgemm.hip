@@ -138,7 +138,8 @@ struct TileIO {
 // FP64 accumulators and cleared, so rounding grows with the chunk length (16) instead of
 // with K.
 template <typename R, typename TA, typename TB, typename TC, int OPA, int OPB, int BM, int BN, bool FOLD = false>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : (BN == 64 ? 2 : 1))))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
+    BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : (BN == 64 || sizeof(R) == 4 ? 2 : 1))))
 void gemm_kernel(Args p) {
   using AR = Arith<R>;
   using RC = typename AR::cplx;
@@ -605,13 +606,24 @@ void launch(hipStream_t s, Args a, int shape) {
     hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 128, 128>), g, t, 0, s, a);
 }
 
+std::atomic<int> g_tile_mode32{0};  // FP32 tile: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128
+
 template <typename TA, typename TB, typename TC, int OPA, int OPB>
 void launch_f32(hipStream_t s, Args a) {
-  a.tiles_m = (a.M + 63) / 64;
-  a.tiles_n = (a.N + 63) / 64;
+  const int tb = (a.tri >> GEMM_TILE32_SHIFT) & 3;
+  const int mode = ((a.tri & (GEMM_SPLIT_BF16 | GEMM_FOLD64)) == 0) ? (tb ? tb - 1 : g_tile_mode32.load()) : 0;
+  const int bm = mode == 0 ? 64 : 128, bn = mode == 2 ? 128 : 64;
+  a.tiles_m = (a.M + bm - 1) / bm;
+  a.tiles_n = (a.N + bn - 1) / bn;
   const long nblocks = static_cast<long>(a.batch) * a.tiles_m * a.tiles_n;
   THCGW_REQUIRE(nblocks < (1L << 31), "cgemm: grid too large");
-  if (a.tri & GEMM_SPLIT_BF16) {
+  if (mode == 1) {
+    hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 128, 64>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(NT), 0, s, a);
+  } else if (mode == 2) {
+    hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 128, 128>), dim3(static_cast<unsigned>(nblocks)),
+                       dim3(NT), 0, s, a);
+  } else if (a.tri & GEMM_SPLIT_BF16) {
     if (a.tri & GEMM_FOLD64)
       hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, OPA, OPB, true>), dim3(static_cast<unsigned>(nblocks)),
                          dim3(NT), 0, s, a);
@@ -755,6 +767,7 @@ void gemm_stats_reset() {
 }
 
 void gemm_set_tile_mode(int mode) { g_tile_mode = mode; }
+void gemm_set_tile_mode32(int mode) { g_tile_mode32 = mode; }
 
 void gemm_profile_enable(bool on) {
   std::lock_guard<std::mutex> lk(g_prof_mu);

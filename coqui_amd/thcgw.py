@@ -21,7 +21,8 @@ EXPORTS = (
     "thcgw_aux_to_primary", "thcgw_pi", "thcgw_tau_to_w", "thcgw_dyson_w", "thcgw_w_to_tau", "thcgw_sigma",
     "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
     "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf", "thcgw_dyson_g",
-    "thcgw_nelec", "thcgw_update_mu",
+    "thcgw_nelec", "thcgw_update_mu", "thcgw_batch_create", "thcgw_batch_destroy", "thcgw_batch_last_error",
+    "thcgw_batch_set", "thcgw_batch_gw_iteration",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -77,6 +78,12 @@ def load_library(path=LIB_PATH):
         "thcgw_dyson_g": (i, [p, p, p, p, p, d, p, p, p]),
         "thcgw_nelec": (i, [p, p, p, p, p, d, ctypes.POINTER(d), p]),
         "thcgw_update_mu": (i, [p, p, p, p, p, d, d, d, ctypes.POINTER(d), ctypes.POINTER(d), p]),
+        "thcgw_batch_create": (i, [i, ctypes.POINTER(p), ctypes.POINTER(p)]),
+        "thcgw_batch_destroy": (None, [p]),
+        "thcgw_batch_last_error": (ctypes.c_char_p, [p]),
+        "thcgw_batch_set": (i, [p, i, i]),
+        "thcgw_batch_gw_iteration": (i, [p, ctypes.POINTER(p), ctypes.POINTER(p), ctypes.POINTER(p),
+                                         ctypes.POINTER(p), ctypes.POINTER(p), ctypes.POINTER(d), i, p]),
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
         "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
@@ -316,3 +323,54 @@ class ThcGW:
         e = ctypes.c_double()
         self._check(self.lib.thcgw_gm_energy(self.h, _ptr(G), _ptr(Sigma), ctypes.byref(e), self._stream(stream)))
         return e.value
+
+
+class ThcBatch:
+    """Independent systems (one ThcGW each, uneven sizes) iterated together: thcgw_batch_*.
+
+    The contexts share one scratch pool while the batch lives; close the batch before
+    its contexts.  The first iteration runs eagerly (and tunes the FP32 tile shape of the
+    mixed-precision systems); later ones replay one captured HIP graph."""
+
+    def __init__(self, contexts, use_graph=True, tune=True):
+        self.ctxs = list(contexts)
+        self.lib = load_library()
+        self._torch = self.ctxs[0]._torch
+        arr = (ctypes.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = ctypes.c_void_p()
+        rc = self.lib.thcgw_batch_create(len(self.ctxs), arr, ctypes.byref(h))
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_batch_last_error(None).decode())
+        self.h = h
+        self._check(self.lib.thcgw_batch_set(self.h, int(bool(use_graph)), int(bool(tune))))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.thcgw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gw_iteration(self, G, X, V, Sigma, W_qt, with_ec=True, hermitize=True, stream=None):
+        """One iteration of every system; G..W_qt are lists of device tensors (one per system).
+        Returns the list of Ec values (or None)."""
+        n = len(self.ctxs)
+        for lst in (G, X, V, Sigma, W_qt):
+            assert len(lst) == n, "one array per system"
+        for c, g, x, v, sg, w in zip(self.ctxs, G, X, V, Sigma, W_qt):
+            for t in (g, x, v, sg, w):
+                c._dev(t)  # type and layout checks (no copy for device tensors)
+        mk = lambda lst: (ctypes.c_void_p * n)(*[t.data_ptr() for t in lst])
+        ec = (ctypes.c_double * n)() if with_ec else None
+        s = stream if stream is not None else self._torch.cuda.current_stream()
+        self._check(self.lib.thcgw_batch_gw_iteration(self.h, mk(G), mk(X), mk(V), mk(Sigma), mk(W_qt), ec,
+                                                       int(bool(hermitize)), ctypes.c_void_p(s.cuda_stream)))
+        return list(ec) if with_ec else None

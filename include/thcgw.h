@@ -194,6 +194,28 @@ int thcgw_update_mu(thcgw_ctx* ctx, const double* H0_skij, const double* F_skij,
                     const double* Sigma_tskij, double nelec_target, double old_mu, double mu_tol, double* mu_out,
                     double* nelec_out, void* stream);
 
+/* Batched form (no counterpart in the reference, which runs one system per job): a batch of
+ * independent systems of uneven size, one context each, iterated together on one stream.
+ * The contexts keep their own flags (precision, Dyson form) and share one scratch pool
+ * (they must live on the same device and may not be used outside the batch meanwhile).
+ * The first thcgw_batch_gw_iteration runs eagerly: it allocates the scratch, fixes the
+ * chunk sizes and, unless switched off, times the three FP32 tile shapes on every
+ * mixed-precision system and keeps the fastest.  Later calls capture the whole batch
+ * iteration into one HIP graph (recaptured when any pointer, the stream or hermitize
+ * changes) and replay it; the Cholesky/LU flags and Ec are read after the replay, and
+ * the captured Hermitian form has no LU fallback (a failure is reported as
+ * THCGW_ERR_SINGULAR).  G[i], X[i], V[i], Sigma[i], W_qt[i] are the per-system arrays of
+ * thcgw_gw_iteration; Ec (nsys doubles) may be NULL.  Synchronises the stream.  Destroy
+ * the batch before its contexts. */
+typedef struct thcgw_batch thcgw_batch;
+int thcgw_batch_create(int nsys, thcgw_ctx* const* ctxs, thcgw_batch** out);
+void thcgw_batch_destroy(thcgw_batch* b);
+const char* thcgw_batch_last_error(const thcgw_batch* b);
+int thcgw_batch_set(thcgw_batch* b, int use_graph, int tune);
+int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G_tskij, const double* const* X_skPa,
+                             const double* const* V_qPQ, double* const* Sigma_tskij, double* const* W_qtPQ,
+                             double* Ec, int hermitize, void* stream);
+
 /* Diagnostics (no counterpart in the reference): bytes of device workspace held by ctx,
  * control flags and GEMM launch timing.
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;
@@ -205,6 +227,8 @@ int thcgw_update_mu(thcgw_ctx* ctx, const double* H0_skij, const double* F_skij,
  *           default), 1 = Hermitian form V = L L^H, M = I - L^H Pi L = L_M L_M^H (two
  *           Cholesky factorisations, triangular products; valid for Hermitian positive
  *           definite V and Hermitian negative semidefinite Pi, with per-chunk fallback to LU);
+ *   flag 8: FP32 GEMM tile shape (process-wide; products without FP64 chunk accumulation):
+ *           0 = 64x64, 1 = 128x64, 2 = 128x128;
  *   flag 7: precision: 0 = FP64 throughout (default); 1 = mixed: the projections X G X^H and
  *           X^H Sigma X, the k-grid transforms with the Hadamard products, and the products
  *           of the Hermitian Dyson form run in FP32 (complex<float> intermediates), while
