@@ -43,7 +43,11 @@ CONFIGS = {
     "cfg3": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
     "cfg4": None,  # batched: see batch_systems()
 }
-BATCH_TABLE = "1e5.1e-06"  # 61 tau points, the nearest shipped table to N_tau = 60
+# The batched workload asks for N_tau = 60; the nearest shipped table (1e5.1e-06, 61 points)
+# makes the Pade continuation of the QP equation move by tens of meV under 1e-7 relative
+# noise in Sigma, so the batch uses the 72-point lambda = 1e3, eps = 1e-15 table of the
+# single-system workloads.
+BATCH_TABLE = "1e3.1e-15"
 
 
 def batch_systems(n=24):
@@ -263,6 +267,8 @@ def main():
                 "err_Sigma_rel": rel(Sig, Sig_ref),
                 "err_Ec_Ha": abs(f_ec - ec_ref),
                 "err_qp_meV": float(np.max(np.abs(E_f - E_ref)) / HA_PER_MEV),
+                "err_qp_meV_median": float(np.median(np.abs(E_f - E_ref)) / HA_PER_MEV),
+                "qp_states_above_1meV": int(np.sum(np.abs(E_f - E_ref) > HA_PER_MEV)),
                 "gemm_classes": {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
                                      "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
                                  for k, v in f_prof.items()},
@@ -427,6 +433,7 @@ def run_batch(args):
     ms_eager, _ = timed(max(1, args.steps), False)
     ms_graph, ec = timed(args.steps, True)
     errs = {"W_q0_rel": 0.0, "Sigma_rel": 0.0, "Ec_Ha": 0.0, "qp_meV": 0.0}
+    dqp = []
     for d, (S0, W00, ec0, E0), e in zip(sysd, ref, ec):
         H = torch.from_numpy(d["s"].H).cuda()
         Cd = torch.from_numpy(d["s"].C).cuda()
@@ -436,6 +443,11 @@ def run_batch(args):
         errs["Sigma_rel"] = max(errs["Sigma_rel"], rel(d["Sig"], S0))
         errs["Ec_Ha"] = max(errs["Ec_Ha"], abs(e - ec0))
         errs["qp_meV"] = max(errs["qp_meV"], float(np.max(np.abs(E - E0)) / HA_PER_MEV))
+        dqp.append(np.abs(E - E0).ravel() / HA_PER_MEV)
+    dqp = np.concatenate(dqp)
+    errs["qp_meV_median"] = float(np.median(dqp))
+    errs["qp_states_above_1meV"] = int(np.sum(dqp > 1.0))
+    errs["qp_states"] = int(dqp.size)
     out = {
         "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
         "value": round(ms_graph / 1000.0, 6),

@@ -112,6 +112,8 @@ struct thcgw_batch {
   std::vector<const void*> sig;  // pointers the graph was captured with
   int herm = -1;
   int device = 0;
+  hipStream_t cap = nullptr;  // the batch's own stream: capture and replay (the legacy stream cannot be captured)
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::string err;
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -356,9 +358,9 @@ bool dyson_chunk_hpd(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Lv, i
   if (solve) {
     thcgw::copy_batch(s, Lv, M, nq, Z, M, M, b0, cb);
     thcgw::trsm_right_lower_conj(s, Mb, M, n, Z, M, n, n, n, cb);
+    // W = K K^H - V, both triangles written from the lower tiles
     thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), n, thcgw::bp(Z, M), n, kZero, P, M, n,
-                 cb, kZero, thcgw::GEMM_TRI_LOWER);
-    thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
+                 cb, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB, thcgw::bp(V, M, nq, 1, b0));
   }
   return true;
 }
@@ -388,9 +390,15 @@ bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv
     thcgw::copy_batch(s, Lv64, M, nq, Z, M, M, b0, cb);
     thcgw::to_c64(s, Mb, Mb64, static_cast<long>(cb) * M);  // L_M rounded once for the FP32 solve
     thcgw::trsm_right_lower_conj_f32(s, static_cast<const c64*>(Mb64), M, n, Z, M, n, n, n, cb, ctx->pf());
-    thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), DT_C64, n, thcgw::bp(Z, M), DT_C64, n,
-                 kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER | ctx->pf());
-    thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
+    if (ctx->precision == 2) {  // the split-BF16 kernel has no Hermitian epilogue
+      thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), DT_C64, n, thcgw::bp(Z, M), DT_C64,
+                   n, kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER | ctx->pf());
+      thcgw::herm_finish(s, P, M, V, M, nq, b0, n, cb);
+    } else {  // W = K K^H - V, both triangles written from the lower tiles
+      thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), DT_C64, n, thcgw::bp(Z, M), DT_C64,
+                   n, kZero, P, DT_C128, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB | ctx->pf(),
+                   thcgw::bp(V, M, nq, 1, b0));
+    }
   }
   return true;
 }
@@ -1043,6 +1051,16 @@ int thcgw_batch_create(int nsys, thcgw_ctx* const* ctxs, thcgw_batch** out) {
   }
   b->ctxs.assign(ctxs, ctxs + nsys);
   b->device = ctxs[0]->device;
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    g_create_error = "thcgw_batch_create: stream or event creation failed";
+    if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    delete b;
+    return THCGW_ERR_HIP;
+  }
   for (thcgw_ctx* c : b->ctxs) {  // scratch moves to the shared pool
     for (auto& kv : c->ws)
       if (!thcgw_ctx::private_buf(kv.first) && kv.second.p) (void)hipFree(kv.second.p);
@@ -1058,6 +1076,12 @@ void thcgw_batch_destroy(thcgw_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   b->drop_graph();
+  if (b->cap) {
+    (void)hipStreamSynchronize(b->cap);
+    (void)hipStreamDestroy(b->cap);
+  }
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
   for (thcgw_ctx* c : b->ctxs) c->pool = nullptr;
   for (auto& kv : b->pool)
     if (kv.second.p) (void)hipFree(kv.second.p);
@@ -1135,35 +1159,42 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
     sig.push_back(stream);
     sig.push_back(reinterpret_cast<const void*>(static_cast<long>(want_ec)));
     if (b->use_graph) {
+      // the caller's earlier work first, then capture / replay on the batch's own stream
+      THCGW_HIP(hipEventRecord(b->ev_in, s));
+      THCGW_HIP(hipStreamWaitEvent(b->cap, b->ev_in, 0));
+      hipStream_t cs = b->cap;
       if (!b->exec || sig != b->sig || b->herm != herm) {
         b->drop_graph();
         for (thcgw_ctx* c : b->ctxs) c->deferred = true;
-        hipError_t ce = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        hipError_t ce = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
         if (ce != hipSuccess) {
           for (thcgw_ctx* c : b->ctxs) c->deferred = false;
           THCGW_HIP(ce);
         }
         try {
           for (int i = 0; i < n; ++i)
-            enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0, s);
+            enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0, cs);
         } catch (...) {
           hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(s, &g);
+          (void)hipStreamEndCapture(cs, &g);
           if (g) (void)hipGraphDestroy(g);
+          (void)hipGetLastError();
           for (thcgw_ctx* c : b->ctxs) c->deferred = false;
           throw;
         }
         for (thcgw_ctx* c : b->ctxs) c->deferred = false;
-        THCGW_HIP(hipStreamEndCapture(s, &b->graph));
+        THCGW_HIP(hipStreamEndCapture(cs, &b->graph));
         THCGW_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->sig = sig;
         b->herm = herm;
       }
-      THCGW_HIP(hipGraphLaunch(b->exec, s));
-      THCGW_HIP(hipStreamSynchronize(s));
+      THCGW_HIP(hipGraphLaunch(b->exec, cs));
+      THCGW_HIP(hipEventRecord(b->ev_out, cs));
+      THCGW_HIP(hipStreamWaitEvent(s, b->ev_out, 0));
+      THCGW_HIP(hipStreamSynchronize(cs));
       for (int i = 0; i < n; ++i) {
-        check_deferred(b->ctxs[i], s);
-        if (want_ec) Ec[i] = finish_ec(b->ctxs[i], s);
+        check_deferred(b->ctxs[i], cs);
+        if (want_ec) Ec[i] = finish_ec(b->ctxs[i], cs);
       }
     } else {
       for (int i = 0; i < n; ++i) {

@@ -31,12 +31,15 @@ struct Error : std::runtime_error {
 };
 
 
+// A failed HIP call is reported with its message and the runtime's last-error state is
+// cleared, so that the caller's next HIP call does not see it again.
 #define THCGW_HIP(call)                                                                           \
   do {                                                                                            \
     hipError_t e_ = (call);                                                                       \
-    if (e_ != hipSuccess)                                                                         \
-      throw ::thcgw::Error(THCGW_ERR_HIP,                                                \
-                           std::string(#call) + " failed: " + hipGetErrorString(e_));             \
+    if (e_ != hipSuccess) {                                                                       \
+      (void)hipGetLastError();                                                                    \
+      throw ::thcgw::Error(THCGW_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(e_)); \
+    }                                                                                             \
   } while (0)
 
 #define THCGW_REQUIRE(cond, msg)                                                                  \
@@ -78,18 +81,22 @@ constexpr int GEMM_SPLIT_BF16 = 16;
 // cgemm only: bits 6-7 hold (FP32 tile shape + 1) for products without FP64 folding
 // (0 = the process default, see gemm_set_tile_mode32).
 constexpr int GEMM_TILE32_SHIFT = 6;
+// With GEMM_TRI_LOWER on a square product X = alpha op(A) op(B) that is Hermitian: write
+// C = X - H on both triangles from the lower tiles (H complex<double>, same layout as C;
+// no beta, gamma or split BF16).
+constexpr int GEMM_HERM_SUB = 256;
 
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I   (row-major, M x N)
 void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
            BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch,
-           Cplx gamma = Cplx{0.0, 0.0}, int tri = 0);
+           Cplx gamma = Cplx{0.0, 0.0}, int tri = 0, BatchPtr hsub = BatchPtr{nullptr, 0, 1, 1, 0});
 
 // The same product in FP32 arithmetic (complex<float> MFMA, FP32 accumulation) with each
 // operand and the result stored as complex<double> or complex<float> (ta, tb, tc); alpha,
 // beta and gamma are applied in double.  Ops N/T/C for A and B in {N, C} x {N, C}.
 void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, Dt ta, long lda,
            BatchPtr B, Dt tb, long ldb, Cplx beta, void* C, Dt tc, long strideC, long ldc, int batch,
-           Cplx gamma = Cplx{0.0, 0.0}, int tri = 0);
+           Cplx gamma = Cplx{0.0, 0.0}, int tri = 0, BatchPtr hsub = BatchPtr{nullptr, 0, 1, 1, 0});
 
 // Number of FP64 GEMM launches and their modelled flops since the last reset (diagnostics).
 struct GemmStats {

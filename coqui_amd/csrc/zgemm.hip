@@ -36,6 +36,7 @@ struct Args {
   int M, N, K, batch, tiles_m, tiles_n, tri;
   double ar, ai, br, bi, gr, gi;
   BatchPtr A, B;
+  BatchPtr H;  // GEMM_HERM_SUB: complex<double> matrix subtracted from the Hermitian result
   long lda, ldb;
   void* C;
   long strideC, ldc;
@@ -289,6 +290,20 @@ void gemm_kernel(Args p) {
           double outr = p.ar * xr - p.ai * xi;
           double outi = p.ar * xi + p.ai * xr;
           TC* dst = C + static_cast<long>(r) * p.ldc + c;
+          using S = decltype(TC{}.x);
+          if (p.tri & GEMM_HERM_SUB) {
+            // lower element (r >= c) of a Hermitian product: C = X - H on both triangles
+            if (r >= c) {
+              const c128* Hb = batch_ptr<c128>(p.H, b);
+              const c128 h = Hb[static_cast<long>(r) * p.ldc + c];
+              *dst = TC{static_cast<S>(outr - h.x), static_cast<S>(outi - h.y)};
+              if (r > c) {
+                const c128 ht = Hb[static_cast<long>(c) * p.ldc + r];
+                C[static_cast<long>(c) * p.ldc + r] = TC{static_cast<S>(outr - ht.x), static_cast<S>(-outi - ht.y)};
+              }
+            }
+            continue;
+          }
           if (!beta0) {
             const TC old = *dst;
             const double o_r = old.x, o_i = old.y;
@@ -299,7 +314,6 @@ void gemm_kernel(Args p) {
             outr += p.gr;
             outi += p.gi;
           }
-          using S = decltype(TC{}.x);
           *dst = TC{static_cast<S>(outr), static_cast<S>(outi)};
         }
       }
@@ -652,8 +666,13 @@ int tile_shape(int, int, int) {
 namespace {
 
 Args make_args(int M, int N, int K, Cplx alpha, BatchPtr A, long lda, BatchPtr B, long ldb, Cplx beta, void* C,
-               long strideC, long ldc, int batch, Cplx gamma, int tri) {
+               long strideC, long ldc, int batch, Cplx gamma, int tri, BatchPtr hsub) {
+  THCGW_REQUIRE(!(tri & GEMM_HERM_SUB) || (hsub.p && M == N && beta.re == 0.0 && beta.im == 0.0 &&
+                                           gamma.re == 0.0 && gamma.im == 0.0 && (tri & GEMM_TRI_LOWER) &&
+                                           !(tri & GEMM_SPLIT_BF16)),
+                "gemm: GEMM_HERM_SUB needs a square lower product without beta, gamma or split BF16");
   Args a;
+  a.H = hsub;
   a.M = M;
   a.N = N;
   a.K = K;
@@ -677,7 +696,7 @@ Args make_args(int M, int N, int K, Cplx alpha, BatchPtr A, long lda, BatchPtr B
 
 // Modelled flops; triangular structure (square operands) halves the work per flag.
 double model_flops(int M, int N, int K, int batch, int tri) {
-  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL;  // the precision flags do not change the work
+  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL;  // the other flags do not change the work
   double flops = 8.0 * double(M) * N * K * batch;
   if ((tri & GEMM_TRI_LOWER) && (tri & (GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL))) flops /= 6.0;
   else if (tri) flops /= 2.0;
@@ -724,11 +743,12 @@ void dispatch_f32_c(hipStream_t s, int opA, int opB, Dt tc, const Args& a) {
 }  // namespace
 
 void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
-           BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch, Cplx gamma, int tri) {
+           BatchPtr B, long ldb, Cplx beta, c128* C, long strideC, long ldc, int batch, Cplx gamma, int tri,
+           BatchPtr hsub) {
   THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "zgemm: negative size");
   THCGW_REQUIRE(opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2, "zgemm: bad op");
   if (M == 0 || N == 0 || batch == 0) return;
-  const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri);
+  const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri, hsub);
   const int nb = tile_shape(M, N, K);
   timed(s, model_flops(M, N, K, batch, tri), opA * 3 + opB, [&] {
     switch (opA * 3 + opB) {
@@ -747,10 +767,11 @@ void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
 
 void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, Dt ta, long lda,
            BatchPtr B, Dt tb, long ldb, Cplx beta, void* C, Dt tc, long strideC, long ldc, int batch, Cplx gamma,
-           int tri) {
+           int tri, BatchPtr hsub) {
   THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm: negative size");
   if (M == 0 || N == 0 || batch == 0) return;
-  const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri);
+  THCGW_REQUIRE(!(tri & GEMM_HERM_SUB) || tc == DT_C128, "cgemm: GEMM_HERM_SUB writes complex<double>");
+  const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri, hsub);
   const int cls = ((tri & GEMM_SPLIT_BF16) ? 13 : 9) + (opA == OP_C ? 2 : 0) + (opB == OP_C ? 1 : 0);
   timed(s, model_flops(M, N, K, batch, tri), cls, [&] {
     if (ta == DT_C128 && tb == DT_C128) dispatch_f32_c<c128, c128>(s, opA, opB, tc, a);
