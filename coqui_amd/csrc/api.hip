@@ -41,6 +41,7 @@ struct thcgw_ctx {
   c128* d_Ttw_ff = nullptr; // nt x nw_f
   c128* d_Tbeta = nullptr;  // 1 x nt
   long* d_wn_f = nullptr;   // nw_f fermionic Matsubara indices
+  int* d_kmq = nullptr;     // k - q table [k][q] and its transpose [q][k] (k-space convolutions)
   std::map<std::string, DevBuf> ws;
   std::string err;
   bool force_fallback = false;
@@ -90,6 +91,8 @@ struct thcgw_ctx {
     if (d_Ttw_ff) (void)hipFree(d_Ttw_ff);
     if (d_Tbeta) (void)hipFree(d_Tbeta);
     if (d_wn_f) (void)hipFree(d_wn_f);
+    if (d_kmq) (void)hipFree(d_kmq);
+    d_kmq = nullptr;
     d_tau = nullptr;
     d_Tplus = d_Tminus = d_Twt_ff = d_Ttw_ff = d_Tbeta = nullptr;
     d_wn_f = nullptr;
@@ -549,6 +552,63 @@ void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c1
   if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
 }
 
+// ---- k-space (convolution) variants of chi0 and Sigma ----------------------------------
+
+const int* kmq_tables(thcgw_ctx* ctx) {
+  if (!ctx->d_kmq) {
+    const int nk = ctx->nk;
+    std::vector<int> t = thcgw::kminus_table(ctx->d.kgrid), both(static_cast<size_t>(2) * nk * nk);
+    for (int a = 0; a < nk; ++a)
+      for (int b = 0; b < nk; ++b) {
+        both[static_cast<size_t>(a) * nk + b] = t[static_cast<size_t>(a) * nk + b];
+        both[static_cast<size_t>(nk) * nk + static_cast<size_t>(a) * nk + b] = t[static_cast<size_t>(b) * nk + a];
+      }
+    THCGW_HIP(hipMalloc(&ctx->d_kmq, sizeof(int) * both.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_kmq, both.data(), sizeof(int) * both.size(), hipMemcpyHostToDevice));
+  }
+  return ctx->d_kmq;
+}
+
+// Pi^k_PQ(tau_i, q) = (sp / N_k) sum_k conj(Gp(k - q)) Gn(k)   (rpa_pi.icc:243, 328-333)
+void run_pi_kspace(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const long M = ctx->M;
+  const long gs = static_cast<long>(nk) * no * no;
+  const int* kmq = kmq_tables(ctx);
+  c128* GP = ctx->buf<c128>("pi_gp", static_cast<size_t>(nk) * M);
+  c128* GN = ctx->buf<c128>("pi_gn", static_cast<size_t>(nk) * M);
+  c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(nk) * nmu * no);
+  for (int i = 0; i < nth; ++i) {
+    project_p2a(ctx, s, X, G + i * gs, GP, nk, Ab);
+    project_p2a(ctx, s, X, G + (nt - 1 - i) * gs, GN, nk, Ab);
+    thcgw::kconv_pi(s, GP, GN, kmq + static_cast<long>(nk) * nk, nk, Pi + static_cast<long>(i) * nk * M, M,
+                    -2.0 / nk);
+  }
+}
+
+// Sigma_PQ(tau, k) = -(1/N_k) sum_q G(k - q) W(q), both particle-hole passes   (thc_gw.icc:342-443)
+void run_sigma_kspace(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm,
+                      hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const long M = ctx->M;
+  const long gs = static_cast<long>(nk) * no * no;
+  const int* kmq = kmq_tables(ctx);
+  c128* GP = ctx->buf<c128>("sg_gp", static_cast<size_t>(nk) * M);
+  c128* SP = ctx->buf<c128>("sg_sp", static_cast<size_t>(nk) * M);
+  c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(nk) * nmu * no);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ntp = pass == 0 ? nth : nt / 2;
+    for (int it = 0; it < ntp; ++it) {
+      const int itg = pass == 0 ? it : nt - it - 1;
+      project_p2a(ctx, s, X, G + itg * gs, GP, nk, Ab);
+      thcgw::kconv_sigma(s, GP, Wqt + static_cast<long>(it) * M, static_cast<long>(nth) * M, kmq, nk, SP, M,
+                         -1.0 / nk);
+      project_a2p(ctx, s, X, SP, Sig + itg * gs, nk, Ab);
+    }
+  }
+  if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
+}
+
 // ---- one iteration as enqueue + finish (the batch captures the enqueue part) -----------
 
 void enqueue_iteration(thcgw_ctx* ctx, const c128* G, const c128* X, const c128* V, c128* Sig, c128* Wqt, bool want_ec,
@@ -821,6 +881,23 @@ int thcgw_pi(thcgw_ctx* ctx, const double* G, const double* X, double* Pi, void*
   return guarded(ctx, [&] {
     THCGW_REQUIRE(G && X && Pi, "thcgw_pi: null pointer");
     run_pi(ctx, C(G), C(X), Cm(Pi), S(stream));
+  });
+}
+
+int thcgw_pi_kspace(thcgw_ctx* ctx, const double* G, const double* X, double* Pi, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(G && X && Pi, "thcgw_pi_kspace: null pointer");
+    run_pi_kspace(ctx, C(G), C(X), Cm(Pi), S(stream));
+  });
+}
+
+int thcgw_sigma_kspace(thcgw_ctx* ctx, const double* G, const double* Wqt, const double* X, double* Sig, int herm,
+                       void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(G && Wqt && X && Sig, "thcgw_sigma_kspace: null pointer");
+    run_sigma_kspace(ctx, C(G), C(Wqt), C(X), Cm(Sig), herm != 0, S(stream));
   });
 }
 

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace thcgw {
 
@@ -122,6 +123,13 @@ void pi_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64
                  long M, double scale);
 void sigma_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* G1, const c64* G2,
                     const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale);
+
+// kconv.hip: k-space convolutions (chi0 uses the [q][k] table, Sigma the [k][q] one)
+std::vector<int> kminus_table(const int* kgrid);  // t[a * nk + b] = index of k_a - k_b
+void kconv_pi(hipStream_t s, const c128* Gp, const c128* Gn, const int* kmq_qk, int nk, c128* Pi, long M,
+              double scale);
+void kconv_sigma(hipStream_t s, const c128* G, const c128* W, long w_kstride, const int* kmq_kq, int nk, c128* S,
+                 long M, double scale);
 
 // lu.hip
 void getrf_batched(hipStream_t s, c128* A, long sA, int lda, int n, int* ipiv, int* info, int batch);

@@ -22,7 +22,7 @@ EXPORTS = (
     "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
     "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf", "thcgw_dyson_g",
     "thcgw_nelec", "thcgw_update_mu", "thcgw_batch_create", "thcgw_batch_destroy", "thcgw_batch_last_error",
-    "thcgw_batch_set", "thcgw_batch_gw_iteration",
+    "thcgw_batch_set", "thcgw_batch_gw_iteration", "thcgw_pi_kspace", "thcgw_sigma_kspace",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -66,6 +66,8 @@ def load_library(path=LIB_PATH):
         "thcgw_aux_to_primary": (i, [p, l, p, p, p, p]),
         "thcgw_pi": (i, [p, p, p, p, p]),
         "thcgw_tau_to_w": (i, [p, p, p, p]),
+        "thcgw_pi_kspace": (i, [p, p, p, p, p]),
+        "thcgw_sigma_kspace": (i, [p, p, p, p, p, i, p]),
         "thcgw_dyson_w": (i, [p, p, p, p, p, p]),
         "thcgw_w_to_tau": (i, [p, p, p, p]),
         "thcgw_sigma": (i, [p, p, p, p, p, i, p]),
@@ -219,6 +221,21 @@ class ThcGW:
         P = out if out is not None else self._empty(self.nt_half, self.nk, self.nmu, self.nmu)
         self._check(self.lib.thcgw_pi(self.h, _ptr(G), _ptr(X), _ptr(P), self._stream(stream)))
         return P
+
+    def pi_kspace(self, G, X, out=None, stream=None):
+        """k-space chi0 (rpa_pi.icc:182-336): Pi^k_PQ(q) = Pi^R_QP(-q) for Hermitian G."""
+        G, X = self._dev(G), self._dev(X)
+        P = out if out is not None else self._empty(self.nt_half, self.nk, self.nmu, self.nmu)
+        self._check(self.lib.thcgw_pi_kspace(self.h, _ptr(G), _ptr(X), _ptr(P), self._stream(stream)))
+        return P
+
+    def sigma_kspace(self, G, W_qt, X, hermitize=True, out=None, stream=None):
+        """k-space self-energy (thc_gw.icc:342-443)."""
+        G, W_qt, X = self._dev(G), self._dev(W_qt), self._dev(X)
+        S = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        self._check(self.lib.thcgw_sigma_kspace(self.h, _ptr(G), _ptr(W_qt), _ptr(X), _ptr(S), int(bool(hermitize)),
+                                                self._stream(stream)))
+        return S
 
     def tau_to_w(self, Pi_t, out=None, stream=None):
         Pi_t = self._dev(Pi_t)

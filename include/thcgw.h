@@ -99,6 +99,17 @@ int thcgw_aux_to_primary(thcgw_ctx* ctx, long nb, const double* X_skPa, const do
  * scr_coulomb_t::eval_Pi_rpa_Rspace, src/methods/scr_coulomb/rpa_pi.icc:22-180. */
 int thcgw_pi(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa, double* Pi_tqPQ, void* stream);
 
+/* k-space variants (no symmetry: isym = 0, no time reversal) of a4 and a9, the reference's
+ * algorithms for symmetry-reduced grids: Pi^k_PQ(tau_i, q) = (sp/N_k) sum_k
+ * conj(G_PQ(tau_i, k-q)) G_PQ(beta-tau_i, k) (eval_Pi_rpa_kspace, src/methods/GW/rpa_pi.icc:182-336;
+ * equal to Pi^R_QP(-q) of thcgw_pi for Hermitian G) and Sigma_PQ(tau, k) = -(1/N_k) sum_q
+ * G_PQ(tau, k-q) W_PQ(tau, q) in both particle-hole passes (eval_Sigma_all_kspace,
+ * src/methods/GW/thc_gw.icc:342-443).  Same layouts as thcgw_pi / thcgw_sigma; N_k <= 64.
+ * O(N_k^2) per (P,Q) column.  Enqueue only. */
+int thcgw_pi_kspace(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa, double* Pi_tqPQ, void* stream);
+int thcgw_sigma_kspace(thcgw_ctx* ctx, const double* G_tskij, const double* W_qtPQ, const double* X_skPa,
+                       double* Sigma_tskij, int hermitize, void* stream);
+
 /* a5: Pi_tqPQ -> Pi_wqPQ with the particle-hole folded bosonic transform.  Replaces
  * scr_coulomb_t::tau_to_w (scr_coulomb_t.cpp:349-376) -> IAFT::tau_to_w_PHsym (IAFT.icc:49-73). */
 int thcgw_tau_to_w(thcgw_ctx* ctx, const double* Pi_tqPQ, double* Pi_wqPQ, void* stream);

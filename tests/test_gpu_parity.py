@@ -219,3 +219,16 @@ def test_create_rejects_non_ph_symmetric_mesh(torch):
     with pytest.raises(ThcgwError) as e:
         ThcGW((1, 1, 1), 2, 4, ia)
     assert "particle-hole" in str(e.value)
+
+
+@pytest.mark.parametrize("kgrid,no,nmu", [((2, 2, 2), 26, 200), ((4, 4, 4), 7, 24), ((3, 1, 2), 6, 20)])
+def test_kspace_variants(orc, torch, kgrid, no, nmu):
+    """k-space chi0 and Sigma (rpa_pi.icc:182-336, thc_gw.icc:342-443) against the oracle's
+    k-space restatement."""
+    s, ia, tab, G = setup(orc, kgrid, no, nmu, "1e2.1e-15", sid=9)
+    ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
+    PK = to_np(ctx.pi_kspace(G, s.X))
+    assert relmax(PK, orc.pi_kspace(G, s.X, s.kgrid)) < TOL
+    Wqt = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=False)["W_qt"]
+    SK = to_np(ctx.sigma_kspace(G, Wqt, s.X, hermitize=False))
+    assert relmax(SK, orc.sigma_kspace(G, s.X, Wqt, s.kgrid)) < TOL
