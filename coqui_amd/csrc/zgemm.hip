@@ -86,11 +86,19 @@ __device__ __forceinline__ RC ld_elem(const T* X, long ld, int r, int c, int R, 
 // (op(A) = A, op(B) = B^T or B^H); otherwise at X[k][x] (op(A) = A^T or A^H, op(B) = B).
 // CONJ conjugates on load.  Pairs of adjacent elements along the contiguous direction
 // are loaded together; each thread loads P = BX*BK/(2*NT) pairs.
-// Column swizzle of LDS row k: the K-contiguous stores of a thread group write one column
-// of several rows 1-2 KB apart, which would fall into one bank; XOR-ing the column with
-// (k/2) mod 8 spreads them.  The MFMA reads (16 consecutive columns of one row) stay a
-// permutation of the same aligned 16 columns.
-__device__ __forceinline__ int swz(int k) { return (k >> 1) & 7; }
+// Column swizzle of LDS row k (rows are 1-2 KB apart, so every row starts in bank 0).
+//   complex<double> (16 B): a K-contiguous store group (16 lanes) writes columns 0..3 of
+//     rows 0, 2, 4, 6; XOR-ing the column with 4 (k/2) gives the 16 lanes distinct bank
+//     groups.  The fragment reads (16 lanes, 16 consecutive columns, one row) stay within
+//     an aligned 16-column block, i.e. all 64 banks.
+//   complex<float> (8 B): a store group (32 lanes) writes columns 0..3 of rows 0, 2, ..., 14
+//     (XOR with 4 (k/2) separates them), and one fragment read covers rows k and k + 1 with
+//     16 lanes each, so odd rows are moved to the other half of a 32-column block.
+template <typename RC>
+__device__ __forceinline__ int swz(int k) {
+  if constexpr (sizeof(RC) == 16) return ((k >> 1) & 3) << 2;
+  else return (((k >> 1) & 7) << 2) ^ ((k & 1) << 4);
+}
 
 template <typename RC, typename T, bool ROWS, bool CONJ, int BX, int BK>
 struct TileIO {
@@ -125,11 +133,11 @@ struct TileIO {
       int x, k;
       coords(threadIdx.x, p, x, k);
       if (ROWS) {
-        S[k][x ^ swz(k)] = r[2 * p];
-        S[k + 1][x ^ swz(k + 1)] = r[2 * p + 1];
+        S[k][x ^ swz<RC>(k)] = r[2 * p];
+        S[k + 1][x ^ swz<RC>(k + 1)] = r[2 * p + 1];
       } else {
-        S[k][x ^ swz(k)] = r[2 * p];
-        S[k][(x + 1) ^ swz(k)] = r[2 * p + 1];
+        S[k][x ^ swz<RC>(k)] = r[2 * p];
+        S[k][(x + 1) ^ swz<RC>(k)] = r[2 * p + 1];
       }
     }
   }
@@ -223,7 +231,7 @@ void gemm_kernel(Args p) {
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
         const int kk = ks * 4 + lk;
-        const RC v = As[cur][kk][(wm * (BM / 2) + i * 16 + lr) ^ swz(kk)];
+        const RC v = As[cur][kk][(wm * (BM / 2) + i * 16 + lr) ^ swz<RC>(kk)];
         a_r[i] = v.x;
         a_i[i] = v.y;
         a_n[i] = -v.y;
@@ -231,7 +239,7 @@ void gemm_kernel(Args p) {
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         const int kk = ks * 4 + lk;
-        const RC v = Bs[cur][kk][(wn * (BN / 2) + j * 16 + lr) ^ swz(kk)];
+        const RC v = Bs[cur][kk][(wn * (BN / 2) + j * 16 + lr) ^ swz<RC>(kk)];
         b_r[j] = v.x;
         b_i[j] = v.y;
       }
