@@ -2,11 +2,14 @@
 
 The compute lives in libthcgw.so (hand-written HIP kernels for gfx950 behind the C ABI of
 include/thcgw.h).  This package holds the Python binding (thcgw.ThcGW), the IR tables
-(iaft.IAFT) and the seeded synthetic-system generator (system.make_system).
+(iaft.IAFT), the seeded synthetic-system generator (system.make_system) and the
+self-consistent loop (scgw.ScGW) with the learned Krylov Dyson solve (thcgw.LearnedDyson).
 """
 from .iaft import IAFT, available_tables, determine_lambda  # noqa: F401
 from .system import System, make_system  # noqa: F401
-from .thcgw import EXPORTS, LIB_PATH, ThcBatch, ThcGW, ThcgwError, load_library  # noqa: F401
+from .thcgw import EXPORTS, LIB_PATH, LearnedDyson, Mlp, ThcBatch, ThcGW, ThcgwError, load_library  # noqa: F401
+from .scgw import ScGW  # noqa: F401
 
 __all__ = ["IAFT", "available_tables", "determine_lambda", "System", "make_system", "ThcGW", "ThcBatch", "ThcgwError",
+           "LearnedDyson", "Mlp", "ScGW",
            "load_library", "EXPORTS", "LIB_PATH"]

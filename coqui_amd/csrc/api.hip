@@ -2,12 +2,16 @@
 #include "../../include/thcgw.h"
 #include "thcgw_internal.hpp"
 #include "thcgw_host.hpp"
+#include "lri.hpp"
+#include "mlp.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -1310,6 +1314,587 @@ int thcgw_gm_energy(thcgw_ctx* ctx, const double* G, const double* Sigma, double
     THCGW_HIP(hipMemcpyAsync(h.data(), sg, sizeof(c128) * nw, hipMemcpyDeviceToHost, s));
     THCGW_HIP(hipStreamSynchronize(s));
     *e_corr = thcgw::host::gm_from_sigmag(ctx->tab, h);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Learned low-rank approximate inverse for the Dyson solve (Krylov form).
+//
+// For each (n, q), with V = L L^H and A = -L^H Pi(iW_n) L (Hermitian, positive
+// semidefinite), the Hermitian Dyson form needs M^-1 L^H, M = I + A:
+// W - V = L M^-1 L^H - V.  The Krylov form solves M X = [L^H | Q] column by column with
+// preconditioned conjugate gradients in FP64, where Q (n x r) is an orthonormal basis of
+// the sketch A Omega (Omega a fixed Gaussian n x r) and the preconditioner is the
+// rank-r approximate inverse
+//     P^-1 = I - Q C Q^H,   C Hermitian r x r (the core),
+// also used for the warm start X0 = P^-1 B.  The core that makes P^-1 agree with M^-1 on
+// span(Q) in the Frobenius sense is C* = I - Q^H M^-1 Q; the direct solve computes it
+// exactly, the Krylov solve from its last r columns.  The previous call's exact core is
+// carried into the current basis as C_tr = O^H C*_prev O, O = Q_prev^H Q.  The core comes
+// from
+//   mode 0: no preconditioner (C = 0, X0 = 0);
+//   mode 1: Woodbury on span(Q), C_W = T (I + T)^-1, T = Q^H A Q (exact when span(Q) is
+//           invariant under A);
+//   mode 2: C_tr (mode 1 on the first call);
+//   mode 3: C_W plus the correction predicted by the network from (T, C_W, C_tr).
+// Every Krylov solve is accepted only when each column's true FP64 residual
+// ||B - M X|| / ||B|| is at most tol; otherwise its chunk is solved again directly.
+// ---------------------------------------------------------------------------------
+struct thcgw_lri {
+  thcgw_ctx* ctx = nullptr;
+  int r = 0, width = 0;
+  std::unique_ptr<thcgw::Mlp> net;
+  int mode = 3;
+  int max_iter = 200;
+  double tol = 1e-10;     // accepted relative residual (true FP64 residual, every column)
+  double cg_tol = 1e-11;  // CG stops when every recursive residual is below this
+  double t_scale = 0.1;   // feature scale of T blocks
+  double y_scale = 1.0;   // target scale (set from the recorded targets at the first training)
+  bool y_scaled = false;
+  c128* Om = nullptr;     // n x r sketch
+  c128* Qprev = nullptr;  // per (n, q): n x r basis of the previous call
+  c128* Cprev = nullptr;  // per (n, q): r x r exact core of the previous call
+  bool has_prev = false;
+  float* feat = nullptr;  // recorded training pairs
+  float* targ = nullptr;
+  long n_pairs = 0, cap_pairs = 0;
+  float* ybuf = nullptr;  // network output / training scratch
+  long ybuf_count = 0;
+  std::string err;
+  int nin() const { return 3 * r * r + 1; }
+  int nout() const { return r * r; }
+  ~thcgw_lri() {
+    for (void* p : {static_cast<void*>(Om), static_cast<void*>(Qprev), static_cast<void*>(Cprev),
+                    static_cast<void*>(feat), static_cast<void*>(targ), static_cast<void*>(ybuf)})
+      if (p) (void)hipFree(p);
+  }
+  float* scratch(long count) {
+    if (count > ybuf_count) {
+      if (ybuf) THCGW_HIP(hipFree(ybuf));
+      ybuf = nullptr;
+      ybuf_count = 0;
+      THCGW_HIP(hipMalloc(&ybuf, sizeof(float) * count));
+      ybuf_count = count;
+    }
+    return ybuf;
+  }
+};
+
+struct thcgw_mlp {
+  std::unique_ptr<thcgw::Mlp> net;
+  int device = 0;
+  float* buf = nullptr;  // Y and dY of train_mse
+  long buf_count = 0;
+  std::string err;
+  ~thcgw_mlp() {
+    if (buf) (void)hipFree(buf);
+  }
+};
+
+namespace {
+
+template <class F>
+int guard_err(std::string& err, int device, F&& f) {
+  try {
+    THCGW_HIP(hipSetDevice(device));
+    f();
+    return THCGW_OK;
+  } catch (const thcgw::Error& e) {
+    err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    err = "host allocation failed";
+    return THCGW_ERR_ALLOC;
+  } catch (const std::exception& e) {
+    err = e.what();
+    return THCGW_ERR_ARG;
+  }
+}
+
+// Woodbury core C = T (I + T)^-1 = I - (I + T)^-1 of r x r Hermitian T (batch), with
+// (I + T)^-1 = L^-H L^-1 from the Cholesky factor of I + T.
+void woodbury_core(thcgw_ctx* ctx, hipStream_t s, const c128* T, int r, int cb, c128* C, c128* W1, c128* W2,
+                   int* info) {
+  const long R2 = static_cast<long>(r) * r;
+  thcgw::copy_batch(s, T, R2, 1 << 30, W1, R2, R2, 0, cb);
+  thcgw::add_diag(s, W1, R2, r, r, 1.0, cb);
+  thcgw::potrf_batched(s, W1, R2, r, r, info, cb);
+  thcgw::set_identity(s, W2, R2, r, cb);
+  thcgw::trsm_right_lower_conj(s, W1, R2, r, W2, R2, r, r, r, cb);  // W2 = L^-H
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, r, r, r, kMinusOne, thcgw::bp(W2, R2), r, thcgw::bp(W2, R2), r, kZero, C,
+               R2, r, cb, kOne);
+  (void)ctx;
+}
+
+// Z = P^-1 Rin = Rin - Q (C (Q^H Rin)) over n x ncol blocks (ld ncol).
+void apply_pinv(hipStream_t s, const c128* Q, const c128* Cc, const c128* Rin, c128* Zout, c128* U, c128* U2, int n,
+                int r, int ncol, int cb) {
+  const long MB = static_cast<long>(n) * ncol, QS = static_cast<long>(n) * r, R2 = static_cast<long>(r) * r,
+             US = static_cast<long>(r) * ncol;
+  thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, r, ncol, n, kOne, thcgw::bp(Q, QS), r, thcgw::bp(Rin, MB), ncol, kZero, U,
+               US, ncol, cb);
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, r, ncol, r, kOne, thcgw::bp(Cc, R2), r, thcgw::bp(U, US), ncol, kZero, U2,
+               US, ncol, cb);
+  if (Zout != Rin) thcgw::copy2d(s, false, Rin, MB, ncol, 1 << 30, 0, Zout, MB, ncol, n, ncol, cb);
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, ncol, r, kMinusOne, thcgw::bp(Q, QS), r, thcgw::bp(U2, US), ncol, kOne,
+               Zout, MB, ncol, cb);
+}
+
+double host_max(const double* d, int count, hipStream_t s) {
+  std::vector<double> h(count);
+  THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  double m = 0.0;
+  for (double v : h) m = std::isnan(v) ? INFINITY : std::max(m, v);
+  return m;
+}
+
+std::vector<double> host_vec(const double* d, int count, hipStream_t s) {
+  std::vector<double> h(count);
+  THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+// ||X - Y||_F^2 and ||Y||_F^2 summed over a batch of r x r matrices (host reduction).
+void core_err_sums(const c128* X, const c128* Y, long count, hipStream_t s, double& num, double& den) {
+  std::vector<std::complex<double>> hx(count), hy(count);
+  THCGW_HIP(hipMemcpyAsync(hx.data(), X, sizeof(c128) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipMemcpyAsync(hy.data(), Y, sizeof(c128) * count, hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  for (long i = 0; i < count; ++i) {
+    num += std::norm(hx[i] - hy[i]);
+    den += std::norm(hy[i]);
+  }
+}
+
+// features: [T t_scale | C_W | C_tr | has_prev]; target: (C* - C_W) y_scale
+void lri_features(thcgw_lri* L, hipStream_t s, const c128* T, const c128* Cw, const c128* Ctr, bool prev, float* F,
+                  int cb) {
+  const int r = L->r, nin = L->nin();
+  const long R2 = static_cast<long>(r) * r;
+  thcgw::pack_herm(s, T, nullptr, R2, r, F, nin, 0, L->t_scale, cb);
+  thcgw::pack_herm(s, Cw, nullptr, R2, r, F, nin, R2, 1.0, cb);
+  thcgw::pack_herm(s, Ctr, nullptr, R2, r, F, nin, 2 * R2, 1.0, cb);
+  thcgw::set_col(s, F, nin, 3 * R2, prev ? 1.0f : 0.0f, cb);
+}
+
+void lri_record(thcgw_lri* L, hipStream_t s, const c128* T, const c128* Cw, const c128* Ctr, bool prev,
+                const c128* Cst, int cb) {
+  const int r = L->r, nin = L->nin(), nout = L->nout();
+  const long R2 = static_cast<long>(r) * r;
+  if (L->n_pairs + cb > L->cap_pairs) {
+    const long cap = std::max(L->n_pairs + cb, 2 * L->cap_pairs + 256);
+    float *f = nullptr, *t = nullptr;
+    THCGW_HIP(hipMalloc(&f, sizeof(float) * cap * nin));
+    THCGW_HIP(hipMalloc(&t, sizeof(float) * cap * nout));
+    if (L->n_pairs) {
+      THCGW_HIP(hipMemcpyAsync(f, L->feat, sizeof(float) * L->n_pairs * nin, hipMemcpyDeviceToDevice, s));
+      THCGW_HIP(hipMemcpyAsync(t, L->targ, sizeof(float) * L->n_pairs * nout, hipMemcpyDeviceToDevice, s));
+      THCGW_HIP(hipStreamSynchronize(s));
+    }
+    if (L->feat) THCGW_HIP(hipFree(L->feat));
+    if (L->targ) THCGW_HIP(hipFree(L->targ));
+    L->feat = f;
+    L->targ = t;
+    L->cap_pairs = cap;
+  }
+  float* F = L->feat + L->n_pairs * nin;
+  float* Y = L->targ + L->n_pairs * nout;
+  lri_features(L, s, T, Cw, Ctr, prev, F, cb);
+  thcgw::pack_herm(s, Cst, Cw, R2, r, Y, nout, 0, L->y_scale, cb);
+  L->n_pairs += cb;
+}
+
+}  // namespace
+
+extern "C" {
+
+int thcgw_lri_create(thcgw_ctx* ctx, int rank, int width, unsigned long long seed, thcgw_lri** out) {
+  if (!ctx || !out) return THCGW_ERR_ARG;
+  *out = nullptr;
+  auto L = std::make_unique<thcgw_lri>();
+  const int rc = guard_err(ctx->err, ctx->device, [&] {
+    THCGW_REQUIRE(rank >= 1 && rank <= 128 && rank <= ctx->d.nmu, "thcgw_lri_create: rank must be in [1, min(128, nmu)]");
+    THCGW_REQUIRE(width >= 1 && width <= 4096, "thcgw_lri_create: width must be in [1, 4096]");
+    L->ctx = ctx;
+    L->r = rank;
+    L->width = width;
+    L->net = std::make_unique<thcgw::Mlp>(L->nin(), width, L->nout(), seed);
+    const int n = ctx->d.nmu, total = ctx->nw_half * ctx->nk;
+    const long R2 = static_cast<long>(rank) * rank;
+    // Omega: complex Gaussian with unit variance per entry (Box-Muller on a seeded generator)
+    std::vector<std::complex<double>> h(static_cast<size_t>(n) * rank);
+    std::mt19937_64 rng(seed ^ 0x5bd1e995ULL);
+    std::normal_distribution<double> g(0.0, std::sqrt(0.5));
+    for (auto& z : h) z = std::complex<double>(g(rng), g(rng));
+    THCGW_HIP(hipMalloc(&L->Om, sizeof(c128) * h.size()));
+    THCGW_HIP(hipMemcpy(L->Om, h.data(), sizeof(c128) * h.size(), hipMemcpyHostToDevice));
+    THCGW_HIP(hipMalloc(&L->Qprev, sizeof(c128) * static_cast<long>(n) * rank * total));
+    THCGW_HIP(hipMalloc(&L->Cprev, sizeof(c128) * R2 * total));
+  });
+  if (rc == THCGW_OK) *out = L.release();
+  return rc;
+}
+
+void thcgw_lri_destroy(thcgw_lri* L) { delete L; }
+
+const char* thcgw_lri_last_error(const thcgw_lri* L) { return L ? L->err.c_str() : "null handle"; }
+
+int thcgw_lri_set(thcgw_lri* L, int mode, int max_iter, double tol) {
+  if (!L) return THCGW_ERR_ARG;
+  if (mode < 0 || mode > 3 || max_iter < 0 || !(tol > 0.0)) {
+    L->err = "thcgw_lri_set: mode in 0..3, max_iter >= 0, tol > 0";
+    return THCGW_ERR_ARG;
+  }
+  L->mode = mode;
+  L->max_iter = max_iter;
+  L->tol = tol;
+  L->cg_tol = 0.1 * tol;
+  return THCGW_OK;
+}
+
+long thcgw_lri_num_pairs(const thcgw_lri* L) { return L ? L->n_pairs : 0; }
+
+int thcgw_lri_clear_pairs(thcgw_lri* L) {
+  if (!L) return THCGW_ERR_ARG;
+  L->n_pairs = 0;
+  return THCGW_OK;
+}
+
+int thcgw_lri_reset_state(thcgw_lri* L) {
+  if (!L) return THCGW_ERR_ARG;
+  L->has_prev = false;
+  return THCGW_OK;
+}
+
+int thcgw_lri_dyson(thcgw_lri* L, double* PiW, const double* V, int krylov, int record, thcgw_lri_stats* st,
+                    void* stream) {
+  if (!L) return THCGW_ERR_ARG;
+  thcgw_ctx* ctx = L->ctx;
+  return guard_err(L->err, ctx->device, [&] {
+    THCGW_REQUIRE(PiW && V, "thcgw_lri_dyson: null pointer");
+    hipStream_t s = S(stream);
+    const int n = ctx->d.nmu, nq = ctx->nk, r = L->r, ncol = n + r;
+    const long M = ctx->M, R2 = static_cast<long>(r) * r, QS = static_cast<long>(n) * r,
+               MB = static_cast<long>(n) * ncol, US = static_cast<long>(r) * ncol;
+    const int total = ctx->nw_half * nq;
+    thcgw_lri_stats acc{};
+    double core_num = 0.0, core_den = 0.0, iter_sum = 0.0;
+    int chunks = 0;
+    // chunk: A, Zb (n x n); B, X, R, Zp, P, AP (n x ncol) per matrix
+    const double per = 16.0 * (2.0 * M + (krylov ? 6.0 * MB : 0.0) + 6.0 * QS + 2.0 * US + 8.0 * R2);
+    const double budget = std::min(0.3 * double(free_device_bytes()), 24.0 * double(1L << 30));
+    const int CH = std::max(1, std::min(total, static_cast<int>(budget / per)));
+    c128* Lv = ctx->buf<c128>("dy_lv", static_cast<size_t>(nq) * M);
+    if (!factor_v(ctx, C(V), Lv, s)) throw thcgw::Error(THCGW_ERR_SINGULAR, "thcgw_lri_dyson: some V_q is not positive definite");
+    c128* A = ctx->buf<c128>("lr_a", static_cast<size_t>(CH) * M);
+    c128* Zb = ctx->buf<c128>("lr_z", static_cast<size_t>(CH) * M);
+    c128* Sk = ctx->buf<c128>("lr_s", static_cast<size_t>(CH) * QS);
+    c128* Q = ctx->buf<c128>("lr_q", static_cast<size_t>(CH) * QS);
+    c128* AQ = ctx->buf<c128>("lr_aq", static_cast<size_t>(CH) * QS);
+    c128* Gs = ctx->buf<c128>("lr_g", static_cast<size_t>(CH) * R2);
+    c128* T = ctx->buf<c128>("lr_t", static_cast<size_t>(CH) * R2);
+    c128* Cc = ctx->buf<c128>("lr_c", static_cast<size_t>(CH) * R2);
+    c128* Cst = ctx->buf<c128>("lr_cst", static_cast<size_t>(CH) * R2);
+    c128* Cw = ctx->buf<c128>("lr_cw", static_cast<size_t>(CH) * R2);
+    c128* Ctr = ctx->buf<c128>("lr_ctr", static_cast<size_t>(CH) * R2);
+    c128* W1 = ctx->buf<c128>("lr_w1", static_cast<size_t>(CH) * R2);
+    c128* W2 = ctx->buf<c128>("lr_w2", static_cast<size_t>(CH) * R2);
+    int* info = ctx->buf<int>("lr_info", static_cast<size_t>(CH));
+    c128 *B = nullptr, *X = nullptr, *R = nullptr, *Zp = nullptr, *P = nullptr, *AP = nullptr, *U = nullptr,
+         *U2 = nullptr;
+    double *bn = nullptr, *rel = nullptr, *rz = nullptr, *rmax = nullptr;
+    if (krylov) {
+      B = ctx->buf<c128>("lr_b", static_cast<size_t>(CH) * MB);
+      X = ctx->buf<c128>("lr_x", static_cast<size_t>(CH) * MB);
+      R = ctx->buf<c128>("lr_r", static_cast<size_t>(CH) * MB);
+      Zp = ctx->buf<c128>("lr_zp", static_cast<size_t>(CH) * MB);
+      P = ctx->buf<c128>("lr_p", static_cast<size_t>(CH) * MB);
+      AP = ctx->buf<c128>("lr_ap", static_cast<size_t>(CH) * MB);
+      U = ctx->buf<c128>("lr_u", static_cast<size_t>(CH) * US);
+      U2 = ctx->buf<c128>("lr_u2", static_cast<size_t>(CH) * US);
+      bn = ctx->buf<double>("lr_bn", static_cast<size_t>(CH) * ncol);
+      rel = ctx->buf<double>("lr_rel", static_cast<size_t>(CH) * ncol);
+      rz = ctx->buf<double>("lr_rz", static_cast<size_t>(CH) * ncol);
+      rmax = ctx->buf<double>("lr_rmax", static_cast<size_t>(CH));
+    }
+    const bool prev = L->has_prev;
+    for (int b0 = 0; b0 < total; b0 += CH) {
+      const int cb = std::min(CH, total - b0);
+      c128* Pw = Cm(PiW) + static_cast<long>(b0) * M;
+      // A = -L^H Pi L (lower tiles, then the upper triangle by reflection)
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(Pw, M), n, thcgw::bp(Lv, M, nq, 1, b0), n,
+                   kZero, Zb, M, n, cb, kZero, thcgw::GEMM_TRI_KFROM_COL);
+      thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(Lv, M, nq, 1, b0), n, thcgw::bp(Zb, M), n,
+                   kZero, A, M, n, cb, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW);
+      thcgw::fill_upper(s, A, M, n, cb);
+      // Q = orth(A Omega) by Cholesky QR; T = Q^H A Q
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, r, n, kOne, thcgw::bp(A, M), n, thcgw::bp(L->Om, 0), r, kZero, Sk,
+                   QS, r, cb);
+      thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, r, r, n, kOne, thcgw::bp(Sk, QS), r, thcgw::bp(Sk, QS), r, kZero, Gs,
+                   R2, r, cb);
+      thcgw::potrf_batched(s, Gs, R2, r, r, info, cb);
+      if (any_nonzero(info, cb, s))
+        throw thcgw::Error(THCGW_ERR_SINGULAR, "thcgw_lri_dyson: the sketch A Omega is rank deficient");
+      thcgw::copy_batch(s, Sk, QS, 1 << 30, Q, QS, QS, 0, cb);
+      thcgw::trsm_right_lower_conj(s, Gs, R2, r, Q, QS, r, n, r, cb);
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, r, n, kOne, thcgw::bp(A, M), n, thcgw::bp(Q, QS), r, kZero, AQ, QS,
+                   r, cb);
+      thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, r, r, n, kOne, thcgw::bp(Q, QS), r, thcgw::bp(AQ, QS), r, kZero, T, R2,
+                   r, cb);
+      thcgw::hermitize(s, T, r, cb);
+      c128* Qp = L->Qprev + static_cast<long>(b0) * QS;
+      c128* Cp = L->Cprev + static_cast<long>(b0) * R2;
+      // Woodbury core, and the previous exact core carried into this basis
+      woodbury_core(ctx, s, T, r, cb, Cw, W1, W2, info);
+      if (prev) {
+        thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, r, r, n, kOne, thcgw::bp(Qp, QS), r, thcgw::bp(Q, QS), r, kZero, W1,
+                     R2, r, cb);  // O = Q_prev^H Q
+        thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, r, r, r, kOne, thcgw::bp(Cp, R2), r, thcgw::bp(W1, R2), r, kZero, W2,
+                     R2, r, cb);
+        thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, r, r, r, kOne, thcgw::bp(W1, R2), r, thcgw::bp(W2, R2), r, kZero, Ctr,
+                     R2, r, cb);
+        thcgw::hermitize(s, Ctr, r, cb);
+      } else {
+        thcgw::copy_batch(s, Cw, R2, 1 << 30, Ctr, R2, R2, 0, cb);
+      }
+      // the core of the selected mode
+      const int mode = L->mode;
+      if (mode == 1) {
+        thcgw::copy_batch(s, Cw, R2, 1 << 30, Cc, R2, R2, 0, cb);
+      } else if (mode == 2) {
+        thcgw::copy_batch(s, Ctr, R2, 1 << 30, Cc, R2, R2, 0, cb);
+      } else if (mode == 3) {
+        const int nin = L->nin(), nout = L->nout();
+        float* F = L->scratch(static_cast<long>(cb) * (nin + nout));
+        float* Yn = F + static_cast<long>(cb) * nin;
+        lri_features(L, s, T, Cw, Ctr, prev, F, cb);
+        L->net->forward(s, cb, F, Yn);
+        thcgw::unpack_herm(s, Yn, nout, 1.0 / L->y_scale, Cw, Cc, R2, r, cb);
+      }
+      bool solved = false;
+      if (krylov) {
+        const long nmat = static_cast<long>(cb);
+        // B = [L^H | Q]
+        thcgw::copy2d(s, true, Lv, M, n, nq, b0 % nq, B, MB, ncol, n, n, cb);
+        thcgw::copy2d(s, false, Q, QS, r, 1 << 30, 0, B + n, MB, ncol, n, r, cb);
+        thcgw::colnorm(s, B, MB, ncol, n, ncol, bn, cb);
+        if (mode == 0) {
+          thcgw::zero_c128(s, X, nmat * MB);
+          thcgw::copy2d(s, false, B, MB, ncol, 1 << 30, 0, R, MB, ncol, n, ncol, cb);
+          thcgw::zero_c128(s, AP, nmat * MB);
+          thcgw::residual(s, B, X, AP, nullptr, MB, ncol, n, ncol, bn, rel, cb);
+        } else {
+          apply_pinv(s, Q, Cc, B, X, U, U2, n, r, ncol, cb);  // warm start X0 = P^-1 B
+          thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, ncol, n, kOne, thcgw::bp(A, M), n, thcgw::bp(X, MB), ncol,
+                       kZero, AP, MB, ncol, cb);
+          thcgw::residual(s, B, X, AP, R, MB, ncol, n, ncol, bn, rel, cb);
+        }
+        c128* Zr = mode == 0 ? R : Zp;
+        if (mode != 0) apply_pinv(s, Q, Cc, R, Zp, U, U2, n, r, ncol, cb);
+        thcgw::cg_init(s, R, Zr, P, MB, ncol, n, ncol, rz, cb);
+        int it = 0;
+        for (; it < L->max_iter; ++it) {
+          thcgw::rowmax(s, rel, ncol, rmax, cb);
+          if (host_max(rmax, cb, s) <= L->cg_tol) break;
+          thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, ncol, n, kOne, thcgw::bp(A, M), n, thcgw::bp(P, MB), ncol, kZero,
+                       AP, MB, ncol, cb);
+          thcgw::cg_step(s, X, R, P, AP, MB, ncol, n, ncol, rz, bn, rel, cb);
+          if (mode != 0) apply_pinv(s, Q, Cc, R, Zp, U, U2, n, r, ncol, cb);
+          thcgw::cg_dir(s, R, Zr, P, MB, ncol, n, ncol, rz, cb);
+        }
+        // acceptance: true FP64 residual of every column
+        thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, ncol, n, kOne, thcgw::bp(A, M), n, thcgw::bp(X, MB), ncol, kZero,
+                     AP, MB, ncol, cb);
+        thcgw::residual(s, B, X, AP, nullptr, MB, ncol, n, ncol, bn, rel, cb);
+        thcgw::rowmax(s, rel, ncol, rmax, cb);
+        const std::vector<double> res = host_vec(rmax, cb, s);
+        int bad = 0;
+        double worst = 0.0;
+        for (double v : res) {
+          if (!(v <= L->tol)) ++bad;
+          else worst = std::max(worst, v);
+        }
+        iter_sum += it;
+        ++chunks;
+        acc.iterations = std::max(acc.iterations, it);
+        if (bad == 0) {
+          // W = L X_B - V;  C* = I - Q^H X_Q
+          thcgw::copy_batch(s, C(V), M, nq, Pw, M, M, b0, cb);
+          thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(Lv, M, nq, 1, b0), n, thcgw::bp(X, MB),
+                       ncol, kMinusOne, Pw, M, n, cb);
+          thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, r, r, n, kMinusOne, thcgw::bp(Q, QS), r, thcgw::bp(X + n, MB), ncol,
+                       kZero, Cst, R2, r, cb, kOne);
+          thcgw::hermitize(s, Cst, r, cb);
+          acc.krylov_accepted += cb;
+          acc.resid_max = std::max(acc.resid_max, worst);
+          solved = true;
+        } else {
+          acc.fallbacks += cb;
+        }
+      }
+      if (!solved) {
+        // direct: M = I + A = L_M L_M^H; W = K K^H - V, K = L L_M^-H; C* = I - Y Y^H, Y = Q^H L_M^-H
+        thcgw::add_diag(s, A, M, n, n, 1.0, cb);
+        thcgw::potrf_batched(s, A, M, n, n, info, cb);
+        if (any_nonzero(info, cb, s))
+          throw thcgw::Error(THCGW_ERR_SINGULAR, "thcgw_lri_dyson: I - L^H Pi L is not positive definite");
+        thcgw::copy_batch(s, Lv, M, nq, Zb, M, M, b0, cb);
+        thcgw::trsm_right_lower_conj(s, A, M, n, Zb, M, n, n, n, cb);
+        thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Zb, M), n, thcgw::bp(Zb, M), n, kZero, Pw, M,
+                     n, cb, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB, thcgw::bp(C(V), M, nq, 1, b0));
+        thcgw::copy2d(s, true, Q, QS, r, 1 << 30, 0, Sk, QS, n, r, n, cb);  // Q^H (r x n)
+        thcgw::trsm_right_lower_conj(s, A, M, n, Sk, QS, n, r, n, cb);
+        thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, r, r, n, kMinusOne, thcgw::bp(Sk, QS), n, thcgw::bp(Sk, QS), n, kZero,
+                     Cst, R2, r, cb, kOne);
+        thcgw::hermitize(s, Cst, r, cb);
+        if (st && mode != 0) core_err_sums(Cc, Cst, static_cast<long>(cb) * R2, s, core_num, core_den);
+      }
+      if (record & 1) lri_record(L, s, T, Cw, Ctr, prev, Cst, cb);
+      if (!(record & 2)) {
+        thcgw::copy_batch(s, Q, QS, 1 << 30, Qp, QS, QS, 0, cb);
+        thcgw::copy_batch(s, Cst, R2, 1 << 30, Cp, R2, R2, 0, cb);
+      }
+      acc.solves += cb;
+    }
+    if (!(record & 2)) L->has_prev = true;
+    THCGW_HIP(hipStreamSynchronize(s));
+    if (st) {
+      acc.iter_mean = chunks ? iter_sum / chunks : 0.0;
+      acc.core_err = core_den > 0.0 ? std::sqrt(core_num / core_den) : 0.0;
+      *st = acc;
+    }
+  });
+}
+
+int thcgw_lri_train(thcgw_lri* L, int epochs, int batch, float lr, unsigned long long seed, float* loss_first,
+                    float* loss_last, void* stream) {
+  if (!L) return THCGW_ERR_ARG;
+  return guard_err(L->err, L->ctx->device, [&] {
+    THCGW_REQUIRE(L->n_pairs > 0, "thcgw_lri_train: no recorded training pairs");
+    THCGW_REQUIRE(epochs >= 1 && batch >= 1 && lr > 0.0f, "thcgw_lri_train: epochs, batch and lr must be positive");
+    hipStream_t s = S(stream);
+    const int nin = L->nin(), nout = L->nout();
+    const long np = L->n_pairs;
+    if (!L->y_scaled) {  // targets scaled to unit RMS once, before the first step
+      std::vector<float> h(static_cast<size_t>(np) * nout);
+      THCGW_HIP(hipMemcpyAsync(h.data(), L->targ, sizeof(float) * h.size(), hipMemcpyDeviceToHost, s));
+      THCGW_HIP(hipStreamSynchronize(s));
+      double ss = 0.0;
+      for (float v : h) ss += double(v) * v;
+      const double rms = std::sqrt(ss / double(h.size()));
+      const double a = rms > 0.0 ? 1.0 / rms : 1.0;
+      thcgw::scale_f32(s, L->targ, static_cast<long>(h.size()), static_cast<float>(a));
+      L->y_scale *= a;
+      L->y_scaled = true;
+    }
+    const int bs = static_cast<int>(std::min<long>(batch, np));
+    const int steps = static_cast<int>((np + bs - 1) / bs);
+    float* buf = L->scratch(static_cast<long>(bs) * (nin + 3L * nout));
+    float *Xb = buf, *Tb = Xb + static_cast<long>(bs) * nin, *Yb = Tb + static_cast<long>(bs) * nout,
+          *dYb = Yb + static_cast<long>(bs) * nout;
+    int* d_idx = L->ctx->buf<int>("lr_idx", static_cast<size_t>(bs));
+    float* d_loss = L->ctx->buf<float>("lr_loss", static_cast<size_t>(epochs) * steps);
+    std::vector<int> perm(np);
+    for (long i = 0; i < np; ++i) perm[i] = static_cast<int>(i);
+    std::mt19937_64 rng(seed);
+    for (int e = 0; e < epochs; ++e) {
+      std::shuffle(perm.begin(), perm.end(), rng);
+      for (int st = 0; st < steps; ++st) {
+        const long lo = static_cast<long>(st) * bs;
+        const int cnt = static_cast<int>(std::min<long>(bs, np - lo));
+        THCGW_HIP(hipMemcpyAsync(d_idx, perm.data() + lo, sizeof(int) * cnt, hipMemcpyHostToDevice, s));
+        thcgw::gather_rows(s, L->feat, nin, d_idx, Xb, cnt);
+        thcgw::gather_rows(s, L->targ, nout, d_idx, Tb, cnt);
+        L->net->train_mse(s, cnt, Xb, Tb, lr, Yb, dYb);
+        THCGW_HIP(hipMemcpyAsync(d_loss + static_cast<long>(e) * steps + st, L->net->loss, sizeof(float),
+                                 hipMemcpyDeviceToDevice, s));
+        THCGW_HIP(hipStreamSynchronize(s));  // the index array is reused by the next step
+      }
+    }
+    std::vector<float> h(static_cast<size_t>(epochs) * steps);
+    THCGW_HIP(hipMemcpyAsync(h.data(), d_loss, sizeof(float) * h.size(), hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    double f = 0.0, l = 0.0;
+    for (int st = 0; st < steps; ++st) {
+      f += h[st];
+      l += h[static_cast<size_t>(epochs - 1) * steps + st];
+    }
+    if (loss_first) *loss_first = static_cast<float>(f / steps);
+    if (loss_last) *loss_last = static_cast<float>(l / steps);
+  });
+}
+
+// ---- the network alone (checks of its kernels) ----
+
+int thcgw_mlp_create(int n_in, int width, int n_out, unsigned long long seed, thcgw_mlp** out) {
+  if (!out) return THCGW_ERR_ARG;
+  *out = nullptr;
+  auto m = std::make_unique<thcgw_mlp>();
+  (void)hipGetDevice(&m->device);
+  const int rc = guard_err(g_create_error, m->device, [&] { m->net = std::make_unique<thcgw::Mlp>(n_in, width, n_out, seed); });
+  if (rc == THCGW_OK) *out = m.release();
+  return rc;
+}
+
+void thcgw_mlp_destroy(thcgw_mlp* m) { delete m; }
+
+const char* thcgw_mlp_last_error(const thcgw_mlp* m) { return m ? m->err.c_str() : g_create_error.c_str(); }
+
+long thcgw_mlp_num_params(const thcgw_mlp* m) { return m ? m->net->n_params : 0; }
+
+int thcgw_mlp_get_params(thcgw_mlp* m, float* host) {
+  if (!m || !host) return THCGW_ERR_ARG;
+  return guard_err(m->err, m->device, [&] { m->net->get_params(host, nullptr); });
+}
+
+int thcgw_mlp_set_params(thcgw_mlp* m, const float* host) {
+  if (!m || !host) return THCGW_ERR_ARG;
+  return guard_err(m->err, m->device, [&] {
+    m->net->set_params(host, nullptr);
+    THCGW_HIP(hipStreamSynchronize(nullptr));
+  });
+}
+
+int thcgw_mlp_forward(thcgw_mlp* m, int batch, const float* X, float* Y, void* stream) {
+  if (!m || !X || !Y) return THCGW_ERR_ARG;
+  return guard_err(m->err, m->device, [&] { m->net->forward(S(stream), batch, X, Y); });
+}
+
+int thcgw_mlp_backward(thcgw_mlp* m, int batch, const float* X, const float* dY, float* grad, void* stream) {
+  if (!m || !X || !dY || !grad) return THCGW_ERR_ARG;
+  return guard_err(m->err, m->device, [&] {
+    hipStream_t s = S(stream);
+    m->net->backward(s, batch, X, dY);
+    THCGW_HIP(hipMemcpyAsync(grad, m->net->g, sizeof(float) * m->net->n_params, hipMemcpyDeviceToDevice, s));
+  });
+}
+
+int thcgw_mlp_adam_step(thcgw_mlp* m, const float* grad, float lr, void* stream) {
+  if (!m || !grad) return THCGW_ERR_ARG;
+  return guard_err(m->err, m->device, [&] { m->net->adam(S(stream), grad, lr); });
+}
+
+int thcgw_mlp_train_mse(thcgw_mlp* m, int batch, const float* X, const float* T, float lr, float* loss, void* stream) {
+  if (!m || !X || !T) return THCGW_ERR_ARG;
+  return guard_err(m->err, m->device, [&] {
+    hipStream_t s = S(stream);
+    const long need = 2L * batch * m->net->n_out;
+    if (need > m->buf_count) {
+      if (m->buf) THCGW_HIP(hipFree(m->buf));
+      m->buf = nullptr;
+      m->buf_count = 0;
+      THCGW_HIP(hipMalloc(&m->buf, sizeof(float) * need));
+      m->buf_count = need;
+    }
+    m->net->train_mse(s, batch, X, T, lr, m->buf, m->buf + static_cast<long>(batch) * m->net->n_out);
+    if (loss) {
+      THCGW_HIP(hipMemcpyAsync(loss, m->net->loss, sizeof(float), hipMemcpyDeviceToHost, s));
+      THCGW_HIP(hipStreamSynchronize(s));
+    }
   });
 }
 

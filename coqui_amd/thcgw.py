@@ -23,6 +23,11 @@ EXPORTS = (
     "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf", "thcgw_dyson_g",
     "thcgw_nelec", "thcgw_update_mu", "thcgw_batch_create", "thcgw_batch_destroy", "thcgw_batch_last_error",
     "thcgw_batch_set", "thcgw_batch_gw_iteration", "thcgw_pi_kspace", "thcgw_sigma_kspace",
+    "thcgw_lri_create", "thcgw_lri_destroy", "thcgw_lri_last_error", "thcgw_lri_set", "thcgw_lri_dyson",
+    "thcgw_lri_train", "thcgw_lri_num_pairs", "thcgw_lri_clear_pairs", "thcgw_lri_reset_state",
+    "thcgw_mlp_create", "thcgw_mlp_destroy", "thcgw_mlp_last_error", "thcgw_mlp_num_params",
+    "thcgw_mlp_get_params", "thcgw_mlp_set_params", "thcgw_mlp_forward", "thcgw_mlp_backward",
+    "thcgw_mlp_adam_step", "thcgw_mlp_train_mse",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -43,6 +48,16 @@ class _Tables(ctypes.Structure):
     _fields_ = [("tau_mesh_f", ctypes.c_void_p), ("wn_mesh_f", ctypes.c_void_p), ("Ttw_ff", ctypes.c_void_p),
                 ("Twt_ff", ctypes.c_void_p), ("Ttw_bb", ctypes.c_void_p), ("Twt_bb", ctypes.c_void_p),
                 ("T_beta_t_ff", ctypes.c_void_p)]
+
+
+class LriStats(ctypes.Structure):
+    """thcgw_lri_stats"""
+    _fields_ = [("solves", ctypes.c_int), ("krylov_accepted", ctypes.c_int), ("fallbacks", ctypes.c_int),
+                ("iterations", ctypes.c_int), ("iter_mean", ctypes.c_double), ("resid_max", ctypes.c_double),
+                ("core_err", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -89,6 +104,26 @@ def load_library(path=LIB_PATH):
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
         "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
+        "thcgw_lri_create": (i, [p, i, i, ctypes.c_ulonglong, ctypes.POINTER(p)]),
+        "thcgw_lri_destroy": (None, [p]),
+        "thcgw_lri_last_error": (ctypes.c_char_p, [p]),
+        "thcgw_lri_set": (i, [p, i, i, d]),
+        "thcgw_lri_dyson": (i, [p, p, p, i, i, ctypes.POINTER(LriStats), p]),
+        "thcgw_lri_train": (i, [p, i, i, ctypes.c_float, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float), p]),
+        "thcgw_lri_num_pairs": (l, [p]),
+        "thcgw_lri_clear_pairs": (i, [p]),
+        "thcgw_lri_reset_state": (i, [p]),
+        "thcgw_mlp_create": (i, [i, i, i, ctypes.c_ulonglong, ctypes.POINTER(p)]),
+        "thcgw_mlp_destroy": (None, [p]),
+        "thcgw_mlp_last_error": (ctypes.c_char_p, [p]),
+        "thcgw_mlp_num_params": (l, [p]),
+        "thcgw_mlp_get_params": (i, [p, p]),
+        "thcgw_mlp_set_params": (i, [p, p]),
+        "thcgw_mlp_forward": (i, [p, i, p, p, p]),
+        "thcgw_mlp_backward": (i, [p, i, p, p, p, p]),
+        "thcgw_mlp_adam_step": (i, [p, p, ctypes.c_float, p]),
+        "thcgw_mlp_train_mse": (i, [p, i, p, p, ctypes.c_float, ctypes.POINTER(ctypes.c_float), p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -391,3 +426,152 @@ class ThcBatch:
         self._check(self.lib.thcgw_batch_gw_iteration(self.h, mk(G), mk(X), mk(V), mk(Sigma), mk(W_qt), ec,
                                                        int(bool(hermitize)), ctypes.c_void_p(s.cuda_stream)))
         return list(ec) if with_ec else None
+
+
+class LearnedDyson:
+    """Learned low-rank approximate inverse for the Dyson solve (thcgw_lri_*): a Krylov
+    (preconditioned CG, FP64) replacement of the direct Hermitian solve, with the exact
+    direct solve as fallback whenever a column's true residual exceeds tol.
+
+    mode 0: no preconditioner; 1: Woodbury core on the sketch basis; 2: previous exact
+    core; 3: previous core plus the network's correction (the learned form)."""
+
+    MODES = {"none": 0, "woodbury": 1, "previous": 2, "learned": 3}
+
+    def __init__(self, ctx, rank=64, width=512, seed=20261202):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self._torch = ctx._torch
+        self.rank, self.width = int(rank), int(width)
+        h = ctypes.c_void_p()
+        rc = self.lib.thcgw_lri_create(ctx.h, self.rank, self.width, ctypes.c_ulonglong(seed), ctypes.byref(h))
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_last_error(ctx.h).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_lri_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.thcgw_lri_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, mode="learned", max_iter=200, tol=1e-10):
+        m = self.MODES[mode] if isinstance(mode, str) else int(mode)
+        self._check(self.lib.thcgw_lri_set(self.h, m, int(max_iter), ctypes.c_double(tol)))
+
+    def dyson(self, PiW, V, krylov=True, record=False, keep_state=False, stream=None):
+        """In place PiW <- W - V; returns the call's statistics (dict).  record stores
+        training pairs; keep_state leaves the previous-iteration state untouched."""
+        PiW, V = self.ctx._dev(PiW), self.ctx._dev(V)
+        st = LriStats()
+        flags = int(bool(record)) | (2 if keep_state else 0)
+        self._check(self.lib.thcgw_lri_dyson(self.h, _ptr(PiW), _ptr(V), int(bool(krylov)), flags,
+                                             ctypes.byref(st), self.ctx._stream(stream)))
+        return st.as_dict()
+
+    def train(self, epochs=30, batch=256, lr=1e-3, seed=7, stream=None):
+        """Adam on the recorded pairs; returns (mean loss of the first epoch, of the last)."""
+        f, l = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.thcgw_lri_train(self.h, int(epochs), int(batch), ctypes.c_float(lr),
+                                             ctypes.c_ulonglong(seed), ctypes.byref(f), ctypes.byref(l),
+                                             self.ctx._stream(stream)))
+        return f.value, l.value
+
+    def num_pairs(self):
+        return int(self.lib.thcgw_lri_num_pairs(self.h))
+
+    def clear_pairs(self):
+        self._check(self.lib.thcgw_lri_clear_pairs(self.h))
+
+    def reset_state(self):
+        self._check(self.lib.thcgw_lri_reset_state(self.h))
+
+
+class Mlp:
+    """The network of the learned form on its own (thcgw_mlp_*), for checks of its kernels.
+    Inputs and outputs are float32 torch tensors on the GPU."""
+
+    def __init__(self, n_in, width, n_out, seed=1):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("coqui_amd.Mlp needs a HIP device; there is no CPU fallback")
+        self._torch = torch
+        self.lib = load_library()
+        torch.cuda.current_device()
+        h = ctypes.c_void_p()
+        rc = self.lib.thcgw_mlp_create(int(n_in), int(width), int(n_out), ctypes.c_ulonglong(seed), ctypes.byref(h))
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_mlp_last_error(None).decode())
+        self.h = h
+        self.n_in, self.width, self.n_out = int(n_in), int(width), int(n_out)
+        self.n_params = int(self.lib.thcgw_mlp_num_params(h))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ThcgwError(rc, self.lib.thcgw_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.thcgw_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        return ctypes.c_void_p(self._torch.cuda.current_stream().cuda_stream)
+
+    def params(self):
+        """FP32 master parameters W0, b0, W1, b1, W2, b2 as a flat numpy array."""
+        h = np.empty(self.n_params, np.float32)
+        self._check(self.lib.thcgw_mlp_get_params(self.h, _np_ptr(h)))
+        return h
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        assert flat.size == self.n_params
+        self._check(self.lib.thcgw_mlp_set_params(self.h, _np_ptr(flat)))
+
+    def _f32(self, a):
+        t = self._torch
+        if isinstance(a, t.Tensor):
+            assert a.is_cuda and a.dtype == t.float32 and a.is_contiguous()
+            return a
+        return t.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+    def forward(self, X):
+        X = self._f32(X)
+        Y = self._torch.empty((X.shape[0], self.n_out), dtype=self._torch.float32, device="cuda")
+        self._check(self.lib.thcgw_mlp_forward(self.h, int(X.shape[0]), _ptr(X), _ptr(Y), self._stream()))
+        return Y
+
+    def backward(self, X, dY):
+        """Gradient of sum(Y * dY) w.r.t. the flat parameters, after forward(X)."""
+        X, dY = self._f32(X), self._f32(dY)
+        g = self._torch.empty(self.n_params, dtype=self._torch.float32, device="cuda")
+        self._check(self.lib.thcgw_mlp_backward(self.h, int(X.shape[0]), _ptr(X), _ptr(dY), _ptr(g), self._stream()))
+        return g
+
+    def adam_step(self, grad, lr):
+        grad = self._f32(grad)
+        self._check(self.lib.thcgw_mlp_adam_step(self.h, _ptr(grad), ctypes.c_float(lr), self._stream()))
+
+    def train_mse(self, X, T, lr):
+        X, T = self._f32(X), self._f32(T)
+        loss = ctypes.c_float()
+        self._check(self.lib.thcgw_mlp_train_mse(self.h, int(X.shape[0]), _ptr(X), _ptr(T), ctypes.c_float(lr),
+                                                 ctypes.byref(loss), self._stream()))
+        return loss.value

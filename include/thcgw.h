@@ -227,6 +227,70 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G_tskij, const
                              const double* const* V_qPQ, double* const* Sigma_tskij, double* const* W_qtPQ,
                              double* Ec, int hermitize, void* stream);
 
+/* Learned form of the Dyson solve (no counterpart in the reference; it replaces the
+ * getrf/getri of scr_coulomb_t.cpp:180-196 inside a self-consistent loop).  With
+ * V = L L^H and A = -L^H Pi(iW_n) L, W - V = L M^-1 L^H - V, M = I + A.  The Krylov solve
+ * runs preconditioned conjugate gradients in FP64 on M X = [L^H | Q] column by column,
+ * where Q is an orthonormal basis (n x rank) of the sketch A Omega (Omega a fixed
+ * Gaussian) and the preconditioner is the rank-`rank` approximate inverse
+ * P^-1 = I - Q C Q^H, also used as the warm start X0 = P^-1 B.  The core C comes from
+ *   mode 0: none (C = 0, X0 = 0); mode 1: Woodbury on span(Q), C_W = T (I+T)^-1 with
+ *   T = Q^H A Q; mode 2: the exact core C* = I - Q^H M^-1 Q of the previous call carried
+ *   into the current basis, C_tr = O^H C*_prev O with O = Q_prev^H Q (mode 1 on the
+ *   first call); mode 3 (default): C_W plus the correction predicted by a three-layer
+ *   network (width `width`, BF16 weights, FP32 accumulation, SiLU) from (T, C_W, C_tr)
+ *   -- trained by thcgw_lri_train on pairs recorded by direct solves.
+ * A Krylov solve is accepted only when every column's true FP64 residual
+ * ||B - M X|| / ||B|| is <= tol; otherwise its chunk is solved directly (Cholesky, as
+ * flag 6 = 1).  krylov = 0 always solves directly.  record bit 0: store the pair
+ * (features, C* - C_W) of every (n, q) for training; bit 1: keep the previous state
+ * (otherwise every call leaves the exact core and the basis Q of each (n, q) as the
+ * "previous" state of the next call, as one iteration of a self-consistent loop).  PiW and V as
+ * in thcgw_dyson_w; synchronises.  stats may be NULL; core_err is
+ * ||C_mode - C*||_F / ||C*||_F over the call's direct solves. */
+typedef struct thcgw_lri thcgw_lri;
+typedef struct thcgw_lri_stats {
+  int solves;           /* (n, q) matrices processed */
+  int krylov_accepted;  /* of which solved by the Krylov form */
+  int fallbacks;        /* of which solved directly after the residual check failed */
+  int iterations;       /* largest CG iteration count of a chunk */
+  double iter_mean;     /* mean CG iterations per chunk */
+  double resid_max;     /* largest accepted relative residual */
+  double core_err;      /* see above */
+} thcgw_lri_stats;
+int thcgw_lri_create(thcgw_ctx* ctx, int rank, int width, unsigned long long seed, thcgw_lri** out);
+void thcgw_lri_destroy(thcgw_lri* L);
+const char* thcgw_lri_last_error(const thcgw_lri* L);
+int thcgw_lri_set(thcgw_lri* L, int mode, int max_iter, double tol);
+int thcgw_lri_dyson(thcgw_lri* L, double* PiW_wqPQ, const double* V_qPQ, int krylov, int record,
+                    thcgw_lri_stats* stats, void* stream);
+/* Adam on the mean squared error of the recorded pairs (targets scaled to unit RMS at the
+ * first call), shuffled minibatches of `batch`; mean loss of the first and last epoch. */
+int thcgw_lri_train(thcgw_lri* L, int epochs, int batch, float lr, unsigned long long seed, float* loss_first,
+                    float* loss_last, void* stream);
+long thcgw_lri_num_pairs(const thcgw_lri* L);
+int thcgw_lri_clear_pairs(thcgw_lri* L);
+int thcgw_lri_reset_state(thcgw_lri* L);
+
+/* The network alone, for checks of its kernels: y = silu(silu(x W0^T + b0) W1^T + b1) W2^T + b2
+ * (row-major batch x n_in input); parameters as one FP32 array W0, b0, W1, b1, W2, b2
+ * (torch.nn.Linear layout) with a BF16 mirror used by the products.  backward writes the
+ * gradient of sum(Y * dY) w.r.t. every parameter (device array of num_params floats) for
+ * the X of the last forward; adam_step applies one Adam step (beta 0.9/0.999, eps 1e-8);
+ * train_mse does forward, MSE gradient, backward and Adam, returning the loss before the
+ * step. */
+typedef struct thcgw_mlp thcgw_mlp;
+int thcgw_mlp_create(int n_in, int width, int n_out, unsigned long long seed, thcgw_mlp** out);
+void thcgw_mlp_destroy(thcgw_mlp* m);
+const char* thcgw_mlp_last_error(const thcgw_mlp* m);
+long thcgw_mlp_num_params(const thcgw_mlp* m);
+int thcgw_mlp_get_params(thcgw_mlp* m, float* host);
+int thcgw_mlp_set_params(thcgw_mlp* m, const float* host);
+int thcgw_mlp_forward(thcgw_mlp* m, int batch, const float* X, float* Y, void* stream);
+int thcgw_mlp_backward(thcgw_mlp* m, int batch, const float* X, const float* dY, float* grad, void* stream);
+int thcgw_mlp_adam_step(thcgw_mlp* m, const float* grad, float lr, void* stream);
+int thcgw_mlp_train_mse(thcgw_mlp* m, int batch, const float* X, const float* T, float lr, float* loss, void* stream);
+
 /* Diagnostics (no counterpart in the reference): bytes of device workspace held by ctx,
  * control flags and GEMM launch timing.
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;

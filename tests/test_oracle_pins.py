@@ -414,3 +414,59 @@ def test_gm_energy_diagonal(orc):
     f = lambda x: 1.0 / (1.0 + np.exp(beta * x))
     ref = s * (f(e1) - f(e2)) / (e1 - e2)
     assert abs(e - ref) < 1e-9
+
+
+# ---- the learned form's network (oracle/mlp_oracle.py) ----
+
+def test_mlp_oracle_backward_finite_differences():
+    """backward() against central differences of forward() in float64 (no BF16 rounding)."""
+    from oracle import mlp_oracle as mo
+
+    dims = (7, 11, 5)
+    rng = np.random.default_rng(3)
+    _, npar = mo.layout(*dims)
+    p = rng.standard_normal(npar) * 0.5
+    X = rng.standard_normal((13, dims[0]))
+    dY = rng.standard_normal((13, dims[2]))
+    loss = lambda q: float(np.sum(mo.forward(q, dims, X, round_bf16=False)[0] * dY))
+    _, cache = mo.forward(p, dims, X, round_bf16=False)
+    g = mo.backward(cache, dims, dY, round_bf16=False)
+    h = 1e-5
+    for _ in range(6):  # random directions
+        d = rng.standard_normal(npar)
+        fd = (loss(p + h * d) - loss(p - h * d)) / (2 * h)
+        assert abs(fd - g @ d) < 1e-7 * max(1.0, abs(fd))
+    for idx in (0, npar // 3, npar // 2, npar - 1):  # single parameters in W0, W1, W2, b2
+        e = np.zeros(npar)
+        e[idx] = 1.0
+        fd = (loss(p + h * e) - loss(p - h * e)) / (2 * h)
+        assert abs(fd - g[idx]) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_mlp_oracle_bf16_rounding():
+    """Round-to-nearest-even at the halfway points, and torch's bfloat16 conversion."""
+    import torch
+
+    from oracle import mlp_oracle as mo
+
+    ulp = 2.0 ** -7
+    cases = {1.0: 1.0, 1.0 + ulp / 2: 1.0, 1.0 + 1.5 * ulp: 1.0 + 2 * ulp, -(1.0 + ulp / 2): -1.0,
+             3.0 + 2 * ulp * 0.49: 3.0}
+    for x, want in cases.items():
+        assert mo.bf16(np.array([x]))[0] == want
+    x = np.random.default_rng(0).standard_normal(10000).astype(np.float32) * 100
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(mo.bf16(x), ref)
+
+
+def test_adam_oracle_closed_form():
+    """First step moves every weight by lr g / (|g| + eps); a constant gradient keeps m_hat = g."""
+    from oracle import mlp_oracle as mo
+
+    g = np.array([0.3, -2.0, 1e-3])
+    w, m, v = mo.adam(np.zeros(3), np.zeros(3), np.zeros(3), g, 1, 0.01)
+    assert np.allclose(w, -0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=0)
+    for step in range(2, 6):
+        w, m, v = mo.adam(w, m, v, g, step, 0.01)
+    assert np.allclose(m / (1 - 0.9 ** 5), g, rtol=1e-12)
+    assert np.allclose(np.sqrt(v / (1 - 0.999 ** 5)), np.abs(g), rtol=1e-12)
