@@ -42,6 +42,7 @@ CONFIGS = {
     "cfg1": ((2, 2, 2), 26, 200, "1e2.1e-15"),
     "cfg3": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
     "cfg4": None,  # batched: see batch_systems()
+    "cfg5": ((4, 4, 4), 78, 1200, "1e3.1e-15"),  # self-consistent GW, learned Dyson form: see run_scgw()
 }
 # The batched workload asks for N_tau = 60; the nearest shipped table (1e5.1e-06, 61 points)
 # makes the Pade continuation of the QP equation move by tens of meV under 1e-7 relative
@@ -168,6 +169,8 @@ def main():
     ap.add_argument("--gemm-tile32", type=int, default=0, help="FP32 GEMM tile shape: 0 64x64, 1 128x64, 2 128x128")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
+    ap.add_argument("--scgw-iters", type=int, default=10, help="cfg5: self-consistent iterations")
+    ap.add_argument("--krylov-max-iter", type=int, default=60, help="cfg5: CG iterations before the direct fallback")
     ap.add_argument("--forms", default="plain,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
     args = ap.parse_args()
@@ -189,6 +192,8 @@ def main():
 
     if args.config == "cfg4":
         return run_batch(args)
+    if args.config == "cfg5":
+        return run_scgw(args)
     kgrid, no, nmu, table = CONFIGS[args.config]
     beta = 1000.0
     log("bench: config %s kgrid=%s no=%d nmu=%d table=%s" % (args.config, kgrid, no, nmu, table))
@@ -472,6 +477,91 @@ def run_batch(args):
     b.close()
     for d in sysd:
         d["ctx"].close()
+
+
+def run_scgw(args):
+    """Learned form: self-consistent GW (scf_driver.cpp:96-170 order) for --scgw-iters
+    iterations, first with the plain FP64 Dyson solve (LU, the reference algorithm), then
+    with the learned form: direct Hermitian solves that record training pairs in iterations
+    1-2, training of the network, and from iteration 3 preconditioned CG with the learned
+    rank-64 approximate inverse, each solve accepted at a true FP64 residual <= 1e-10 or
+    redone directly.  Reports the wall time per iteration of both, the CG iterations and
+    fallbacks, and the difference of the final state (QP energies from the final Sigma,
+    Galitskii-Migdal energy, chemical potential)."""
+    import torch
+
+    from coqui_amd import IAFT, LearnedDyson, ScGW, ThcGW, make_system
+
+    kgrid, no, nmu, table = CONFIGS["cfg5"]
+    beta = 1000.0
+    torch.cuda.set_device(0)
+    ia = IAFT.from_table(beta, table)
+    s = make_system(kgrid, no, nmu, system_id=0)
+    log("bench: cfg5 scGW kgrid=%s no=%d nmu=%d nt=%d, %d iterations" % (kgrid, no, nmu, ia.nt, args.scgw_iters))
+
+    def final_state(ctx, sc):
+        H = torch.from_numpy(s.H).cuda()
+        Cd = torch.from_numpy(s.C).cuda()
+        E = ctx.solve_qp(sc.Sigma, Cd, H, s.E, mu=sc.mu)
+        return dict(E=E, mu=sc.mu, e_corr=sc.history[-1]["e_corr"], Sigma=sc.Sigma.clone())
+
+    def show(tag):
+        return lambda r: log("bench: %s it %d  %.2f s (Sigma %.2f s)  dSigma %.2e  mu %.6f  Ec(GM) %.8f  %s" % (
+            tag, r["it"], r["t_total"], r["t_sigma"], r["sigma_diff"], r["mu"], r["e_corr"],
+            "" if not r["lri"] else "CG %(iter_mean).1f fallbacks %(fallbacks)d accepted %(krylov_accepted)d" % r["lri"]))
+
+    # plain FP64 GPU path
+    ctx = ThcGW(kgrid, no, nmu, ia)
+    sc = ScGW(ctx, s)
+    hist_p = sc.run(args.scgw_iters, dyson="direct", log=show("plain"))
+    ref = final_state(ctx, sc)
+    del sc
+    ctx.close()
+    torch.cuda.empty_cache()
+
+    # learned form
+    ctx = ThcGW(kgrid, no, nmu, ia)
+    sc = ScGW(ctx, s)
+    lri = LearnedDyson(ctx, rank=64, width=512, seed=20261202)
+    hist_l = sc.run(args.scgw_iters, dyson="learned", lri=lri, train_iters=2,
+                    train_kw=dict(epochs=30, batch=256, lr=1e-3), mode="learned",
+                    max_iter=args.krylov_max_iter, tol=1e-10, log=show("learned"))
+    got = final_state(ctx, sc)
+    kry = [r for r in hist_l if r["lri"] and r["lri"]["krylov_accepted"] + r["lri"]["fallbacks"] > 0]
+    t_plain = sum(r["t_total"] for r in hist_p)
+    t_learned = sum(r["t_total"] for r in hist_l)
+    dqp = np.abs(got["E"] - ref["E"]) / HA_PER_MEV
+    out = {
+        "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
+        "value": round(t_learned / len(hist_l), 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": len(hist_l),
+        "warmup": 0,
+        "ms_per_step": round(1000.0 * t_learned / len(hist_l), 1),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (FP64 Krylov and direct solves); network BF16 weights, FP32 accumulation",
+        "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
+        "config": {"workload": "cfg5 self-consistent GW, learned Krylov Dyson form", "kgrid": list(kgrid),
+                   "n_orb": no, "n_mu": nmu, "ir_table": table, "n_tau": ia.nt, "beta": beta,
+                   "iterations": args.scgw_iters, "rank": 64, "width": 512, "parallelism": "single GPU"},
+        "plain_s_per_iteration": round(t_plain / len(hist_p), 4),
+        "speedup_vs_plain": round(t_plain / t_learned, 3),
+        "learned_krylov_s_per_iteration": round(sum(r["t_total"] for r in kry) / max(1, len(kry)), 4),
+        "cg_iterations_mean": [round(r["lri"]["iter_mean"], 1) for r in kry],
+        "fallbacks": sum(r["lri"]["fallbacks"] for r in kry),
+        "krylov_accepted": sum(r["lri"]["krylov_accepted"] for r in kry),
+        "resid_max": max([r["lri"]["resid_max"] for r in kry] or [0.0]),
+        "training": getattr(sc, "train_info", None),
+        "errors_vs_plain": {"qp_meV_max": float(dqp.max()), "qp_meV_median": float(np.median(dqp)),
+                            "e_corr_gm_Ha": abs(got["e_corr"] - ref["e_corr"]), "mu_Ha": abs(got["mu"] - ref["mu"]),
+                            "Sigma_rel": float((got["Sigma"] - ref["Sigma"]).abs().max() / ref["Sigma"].abs().max())},
+    }
+    print(json.dumps(out), flush=True)
+    lri.close()
+    ctx.close()
 
 
 if __name__ == "__main__":
