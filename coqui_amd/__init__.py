@@ -9,7 +9,8 @@ from .iaft import IAFT, available_tables, determine_lambda  # noqa: F401
 from .system import System, make_system  # noqa: F401
 from .thcgw import EXPORTS, LIB_PATH, LearnedDyson, Mlp, ThcBatch, ThcGW, ThcgwError, load_library  # noqa: F401
 from .scgw import ScGW  # noqa: F401
+from .eri import read_thc_eri, write_thc_eri  # noqa: F401
 
 __all__ = ["IAFT", "available_tables", "determine_lambda", "System", "make_system", "ThcGW", "ThcBatch", "ThcgwError",
-           "LearnedDyson", "Mlp", "ScGW",
+           "LearnedDyson", "Mlp", "ScGW", "read_thc_eri", "write_thc_eri",
            "load_library", "EXPORTS", "LIB_PATH"]

@@ -1,4 +1,5 @@
-"""The C-ABI library loads and exports every symbol declared in include/thcgw.h (no GPU)."""
+"""The C-ABI libraries load and export every symbol declared in include/thcgw.h
+(libthcgw.so) and include/thcgw_eri.h (libthcgw_eri.so); no GPU."""
 import ctypes
 import os
 import re
@@ -8,8 +9,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    txt = open(os.path.join(ROOT, "include", "thcgw.h")).read()
+def header_functions(header="thcgw.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(thcgw_[a-z_0-9A-Z]+)\s*\(", txt)))
 
@@ -44,3 +45,14 @@ def test_last_error_without_context():
 
     lib = load_library()
     assert lib.thcgw_last_error(None) == b""
+
+
+def test_eri_library_exports_all_symbols():
+    from coqui_amd.eri import ERI_EXPORTS, ERI_LIB_PATH, load_eri_library
+
+    if not os.path.exists(ERI_LIB_PATH):
+        pytest.fail("libthcgw_eri.so not built: run __graft_entry__.build()")
+    assert sorted(ERI_EXPORTS) == header_functions("thcgw_eri.h")
+    lib = load_eri_library()
+    for f in header_functions("thcgw_eri.h"):
+        assert getattr(lib, f).argtypes is not None
