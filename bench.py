@@ -10,7 +10,9 @@ Forms (--forms, the first is the headline value):
   hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
              transforms, Cholesky factorisations and ln det;
-  mixed_bf16 as mixed, with the FP32 products done as split-BF16 MFMA products.
+  mixed_bf16 as mixed, with the FP32 products done as split-BF16 MFMA products;
+  *_trev     the form with the Dyson step on the time-reversal irreducible q only (needs
+             --trev, which draws the system time-reversal symmetric).
 Every further form is reported with its speedup over the first and its errors against it
 (W at q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
 
@@ -65,8 +67,9 @@ def batch_systems(n=24):
     return out
 
 
-# form: (flag 6 Dyson algorithm, flag 7 precision)
-FORMS = {"plain": (0, 0), "hermitian": (1, 0), "mixed": (1, 1), "mixed_bf16": (1, 2)}
+# form: (flag 6 Dyson algorithm, flag 7 precision, flag 9 time reversal)
+FORMS = {"plain": (0, 0, 0), "hermitian": (1, 0, 0), "mixed": (1, 1, 0), "mixed_bf16": (1, 2, 0),
+         "plain_trev": (0, 0, 1), "hermitian_trev": (1, 0, 1), "mixed_trev": (1, 1, 1)}
 HA_PER_MEV = 1.0 / 27211.386245988
 
 
@@ -170,6 +173,7 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
     ap.add_argument("--scgw-iters", type=int, default=10, help="cfg5: self-consistent iterations")
+    ap.add_argument("--trev", action="store_true", help="draw the system time-reversal symmetric (for *_trev forms)")
     ap.add_argument("--krylov-max-iter", type=int, default=60, help="cfg5: CG iterations before the direct fallback")
     ap.add_argument("--forms", default="plain,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
@@ -178,6 +182,8 @@ def main():
     for f in forms:
         if f not in FORMS:
             ap.error("unknown form %r (choose from %s)" % (f, ", ".join(FORMS)))
+        if FORMS[f][2] and not args.trev:
+            ap.error("form %r needs --trev" % f)
     if args.cpu_child:
         cpu_baseline_child(args.cpu_child)
         return
@@ -199,7 +205,7 @@ def main():
     log("bench: config %s kgrid=%s no=%d nmu=%d table=%s" % (args.config, kgrid, no, nmu, table))
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    s = make_system(kgrid, no, nmu, system_id=0, with_v=False)
+    s = make_system(kgrid, no, nmu, system_id=0, with_v=False, trev=args.trev)
     ia = IAFT.from_table(beta, table)
     ctx = ThcGW(kgrid, no, nmu, ia)
     ctx.set_flag(5, args.gemm_tile)
@@ -222,6 +228,7 @@ def main():
     def run_form(form, steps, warmup):
         ctx.set_flag(6, FORMS[form][0])
         ctx.set_flag(7, FORMS[form][1])
+        ctx.set_flag(9, FORMS[form][2])
         ec = None
         for i in range(warmup):
             _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
@@ -283,6 +290,7 @@ def main():
         del Sig_ref, W0_ref, H, Cd
         ctx.set_flag(6, FORMS[forms[0]][0])
         ctx.set_flag(7, FORMS[forms[0]][1])
+        ctx.set_flag(9, FORMS[forms[0]][2])
 
     if args.copy_calib:
         xa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -344,7 +352,8 @@ def main():
         "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
         "config": {"workload": "%s %s THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % (args.config, forms[0]),
                    "kgrid": list(kgrid), "nk": s.nk, "n_orb": no, "n_mu": nmu, "n_tau": ia.nt,
-                   "n_iw_b": ia.nw_b, "ir_table": table, "beta": beta, "parallelism": "single GPU"},
+                   "n_iw_b": ia.nw_b, "ir_table": table, "beta": beta, "parallelism": "single GPU",
+                   "time_reversal_symmetric_system": bool(args.trev)},
         "Ec_Ha": ec,
         "form": forms[0],
         "speedup_vs_plain": 1.0 if forms[0] == "plain" else None,

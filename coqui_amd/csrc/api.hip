@@ -46,6 +46,11 @@ struct thcgw_ctx {
   c128* d_Tbeta = nullptr;  // 1 x nt
   long* d_wn_f = nullptr;   // nw_f fermionic Matsubara indices
   int* d_kmq = nullptr;     // k - q table [k][q] and its transpose [q][k] (k-space convolutions)
+  int trev = 0;             // time-reversal symmetric input: Dyson on the irreducible q only
+  int nq_ir = 0;            // irreducible q (q <= -q) under time reversal
+  int* d_trq = nullptr;     // irreducible q list (nq_ir) then the -q map (nk)
+  int dy_nq = 0;            // V members of the Dyson chunk functions (0 = nk)
+  int vq() const { return dy_nq > 0 ? dy_nq : nk; }
   std::map<std::string, DevBuf> ws;
   std::string err;
   bool force_fallback = false;
@@ -97,6 +102,8 @@ struct thcgw_ctx {
     if (d_wn_f) (void)hipFree(d_wn_f);
     if (d_kmq) (void)hipFree(d_kmq);
     d_kmq = nullptr;
+    if (d_trq) (void)hipFree(d_trq);
+    d_trq = nullptr;
     d_tau = nullptr;
     d_Tplus = d_Tminus = d_Twt_ff = d_Ttw_ff = d_Tbeta = nullptr;
     d_wn_f = nullptr;
@@ -305,7 +312,7 @@ int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
 // (scr_coulomb_t.cpp:180-196, thc_rpa.icc:141-163).
 void dyson_chunk_lu(thcgw_ctx* ctx, c128* PiW, const c128* V, int b0, int cb, c128* d_trace, double* d_ldet,
                     bool solve, bool pi_first, c128* LU, c128* Z, int* ipiv, int* info, hipStream_t s) {
-  const int n = ctx->d.nmu, nq = ctx->nk;
+  const int n = ctx->d.nmu, nq = ctx->vq();
   const long M = ctx->M;
   if (!pi_first)
     thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kMinusOne, thcgw::bp(V, M, nq, 1, b0), n,
@@ -336,7 +343,7 @@ bool any_nonzero(const int* d_info, int count, hipStream_t s) {
 // Lower Cholesky factors L_q of V_q (strict upper triangles zeroed); false when some V_q
 // is not positive definite.
 bool factor_v(thcgw_ctx* ctx, const c128* V, c128* Lv, hipStream_t s) {
-  const int n = ctx->d.nmu, nq = ctx->nk;
+  const int n = ctx->d.nmu, nq = ctx->vq();
   const long M = ctx->M;
   int* info = ctx->buf<int>("dy_vinfo", static_cast<size_t>(nq));
   thcgw::copy_batch(s, V, M, nq, Lv, M, M, 0, nq);
@@ -351,7 +358,7 @@ bool factor_v(thcgw_ctx* ctx, const c128* V, c128* Lv, hipStream_t s) {
 // the chunk's Pi slots untouched, when some M is not positive definite.
 bool dyson_chunk_hpd(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Lv, int b0, int cb, c128* d_trace,
                      double* d_ldet, bool solve, c128* Mb, c128* Z, int* info, hipStream_t s) {
-  const int n = ctx->d.nmu, nq = ctx->nk;
+  const int n = ctx->d.nmu, nq = ctx->vq();
   const long M = ctx->M;
   c128* P = PiW + static_cast<long>(b0) * M;
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, n, n, n, kOne, thcgw::bp(P, M), n, thcgw::bp(Lv, M, nq, 1, b0), n, kZero,
@@ -376,7 +383,7 @@ bool dyson_chunk_hpd(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Lv, i
 // M, its Cholesky factor and ln det in FP64.
 bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv64, int b0, int cb, c128* d_trace,
                          double* d_ldet, bool solve, c128* Mb, c64* Z, c64* Mb64, int* info, hipStream_t s) {
-  const int n = ctx->d.nmu, nq = ctx->nk;
+  const int n = ctx->d.nmu, nq = ctx->vq();
   const long M = ctx->M;
   c128* P = PiW + static_cast<long>(b0) * M;
   // Pi rounded once to complex<float> (the FP32 product rounds it on load anyway), which
@@ -410,50 +417,153 @@ bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv
   return true;
 }
 
-// In place Pi(iW) -> W(iW) - V; optional Tr(V Pi) and ln|Re det(I - V Pi)| per matrix
-// (device arrays of length nw_half*nk).  pi_first selects A = I - Pi V (thc_rpa.icc:141)
-// instead of I - V Pi, and solve = false stops after the determinant.  The Hermitian
-// form is used when selected and V and every M are positive definite; chunks where it
-// does not apply run the LU form.
-void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* d_ldet, bool solve, bool pi_first,
-               hipStream_t s) {
-  const int n = ctx->d.nmu, nq = ctx->nk;
-  const long M = ctx->M;
-  const int total = ctx->nw_half * nq;
-  const int CH = choose_dyson_chunk(ctx, total);
-  const bool mixed = ctx->precision != 0;
-  const bool want_hpd = mixed || ctx->dyson_algo == 1;
-  c128* LU = ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M);
-  int* info_all = ctx->buf<int>("dy_info", static_cast<size_t>(total));
+// Factors of V for the Hermitian form (V members: vq()).  hpd is false when the LU form is
+// selected or some V_q is not positive definite.
+struct VFactors {
   bool hpd = false;
   c128* Lv = nullptr;
   c64* Lv64 = nullptr;
-  if (want_hpd) {
-    Lv = ctx->buf<c128>("dy_lv", static_cast<size_t>(nq) * M);
-    hpd = factor_v(ctx, V, Lv, s);
-    if (hpd && mixed) {
-      Lv64 = ctx->buf<c64>("dy_lv64", static_cast<size_t>(nq) * M);
-      thcgw::to_c64(s, Lv, Lv64, static_cast<long>(nq) * M);
-    }
+};
+
+VFactors factor_dyson_v(thcgw_ctx* ctx, const c128* V, hipStream_t s) {
+  VFactors f;
+  const bool mixed = ctx->precision != 0;
+  if (!(mixed || ctx->dyson_algo == 1)) return f;
+  const int nq = ctx->vq();
+  const long M = ctx->M;
+  f.Lv = ctx->buf<c128>("dy_lv", static_cast<size_t>(nq) * M);
+  f.hpd = factor_v(ctx, V, f.Lv, s);
+  if (f.hpd && mixed) {
+    f.Lv64 = ctx->buf<c64>("dy_lv64", static_cast<size_t>(nq) * M);
+    thcgw::to_c64(s, f.Lv, f.Lv64, static_cast<long>(nq) * M);
   }
+  return f;
+}
+
+// In place Pi -> W - V for `total` matrices at PiW (matrix b uses V member b % vq()), in
+// chunks; Tr, ln det and the factorisation flags at d_trace + b, d_ldet + b, info + b.
+void dyson_matrices(thcgw_ctx* ctx, c128* PiW, const c128* V, const VFactors& f, int total, c128* d_trace,
+                    double* d_ldet, bool solve, bool pi_first, int* info_all, hipStream_t s) {
+  const int n = ctx->d.nmu;
+  const long M = ctx->M;
+  const int CH = choose_dyson_chunk(ctx, total);
+  const bool mixed = ctx->precision != 0;
+  c128* LU = ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M);
   c128* Z = nullptr;
   c64* Z64 = nullptr;
   c64* Mb64 = nullptr;
   int* ipiv = nullptr;
   for (int b0 = 0; b0 < total; b0 += CH) {
     const int cb = std::min(CH, total - b0);
-    if (hpd && mixed) {
+    if (f.hpd && mixed) {
       if (!Z64) Z64 = ctx->buf<c64>("dy_z64", static_cast<size_t>(CH) * M);
       if (!Mb64) Mb64 = ctx->buf<c64>("dy_m64", static_cast<size_t>(CH) * M);
-      if (dyson_chunk_hpd_f32(ctx, PiW, V, Lv64, b0, cb, d_trace, d_ldet, solve, LU, Z64, Mb64, info_all + b0, s))
+      if (dyson_chunk_hpd_f32(ctx, PiW, V, f.Lv64, b0, cb, d_trace, d_ldet, solve, LU, Z64, Mb64, info_all + b0, s))
         continue;
-    } else if (hpd) {
+    } else if (f.hpd) {
       if (!Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
-      if (dyson_chunk_hpd(ctx, PiW, V, Lv, b0, cb, d_trace, d_ldet, solve, LU, Z, info_all + b0, s)) continue;
+      if (dyson_chunk_hpd(ctx, PiW, V, f.Lv, b0, cb, d_trace, d_ldet, solve, LU, Z, info_all + b0, s)) continue;
     }
     if (!ipiv) ipiv = ctx->buf<int>("dy_ipiv", static_cast<size_t>(CH) * n);
     if (solve && !Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
     dyson_chunk_lu(ctx, PiW, V, b0, cb, d_trace, d_ldet, solve, pi_first, LU, Z, ipiv, info_all + b0, s);
+  }
+}
+
+// Time-reversal tables: the irreducible q (q <= -q, C order) and the -q map.
+const int* trev_tables(thcgw_ctx* ctx) {
+  if (ctx->d_trq) return ctx->d_trq;
+  const int* g = ctx->d.kgrid;
+  const int nq = ctx->nk;
+  std::vector<int> mq(nq), ir;
+  for (int i = 0; i < g[0]; ++i)
+    for (int j = 0; j < g[1]; ++j)
+      for (int l = 0; l < g[2]; ++l)
+        mq[(i * g[1] + j) * g[2] + l] = (((g[0] - i) % g[0]) * g[1] + (g[1] - j) % g[1]) * g[2] + (g[2] - l) % g[2];
+  for (int q = 0; q < nq; ++q)
+    if (q <= mq[q]) ir.push_back(q);
+  std::vector<int> h(ir);
+  h.insert(h.end(), mq.begin(), mq.end());
+  THCGW_HIP(hipMalloc(&ctx->d_trq, sizeof(int) * h.size()));
+  THCGW_HIP(hipMemcpy(ctx->d_trq, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+  ctx->nq_ir = static_cast<int>(ir.size());
+  return ctx->d_trq;
+}
+
+// Refuse input that is not time-reversal symmetric: max |A(-q) - conj A(q)| over the nq
+// matrices at A against 1e-9 of max |A|.
+void check_trev(thcgw_ctx* ctx, const c128* A, const char* what, hipStream_t s) {
+  const int* mq = trev_tables(ctx) + ctx->nq_ir;
+  const int nbx = 64, nq = ctx->nk;
+  double* d = ctx->buf<double>("tr_chk", static_cast<size_t>(2) * nq * nbx);
+  thcgw::trev_diff(s, A, ctx->M, mq, nq, nbx, d);
+  std::vector<double> h(static_cast<size_t>(2) * nq * nbx);
+  THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  double diff = 0.0, mag = 0.0;
+  for (size_t i = 0; i < h.size(); i += 2) {
+    diff = std::max(diff, h[i]);
+    mag = std::max(mag, h[i + 1]);
+  }
+  if (!(diff <= 1e-9 * mag))
+    throw thcgw::Error(THCGW_ERR_ARG, std::string("time reversal (flag 9): ") + what + " is not symmetric, max |A(-q) - conj A(q)| = " +
+                                          std::to_string(diff) + " against max |A| = " + std::to_string(mag));
+}
+
+// Time-reversal symmetric input: Pi(-q) = conj Pi(q) and V(-q) = conj V(q), hence
+// W(-q) = conj W(q), Tr(V Pi)(-q) = conj Tr(V Pi)(q) and equal determinants.  Only the
+// irreducible q are solved: the matrices of a block of frequencies are gathered into a
+// staging array, solved with the irreducible V members, and scattered back to q and,
+// conjugated, to -q, together with the Ec terms and flags.
+void run_dyson_trev(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* d_ldet, bool solve,
+                    bool pi_first, int* info_all, hipStream_t s) {
+  const int* tq = trev_tables(ctx);
+  const int nqi = ctx->nq_ir, nq = ctx->nk, nw = ctx->nw_half;
+  const int *d_qir = tq, *d_mq = tq + nqi;
+  const long M = ctx->M;
+  if (!ctx->deferred) {
+    check_trev(ctx, V, "V", s);
+    check_trev(ctx, PiW, "Pi(iW_0)", s);
+  }
+  c128* Vir = ctx->buf<c128>("tr_v", static_cast<size_t>(nqi) * M);
+  thcgw::gather_ir(s, V, M, 0, 0, d_qir, nqi, nqi, Vir);
+  struct Restore {
+    thcgw_ctx* c;
+    ~Restore() { c->dy_nq = 0; }
+  } restore{ctx};
+  ctx->dy_nq = nqi;
+  const VFactors f = factor_dyson_v(ctx, Vir, s);
+  const int FB = std::max(1, choose_dyson_chunk(ctx, nw * nqi) / nqi);  // frequencies per block
+  const int cap = std::min(FB, nw) * nqi;
+  c128* S = ctx->buf<c128>("tr_s", static_cast<size_t>(cap) * M);
+  c128* trs = d_trace ? ctx->buf<c128>("tr_tr", static_cast<size_t>(cap)) : nullptr;
+  double* lds = d_ldet ? ctx->buf<double>("tr_ld", static_cast<size_t>(cap)) : nullptr;
+  int* infos = ctx->buf<int>("tr_info", static_cast<size_t>(cap));
+  for (int n0 = 0; n0 < nw; n0 += FB) {
+    const int cnt = std::min(FB, nw - n0) * nqi;
+    thcgw::gather_ir(s, PiW, M, n0, nq, d_qir, nqi, cnt, S);
+    dyson_matrices(ctx, S, Vir, f, cnt, trs, lds, solve, pi_first, infos, s);
+    if (solve) thcgw::scatter_ir(s, S, M, n0, nq, d_qir, d_mq, nqi, cnt, PiW);
+    thcgw::scatter_ir_terms(s, trs, lds, infos, n0, nq, d_qir, d_mq, nqi, cnt, d_trace, d_ldet, info_all);
+  }
+}
+
+// In place Pi(iW) -> W(iW) - V; optional Tr(V Pi) and ln|Re det(I - V Pi)| per matrix
+// (device arrays of length nw_half*nk).  pi_first selects A = I - Pi V (thc_rpa.icc:141)
+// instead of I - V Pi, and solve = false stops after the determinant.  The Hermitian
+// form is used when selected and V and every M are positive definite; chunks where it
+// does not apply run the LU form.  With time reversal (flag 9) only the irreducible q are
+// solved.
+void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* d_ldet, bool solve, bool pi_first,
+               hipStream_t s) {
+  const int nq = ctx->nk;
+  const int total = ctx->nw_half * nq;
+  int* info_all = ctx->buf<int>("dy_info", static_cast<size_t>(total));
+  if (ctx->trev) {
+    run_dyson_trev(ctx, PiW, V, d_trace, d_ldet, solve, pi_first, info_all, s);
+  } else {
+    const VFactors f = factor_dyson_v(ctx, V, s);
+    dyson_matrices(ctx, PiW, V, f, total, d_trace, d_ldet, solve, pi_first, info_all, s);
   }
   if (ctx->deferred) return;  // check_deferred() reads the flags after the stream completes
   std::vector<int> info(total);
@@ -833,6 +943,10 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     } else if (flag == 7) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 7 (precision) takes 0 (FP64), 1 (mixed) or 2 (mixed, split BF16)");
       ctx->precision = static_cast<int>(value);
+    } else if (flag == 9) {
+      THCGW_REQUIRE(value == 0 || value == 1, "flag 9 (time reversal) takes 0 or 1");
+      ctx->trev = static_cast<int>(value);
+      if (ctx->trev) trev_tables(ctx);
     }
     else THCGW_REQUIRE(false, "unknown flag");
   });

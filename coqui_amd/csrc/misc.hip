@@ -262,6 +262,88 @@ __global__ __launch_bounds__(T256) void trace_prod_kernel(const c128* A, const c
   if (threadIdx.x == 0) out[blockIdx.x] = c128{re * scale, im * scale};
 }
 
+
+// ---- time reversal: staged Dyson on the irreducible q ----
+// Matrix t of a staging block is (n, q) = (n0 + t / nqi, qir[t % nqi]).
+
+// dst[t] = src[n q]   (M elements per matrix; grid.y = matrix)
+__global__ __launch_bounds__(T256) void gather_ir_kernel(const c128* __restrict__ src, long M, int n0, int nq,
+                                                         const int* __restrict__ qir, int nqi, c128* __restrict__ dst) {
+  const int t = blockIdx.y;
+  const long from = static_cast<long>(n0 + t / nqi) * nq + qir[t % nqi];
+  const c128* a = src + from * M;
+  c128* b = dst + static_cast<long>(t) * M;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < M; e += static_cast<long>(gridDim.x) * T256)
+    b[e] = a[e];
+}
+
+// dst[n q] = S[t]; dst[n mq(q)] = conj S[t] when mq(q) != q
+__global__ __launch_bounds__(T256) void scatter_ir_kernel(const c128* __restrict__ S, long M, int n0, int nq,
+                                                          const int* __restrict__ qir, const int* __restrict__ mq,
+                                                          int nqi, c128* __restrict__ dst) {
+  const int t = blockIdx.y;
+  const int q = qir[t % nqi], p = mq[q];
+  const long base = static_cast<long>(n0 + t / nqi) * nq;
+  const c128* a = S + static_cast<long>(t) * M;
+  c128* b = dst + (base + q) * M;
+  c128* c = dst + (base + p) * M;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < M; e += static_cast<long>(gridDim.x) * T256) {
+    const c128 v = a[e];
+    b[e] = v;
+    if (p != q) c[e] = c128{v.x, -v.y};
+  }
+}
+
+__global__ void scatter_ir_terms_kernel(const c128* __restrict__ trs, const double* __restrict__ lds,
+                                        const int* __restrict__ infos, int n0, int nq, const int* __restrict__ qir,
+                                        const int* __restrict__ mq, int nqi, int cnt, c128* __restrict__ tr,
+                                        double* __restrict__ ld, int* __restrict__ info) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int q = qir[t % nqi], p = mq[q];
+  const long base = static_cast<long>(n0 + t / nqi) * nq;
+  info[base + q] = infos[t];
+  info[base + p] = infos[t];
+  if (tr) {
+    tr[base + q] = trs[t];
+    tr[base + p] = c128{trs[t].x, -trs[t].y};
+  }
+  if (ld) {
+    ld[base + q] = lds[t];
+    ld[base + p] = lds[t];
+  }
+}
+
+// out[2 (q nbx + bx)] = max |A[mq(q)] - conj A[q]|, out[... + 1] = max |A[q]| over the block's slice
+__global__ __launch_bounds__(T256) void trev_diff_kernel(const c128* __restrict__ A, long M, const int* __restrict__ mq,
+                                                         double* __restrict__ out) {
+  __shared__ double r0[T256], r1[T256];
+  const int q = blockIdx.y;
+  const c128* a = A + static_cast<long>(q) * M;
+  const c128* b = A + static_cast<long>(mq[q]) * M;
+  double d = 0.0, m = 0.0;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < M; e += static_cast<long>(gridDim.x) * T256) {
+    const c128 u = a[e], v = b[e];
+    d = fmax(d, fmax(fabs(v.x - u.x), fabs(v.y + u.y)));
+    m = fmax(m, fmax(fabs(u.x), fabs(u.y)));
+  }
+  r0[threadIdx.x] = d;
+  r1[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = T256 / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r0[threadIdx.x] = fmax(r0[threadIdx.x], r0[threadIdx.x + w]);
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const long o = 2L * (static_cast<long>(q) * gridDim.x + blockIdx.x);
+    out[o] = r0[0];
+    out[o + 1] = r1[0];
+  }
+}
+
 }  // namespace
 
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb) {
@@ -363,6 +445,34 @@ void to_c64(hipStream_t s, const c128* src, c64* dst, long n) {
   if (n <= 0) return;
   const unsigned g = static_cast<unsigned>(std::min<long>((n + T256 - 1) / T256, 1L << 20));
   hipLaunchKernelGGL(to_c64_kernel, dim3(g), dim3(T256), 0, s, src, dst, n);
+  THCGW_HIP(hipGetLastError());
+}
+
+unsigned slice_blocks(long M) { return static_cast<unsigned>(std::min<long>((M + T256 - 1) / T256, 1024)); }
+
+void gather_ir(hipStream_t s, const c128* src, long M, int n0, int nq, const int* qir, int nqi, int cnt, c128* dst) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(gather_ir_kernel, dim3(slice_blocks(M), cnt), dim3(T256), 0, s, src, M, n0, nq, qir, nqi, dst);
+  THCGW_HIP(hipGetLastError());
+}
+
+void scatter_ir(hipStream_t s, const c128* S, long M, int n0, int nq, const int* qir, const int* mq, int nqi, int cnt,
+                c128* dst) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(scatter_ir_kernel, dim3(slice_blocks(M), cnt), dim3(T256), 0, s, S, M, n0, nq, qir, mq, nqi, dst);
+  THCGW_HIP(hipGetLastError());
+}
+
+void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const int* infos, int n0, int nq,
+                      const int* qir, const int* mq, int nqi, int cnt, c128* tr, double* ld, int* info) {
+  if (cnt <= 0) return;
+  hipLaunchKernelGGL(scatter_ir_terms_kernel, dim3((cnt + T256 - 1) / T256), dim3(T256), 0, s, trs, lds, infos, n0, nq,
+                     qir, mq, nqi, cnt, tr, ld, info);
+  THCGW_HIP(hipGetLastError());
+}
+
+void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out) {
+  hipLaunchKernelGGL(trev_diff_kernel, dim3(nbx, nq), dim3(T256), 0, s, A, M, mq, out);
   THCGW_HIP(hipGetLastError());
 }
 

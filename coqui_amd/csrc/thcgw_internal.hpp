@@ -174,5 +174,15 @@ void batched_trace(hipStream_t s, const c128* A, long sA, int n, c128* tr, int b
 void hf_hartree(hipStream_t s, const c128* DP, const c128* V0, c128* d, c128* FP, int nk, int nmu);
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb);
 void trace_prod(hipStream_t s, const c128* A, const c128* B, int nw, int nk, int no, double scale, c128* out);
+// Time reversal: matrix t of a staging block is (n, q) = (n0 + t / nqi, qir[t % nqi]) of an
+// (n, nq) array of M-element matrices.  gather copies it in; scatter writes it back to q and,
+// conjugated, to mq[q]; scatter_ir_terms does the same for Tr (conjugated), ln det and flags
+// (tr, ld nullable).  trev_diff: per (q, block) max |A[mq q] - conj A[q]| and max |A[q]|.
+void gather_ir(hipStream_t s, const c128* src, long M, int n0, int nq, const int* qir, int nqi, int cnt, c128* dst);
+void scatter_ir(hipStream_t s, const c128* S, long M, int n0, int nq, const int* qir, const int* mq, int nqi, int cnt,
+                c128* dst);
+void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const int* infos, int n0, int nq,
+                      const int* qir, const int* mq, int nqi, int cnt, c128* tr, double* ld, int* info);
+void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out);
 
 }  // namespace thcgw

@@ -14,6 +14,13 @@ parity check see the same bytes.  A system is:
     taken to be included in it.
 
 Seeds are 20261202 + 1000 * system_id.
+
+With trev=True the system is made time-reversal symmetric after it is drawn: for every
+k-point whose partner -k (modulo the grid) is a different point, the partner's C, E, X and
+Z are the complex conjugates of the point's own; at the points with -k = k (time-reversal
+invariant momenta) C is replaced by a real orthogonal matrix (QR of the real part) and X
+and Z by their real parts.  Then H(-k) = conj H(k), X(-k) = conj X(k), V(-q) = conj V(q),
+and so Pi(-q) = conj Pi(q), W(-q) = conj W(q) and Sigma(-k) = conj Sigma(k).
 """
 from dataclasses import dataclass, field
 
@@ -49,8 +56,20 @@ def _cgauss(rng, shape):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
 
 
-def make_system(kgrid, no, nmu, system_id=0, with_v=True):
-    """Build a synthetic system.  with_v=False skips V (the caller forms Z Z^H / N_mu itself)."""
+def minus_k(kgrid):
+    """Index of -k (modulo the grid) for every k of the C-order kgrid."""
+    n1, n2, n3 = kgrid
+    out = np.empty(n1 * n2 * n3, np.int64)
+    for i in range(n1):
+        for j in range(n2):
+            for l in range(n3):
+                out[(i * n2 + j) * n3 + l] = (((-i) % n1) * n2 + (-j) % n2) * n3 + (-l) % n3
+    return out
+
+
+def make_system(kgrid, no, nmu, system_id=0, with_v=True, trev=False):
+    """Build a synthetic system.  with_v=False skips V (the caller forms Z Z^H / N_mu itself);
+    trev=True makes it time-reversal symmetric (see the module docstring)."""
     kgrid = tuple(int(v) for v in kgrid)
     nk = int(np.prod(kgrid))
     seed = BASE_SEED + 1000 * int(system_id)
@@ -66,6 +85,17 @@ def make_system(kgrid, no, nmu, system_id=0, with_v=True):
     X = _cgauss(rng, (nk, nmu, no))
     X /= np.linalg.norm(X, axis=2, keepdims=True)
     Z = _cgauss(rng, (nk, nmu, nmu))
+    if trev:
+        mk = minus_k(kgrid)
+        for k in range(nk):
+            p = int(mk[k])
+            if p == k:
+                q, r = np.linalg.qr(C[k].real)
+                C[k] = q * np.sign(np.diagonal(r))[None, :]
+                X[k] = X[k].real / np.linalg.norm(X[k].real, axis=1, keepdims=True)
+                Z[k] = Z[k].real
+            elif p > k:
+                C[p], E[p], X[p], Z[p] = C[k].conj(), E[k], X[k].conj(), Z[k].conj()
     V = None
     if with_v:
         V = np.matmul(Z, np.conj(np.swapaxes(Z, 1, 2))) / nmu

@@ -470,3 +470,27 @@ def test_adam_oracle_closed_form():
         w, m, v = mo.adam(w, m, v, g, step, 0.01)
     assert np.allclose(m / (1 - 0.9 ** 5), g, rtol=1e-12)
     assert np.allclose(np.sqrt(v / (1 - 0.999 ** 5)), np.abs(g), rtol=1e-12)
+
+
+@pytest.mark.parametrize("kgrid", [(2, 2, 1), (3, 3, 1), (2, 3, 2)])
+def test_time_reversal_relations(orc, kgrid):
+    """A time-reversal symmetric system (H(-k) = conj H(k), X(-k) = conj X(k),
+    V(-q) = conj V(q)) has Pi(-q) = conj Pi(q), W(-q) = conj W(q) and
+    Sigma(-k) = conj Sigma(k): what lets the GPU path solve the irreducible q only."""
+    from coqui_amd.system import minus_k
+
+    s = make_system(kgrid, 6, 20, system_id=5, trev=True)
+    mk = minus_k(kgrid)
+    assert np.array_equal(s.V[mk], s.V.conj()) and np.array_equal(s.X[mk], s.X.conj())
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    G = orc.update_G(tab, s.C, s.E)
+    Piw = orc.tau_to_w_phsym(tab, orc.pi_rspace(G, s.X, kgrid))
+    W = orc.dyson(Piw, s.V)
+    S = orc.gw_iteration(tab, G, s.X, s.V, kgrid, with_ec=False)["Sigma"]
+    rel = lambda a, b: np.max(np.abs(a - b)) / np.max(np.abs(b))
+    assert rel(Piw[:, mk], Piw.conj()) < 1e-12
+    assert rel(W[:, mk], W.conj()) < 1e-12
+    assert rel(S[:, :, mk], S.conj()) < 1e-12
+    # a system drawn without the symmetry does not have it
+    s2 = make_system(kgrid, 6, 20, system_id=5)
+    assert rel(s2.V[mk], s2.V.conj()) > 0.1
