@@ -1408,6 +1408,72 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
   }
 }
 
+int thcgw_eps_inv_head(thcgw_ctx* ctx, const double* W_qtPQ, const double* chi_bar_qP, const double* qpts_cart,
+                       double volume, int div_treatment, double* eps_inv_tq, double* eps_inv_t, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(W_qtPQ && chi_bar_qP && qpts_cart && eps_inv_t, "thcgw_eps_inv_head: null pointer");
+    THCGW_REQUIRE(div_treatment >= 0 && div_treatment <= 3, "thcgw_eps_inv_head: div_treatment in 0..3");
+    hipStream_t s = S(stream);
+    const int nq = ctx->nk, nth = ctx->nt_half, n = ctx->d.nmu;
+    c128* d = ctx->buf<c128>("gy_head", static_cast<size_t>(nth) * nq);
+    thcgw::prod_head(s, C(W_qtPQ), C(chi_bar_qP), n, nth, nq, d);
+    std::vector<std::complex<double>> h(static_cast<size_t>(nth) * nq);
+    THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(c128) * h.size(), hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    // eps^-1(t, q) = (|q|^2 / 4 pi) Vol chi_bar^T W conj(chi_bar)   (g0_div_utils.hpp:197-209)
+    const double fpi = 4.0 * M_PI;
+    for (int q = 0; q < nq; ++q) {
+      const double* v = qpts_cart + 3 * q;
+      const double f = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / fpi * volume;
+      for (int t = 0; t < nth; ++t) h[size_t(t) * nq + q] *= f;
+    }
+    if (eps_inv_tq) std::memcpy(eps_inv_tq, h.data(), sizeof(c128) * h.size());
+    // a single q-point cannot be extrapolated (g0_div_utils.hpp:66-70)
+    const int mode = (nq == 1) ? 0 : div_treatment;
+    const std::vector<std::complex<double>> h0 = thcgw::host::eps_inv_q0(ctx->tab, ctx->d, h, qpts_cart, nq, mode, 0.8, 2);
+    std::memcpy(eps_inv_t, h0.data(), sizeof(c128) * h0.size());
+  });
+}
+
+int thcgw_sigma_div_correction(thcgw_ctx* ctx, double* Sigma, const double* G, const double* X_skPa,
+                               const double* chi_head_P, double madelung, const double* eps_inv_t, int div_treatment,
+                               void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(div_treatment >= 0 && div_treatment <= 3, "thcgw_sigma_div_correction: div_treatment in 0..3");
+    if (div_treatment == 0 || ctx->nk == 1) return;  // ignore_g0 (thc_gw.icc:454-461)
+    THCGW_REQUIRE(Sigma && G && X_skPa && chi_head_P && eps_inv_t, "thcgw_sigma_div_correction: null pointer");
+    hipStream_t s = S(stream);
+    const int nt = ctx->d.nt, nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nth = ctx->nt_half;
+    const long O2 = static_cast<long>(no) * no, XS = static_cast<long>(nmu) * no;
+    // T^k = X^H diag(conj chi_head) X   (thc_gw.icc:476-491)
+    c128* Y = ctx->buf<c128>("gy_y", static_cast<size_t>(nk) * XS);
+    c128* T = ctx->buf<c128>("gy_t", static_cast<size_t>(nk) * O2);
+    thcgw::scale_rows_conj(s, C(X_skPa), C(chi_head_P), nk, nmu, no, Y);
+    thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(C(X_skPa), XS), no, thcgw::bp(Y, XS), no,
+                 kZero, T, O2, no, nk);
+    // Delta(t, k) = -madelung Re eps^-1(t') T G(t, k) T^H, t' = t or nt - 1 - t   (thc_gw.icc:496-519)
+    std::vector<double> f(nt);
+    for (int t = 0; t < nt; ++t) {
+      const int tp = t < nth ? t : nt - t - 1;
+      f[t] = -madelung * eps_inv_t[2 * tp];
+    }
+    double* d_f = ctx->buf<double>("gy_f", static_cast<size_t>(nt));
+    THCGW_HIP(hipMemcpyAsync(d_f, f.data(), sizeof(double) * nt, hipMemcpyHostToDevice, s));
+    const long per_t = static_cast<long>(nk) * O2;
+    c128* Gs = ctx->buf<c128>("gy_g", static_cast<size_t>(nt) * per_t);
+    c128* B1 = ctx->buf<c128>("gy_b", static_cast<size_t>(nt) * per_t);
+    THCGW_HIP(hipMemcpyAsync(Gs, G, sizeof(c128) * nt * per_t, hipMemcpyDeviceToDevice, s));
+    thcgw::scale_by_t(s, Gs, d_f, per_t, nt);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, no, kOne, thcgw::bp(T, O2, nk), no, thcgw::bp(Gs, O2), no, kZero,
+                 B1, O2, no, nt * nk);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, no, no, no, kOne, thcgw::bp(B1, O2), no, thcgw::bp(T, O2, nk), no, kOne,
+                 Cm(Sigma), O2, no, nt * nk);
+    THCGW_HIP(hipStreamSynchronize(s));  // f lives on the host stack
+  });
+}
+
 int thcgw_gm_energy(thcgw_ctx* ctx, const double* G, const double* Sigma, double* e_corr, void* stream) {
   if (!ctx) return THCGW_ERR_ARG;
   return guarded(ctx, [&] {

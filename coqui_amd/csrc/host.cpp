@@ -62,6 +62,101 @@ void fold_ph(const Tables& tab, const thcgw_dims& d, std::vector<cd>& Tp, std::v
     }
 }
 
+std::vector<cd> eps_inv_q0(const Tables& tab, const thcgw_dims& d, const std::vector<cd>& eps, const double* qcart,
+                           int nq, int mode, double q_max, int fit_order) {
+  const int nth = (d.nt % 2 == 0) ? d.nt / 2 : d.nt / 2 + 1;
+  const int nwh = (d.nw_b % 2 == 0) ? d.nw_b / 2 : d.nw_b / 2 + 1;
+  std::vector<double> qa(nq);
+  for (int q = 0; q < nq; ++q)
+    qa[q] = std::sqrt(qcart[3 * q] * qcart[3 * q] + qcart[3 * q + 1] * qcart[3 * q + 1] + qcart[3 * q + 2] * qcart[3 * q + 2]);
+  std::vector<cd> out(nth);
+  auto column = [&](int q) {
+    for (int t = 0; t < nth; ++t) out[t] = eps[size_t(t) * nq + q];
+  };
+  if (mode == 2 || mode == 3) {
+    // filter_qpts: 0 < |q| <= q_max, distinct |q|, (2D: q_z == 0), in q order
+    std::vector<int> sel;
+    std::vector<double> seen;
+    for (int q = 0; q < nq; ++q) {
+      const double r = qa[q];
+      if (r > 0.0 && r <= q_max && (mode != 3 || qcart[3 * q + 2] == 0.0) &&
+          std::find(seen.begin(), seen.end(), r) == seen.end()) {
+        seen.push_back(r);
+        sel.push_back(q);
+      }
+    }
+    if (sel.empty()) throw std::runtime_error("Gygi extrapolation: no q-point with 0 < |q| <= q_max");
+    if (sel.size() < 3) {
+      column(sel[0]);
+      return out;
+    }
+    // tau -> iW (PH folded), fit per frequency, iW -> tau
+    std::vector<cd> Tp, Tm;
+    fold_ph(tab, d, Tp, Tm);
+    const int ns = static_cast<int>(sel.size()), nc = fit_order + 1;
+    std::vector<double> A(size_t(ns) * nc);
+    for (int i = 0; i < ns; ++i)
+      for (int j = 0; j < nc; ++j) A[size_t(i) * nc + j] = std::pow(qa[sel[i]], j);
+    // (A^T A)^-1 by Gauss-Jordan with partial pivoting (A is real)
+    std::vector<double> N(size_t(nc) * nc, 0.0), Ni(size_t(nc) * nc, 0.0);
+    for (int a = 0; a < nc; ++a)
+      for (int b = 0; b < nc; ++b)
+        for (int i = 0; i < ns; ++i) N[size_t(a) * nc + b] += A[size_t(i) * nc + a] * A[size_t(i) * nc + b];
+    for (int a = 0; a < nc; ++a) Ni[size_t(a) * nc + a] = 1.0;
+    for (int c = 0; c < nc; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < nc; ++r)
+        if (std::fabs(N[size_t(r) * nc + c]) > std::fabs(N[size_t(piv) * nc + c])) piv = r;
+      if (N[size_t(piv) * nc + c] == 0.0) throw std::runtime_error("Gygi extrapolation: singular normal equations");
+      for (int j = 0; j < nc; ++j) {
+        std::swap(N[size_t(c) * nc + j], N[size_t(piv) * nc + j]);
+        std::swap(Ni[size_t(c) * nc + j], Ni[size_t(piv) * nc + j]);
+      }
+      const double inv = 1.0 / N[size_t(c) * nc + c];
+      for (int j = 0; j < nc; ++j) {
+        N[size_t(c) * nc + j] *= inv;
+        Ni[size_t(c) * nc + j] *= inv;
+      }
+      for (int r = 0; r < nc; ++r)
+        if (r != c) {
+          const double f = N[size_t(r) * nc + c];
+          for (int j = 0; j < nc; ++j) {
+            N[size_t(r) * nc + j] -= f * N[size_t(c) * nc + j];
+            Ni[size_t(r) * nc + j] -= f * Ni[size_t(c) * nc + j];
+          }
+        }
+    }
+    std::vector<cd> ew0(nwh);
+    for (int n = 0; n < nwh; ++n) {
+      std::vector<cd> atb(nc, 0.0);
+      for (int i = 0; i < ns; ++i) {
+        cd e = 0.0;  // eps(iW_n, q_i)
+        for (int t = 0; t < nth; ++t) e += Tp[size_t(n) * nth + t] * eps[size_t(t) * nq + sel[i]];
+        for (int a = 0; a < nc; ++a) atb[a] += A[size_t(i) * nc + a] * e;
+      }
+      cd x0 = 0.0;
+      for (int a = 0; a < nc; ++a) x0 += Ni[a] * atb[a];
+      ew0[n] = x0;
+    }
+    for (int t = 0; t < nth; ++t) {
+      cd v = 0.0;
+      for (int n = 0; n < nwh; ++n) v += Tm[size_t(t) * nwh + n] * ew0[n];
+      out[t] = v;
+    }
+    return out;
+  }
+  // find_smallest_qabs (a single q-point is its own answer)
+  int idx = 0;
+  if (nq > 1) {
+    idx = -1;
+    for (int q = 0; q < nq; ++q)
+      if (qa[q] > 0.0 && (idx < 0 || qa[q] < qa[idx])) idx = q;
+    if (idx < 0) throw std::runtime_error("Gygi correction: no nonzero q-point");
+  }
+  column(idx);
+  return out;
+}
+
 double ec_from_ew(const Tables& tab, const std::vector<cd>& ew, int nq) {
   cd e = 0.0;
   for (int it = 0; it < tab.nt; ++it) {

@@ -344,6 +344,62 @@ __global__ __launch_bounds__(T256) void trev_diff_kernel(const c128* __restrict_
   }
 }
 
+
+// ---- long-wavelength (Gygi) correction ----
+// out[t * nq + q] = sum_P c[q][P] sum_Q W[q][t][P][Q] conj(c[q][Q])   (g0_div_utils.hpp:199-209)
+// One workgroup per (q, t) matrix; each wave takes rows P = wave, wave + 4, ..., its lanes
+// read consecutive Q.
+__global__ __launch_bounds__(T256) void prod_head_kernel(const c128* __restrict__ W, const c128* __restrict__ cbar,
+                                                         int n, int nt_half, int nq, c128* __restrict__ out) {
+  __shared__ double red[2][T256];
+  const int t = blockIdx.x, q = blockIdx.y;
+  const c128* w = W + (static_cast<long>(q) * nt_half + t) * static_cast<long>(n) * n;
+  const c128* c = cbar + static_cast<long>(q) * n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double ar = 0.0, ai = 0.0;
+  for (int P = wave; P < n; P += 4) {
+    const c128* row = w + static_cast<long>(P) * n;
+    double rr = 0.0, ri = 0.0;  // sum_Q W[P][Q] conj(c[Q]), this lane's share
+    for (int Q = lane; Q < n; Q += 64) {
+      const c128 a = row[Q], b = c[Q];
+      rr += a.x * b.x + a.y * b.y;
+      ri += a.y * b.x - a.x * b.y;
+    }
+    const c128 cp = c[P];  // each lane adds c[P] times its share
+    ar += cp.x * rr - cp.y * ri;
+    ai += cp.x * ri + cp.y * rr;
+  }
+  red[0][threadIdx.x] = ar;
+  red[1][threadIdx.x] = ai;
+  __syncthreads();
+  for (int h = T256 / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[static_cast<long>(t) * nq + q] = c128{red[0][0], red[1][0]};
+}
+
+// Y[k][P][a] = conj(h[P]) X[k][P][a]
+__global__ __launch_bounds__(T256) void scale_rows_conj_kernel(const c128* __restrict__ X, const c128* __restrict__ h,
+                                                               int nmu, int no, long total, c128* __restrict__ Y) {
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= total) return;
+  const int P = static_cast<int>((e / no) % nmu);
+  const c128 x = X[e], g = h[P];
+  Y[e] = c128{g.x * x.x + g.y * x.y, g.x * x.y - g.y * x.x};
+}
+
+// A[t][k][i][j] *= f[t]
+__global__ __launch_bounds__(T256) void scale_by_t_kernel(c128* A, const double* __restrict__ f, long per_t, long total) {
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= total) return;
+  const double g = f[e / per_t];
+  A[e] = c128{A[e].x * g, A[e].y * g};
+}
+
 }  // namespace
 
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb) {
@@ -468,6 +524,25 @@ void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const i
   if (cnt <= 0) return;
   hipLaunchKernelGGL(scatter_ir_terms_kernel, dim3((cnt + T256 - 1) / T256), dim3(T256), 0, s, trs, lds, infos, n0, nq,
                      qir, mq, nqi, cnt, tr, ld, info);
+  THCGW_HIP(hipGetLastError());
+}
+
+void prod_head(hipStream_t s, const c128* W, const c128* cbar, int n, int nt_half, int nq, c128* out) {
+  hipLaunchKernelGGL(prod_head_kernel, dim3(nt_half, nq), dim3(T256), 0, s, W, cbar, n, nt_half, nq, out);
+  THCGW_HIP(hipGetLastError());
+}
+
+void scale_rows_conj(hipStream_t s, const c128* X, const c128* h, int nk, int nmu, int no, c128* Y) {
+  const long total = static_cast<long>(nk) * nmu * no;
+  hipLaunchKernelGGL(scale_rows_conj_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, X,
+                     h, nmu, no, total, Y);
+  THCGW_HIP(hipGetLastError());
+}
+
+void scale_by_t(hipStream_t s, c128* A, const double* f, long per_t, int nt) {
+  const long total = per_t * nt;
+  hipLaunchKernelGGL(scale_by_t_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, A, f,
+                     per_t, total);
   THCGW_HIP(hipGetLastError());
 }
 

@@ -40,6 +40,15 @@ void solve_qp_diag(const Tables& tab, int nk, int no, const cd* sigma_diag_t, co
 // GM energy from SigmaG(iw_n) = sum_k w_k Tr[Sigma(iw_n) G(iw_n)] (nw_f values), scf_common.cpp:119-128.
 double gm_from_sigmag(const Tables& tab, const std::vector<cd>& sg);
 
+// Head of the inverse dielectric function at q -> 0 from eps (nt_half x nq, tau-major)
+// (g0_div_utils.hpp:55-111 eps_inv_head_t, 264-322 find_smallest_qabs / extrapolate_to_q0):
+// mode 0 (ignore_g0) and 1 (gygi): the column of the smallest nonzero |q|; 2 (gygi_extrplt)
+// and 3 (gygi_extrplt_2d, q_z = 0 only): per bosonic frequency a least-squares polynomial of
+// order fit_order in |q| through the q with 0 < |q| <= q_max (distinct |q|), evaluated at
+// 0 and brought back to tau; with fewer than 3 such q, the first of them.
+std::vector<cd> eps_inv_q0(const Tables& tab, const thcgw_dims& d, const std::vector<cd>& eps, const double* qcart,
+                           int nq, int mode, double q_max, int fit_order);
+
 // Pade frequency selection of pade_driver::init (pade_driver.hpp:38-60) for a symmetric mesh.
 std::vector<int> pade_fit_indices(int niw, int nfit);
 

@@ -184,5 +184,10 @@ void scatter_ir(hipStream_t s, const c128* S, long M, int n0, int nq, const int*
 void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const int* infos, int n0, int nq,
                       const int* qir, const int* mq, int nqi, int cnt, c128* tr, double* ld, int* info);
 void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out);
+// Gygi correction: out[t nq + q] = sum_PQ cbar[q][P] W[q][t][P][Q] conj(cbar[q][Q]);
+// Y[k][P][a] = conj(h[P]) X[k][P][a]; A[t][...] *= f[t] (per_t elements per t).
+void prod_head(hipStream_t s, const c128* W, const c128* cbar, int n, int nt_half, int nq, c128* out);
+void scale_rows_conj(hipStream_t s, const c128* X, const c128* h, int nk, int nmu, int no, c128* Y);
+void scale_by_t(hipStream_t s, c128* A, const double* f, long per_t, int nt);
 
 }  // namespace thcgw

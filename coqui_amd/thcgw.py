@@ -27,7 +27,7 @@ EXPORTS = (
     "thcgw_lri_train", "thcgw_lri_num_pairs", "thcgw_lri_clear_pairs", "thcgw_lri_reset_state",
     "thcgw_mlp_create", "thcgw_mlp_destroy", "thcgw_mlp_last_error", "thcgw_mlp_num_params",
     "thcgw_mlp_get_params", "thcgw_mlp_set_params", "thcgw_mlp_forward", "thcgw_mlp_backward",
-    "thcgw_mlp_adam_step", "thcgw_mlp_train_mse",
+    "thcgw_mlp_adam_step", "thcgw_mlp_train_mse", "thcgw_eps_inv_head", "thcgw_sigma_div_correction",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -104,6 +104,8 @@ def load_library(path=LIB_PATH):
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
         "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
+        "thcgw_eps_inv_head": (i, [p, p, p, p, d, i, p, p, p]),
+        "thcgw_sigma_div_correction": (i, [p, p, p, p, p, d, p, i, p]),
         "thcgw_lri_create": (i, [p, i, i, ctypes.c_ulonglong, ctypes.POINTER(p)]),
         "thcgw_lri_destroy": (None, [p]),
         "thcgw_lri_last_error": (ctypes.c_char_p, [p]),
@@ -369,6 +371,30 @@ class ThcGW:
                                              ctypes.c_double(mu_tol), ctypes.byref(mu), ctypes.byref(n),
                                              self._stream(stream)))
         return mu.value, n.value
+
+    DIV = {"ignore_g0": 0, "gygi": 1, "gygi_extrplt": 2, "gygi_extrplt_2d": 3}
+
+    def eps_inv_head(self, W_qt, chi_bar, qpts_cart, volume, div_treatment="gygi_extrplt", stream=None):
+        """Head of the inverse dielectric function: returns (eps_inv_tq (nt_half, nk), eps_inv_t (nt_half))."""
+        W_qt, chi_bar = self._dev(W_qt), self._dev(chi_bar)
+        q = np.ascontiguousarray(qpts_cart, np.float64)
+        tq = np.zeros((self.nt_half, self.nk), np.complex128)
+        t0 = np.zeros(self.nt_half, np.complex128)
+        mode = self.DIV[div_treatment] if isinstance(div_treatment, str) else int(div_treatment)
+        self._check(self.lib.thcgw_eps_inv_head(self.h, _ptr(W_qt), _ptr(chi_bar), _np_ptr(q), ctypes.c_double(volume),
+                                                mode, _np_ptr(tq), _np_ptr(t0), self._stream(stream)))
+        return tq, t0
+
+    def sigma_div_correction(self, Sigma, G, X, chi_head, madelung, eps_inv_t, div_treatment="gygi_extrplt",
+                             stream=None):
+        """In place: Sigma += the long-wavelength correction (thc_gw.icc:446-526)."""
+        Sigma, G, X, chi_head = self._dev(Sigma), self._dev(G), self._dev(X), self._dev(chi_head)
+        e = np.ascontiguousarray(eps_inv_t, np.complex128)
+        mode = self.DIV[div_treatment] if isinstance(div_treatment, str) else int(div_treatment)
+        self._check(self.lib.thcgw_sigma_div_correction(self.h, _ptr(Sigma), _ptr(G), _ptr(X), _ptr(chi_head),
+                                                        ctypes.c_double(madelung), _np_ptr(e), mode,
+                                                        self._stream(stream)))
+        return Sigma
 
     def gm_energy(self, G, Sigma, stream=None):
         G, Sigma = self._dev(G), self._dev(Sigma)

@@ -227,6 +227,27 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G_tskij, const
                              const double* const* V_qPQ, double* const* Sigma_tskij, double* const* W_qtPQ,
                              double* Ec, int hermitize, void* stream);
 
+/* a8 + a11: treatment of the long-wavelength divergence (Gygi).  div_treatment: 0 = ignore_g0,
+ * 1 = gygi (head of the smallest nonzero |q|), 2 = gygi_extrplt (per bosonic frequency a
+ * least-squares quadratic in |q| through the q with 0 < |q| <= 0.8, distinct |q|,
+ * evaluated at q = 0; the first such q when there are fewer than 3), 3 = gygi_extrplt_2d
+ * (the same over q_z = 0).
+ * thcgw_eps_inv_head: eps^-1_head(tau, q) = (|q|^2/4pi) Vol chi_bar_q^T W(tau, q) conj(chi_bar_q)
+ * from W_qtPQ (thcgw_w_to_tau's output), chi_bar_qP = thc_reader_t::basis_bar_head()
+ * (device, (nk, nmu) complex), q-points in Cartesian units (host, (nk, 3)) and the cell
+ * volume; eps_inv_tq (host (nt_half, nk) complex, may be NULL) receives every q and
+ * eps_inv_t (host nt_half complex) the q -> 0 head.  Replaces div_utils::eps_inv_head_t /
+ * eval_eps_inv_q (src/methods/GW/g0_div_utils.hpp:55-111,180-214).  Synchronises.
+ * thcgw_sigma_div_correction: Sigma(tau, k) += -madelung Re eps^-1_head(tau') T^k G(tau, k) T^k^H
+ * with T^k = X^k^H diag(conj chi_head) X^k, tau' = tau folded to [0, beta/2], chi_head =
+ * basis_head()(0, :) (device nmu complex); a no-op for ignore_g0 or one k-point.  Replaces
+ * gw_t::Sigma_div_correction (src/methods/GW/thc_gw.icc:446-526).  Synchronises. */
+int thcgw_eps_inv_head(thcgw_ctx* ctx, const double* W_qtPQ, const double* chi_bar_qP, const double* qpts_cart,
+                       double volume, int div_treatment, double* eps_inv_tq, double* eps_inv_t, void* stream);
+int thcgw_sigma_div_correction(thcgw_ctx* ctx, double* Sigma_tskij, const double* G_tskij, const double* X_skPa,
+                               const double* chi_head_P, double madelung, const double* eps_inv_t, int div_treatment,
+                               void* stream);
+
 /* Learned form of the Dyson solve (no counterpart in the reference; it replaces the
  * getrf/getri of scr_coulomb_t.cpp:180-196 inside a self-consistent loop).  With
  * V = L L^H and A = -L^H Pi(iW_n) L, W - V = L M^-1 L^H - V, M = I + A.  The Krylov solve

@@ -40,6 +40,12 @@ Parity pins (what checks each oracle function; see tests/test_oracle_pins.py):
                                                         reference's compute_eigenspectra);
                                                         Sigma = 0: 2 sum_k w_k sum_b f(e_b - mu)
                                                         and the mu that solves it
+  eps_inv_head    (g0_div_utils.hpp:55-111,180-322)    real for Hermitian W (a misplaced conjugate is
+                                                        not); W = a I closed form; the q -> 0 fit
+                                                        reproduces a quadratic in |q| exactly; the
+                                                        closest-q and few-q rules
+  sigma_div_correction (thc_gw.icc:446-526)            X = [I; 0] closed form conj(h_i) G_ij h_j;
+                                                        tau and beta - tau share the head
   thc_hf          (thc_hf.icc:18-300)                  N_k = 1 closed form X^H[diag(J) - D^P o V]X;
                                                         brute-force k - q convolution of the
                                                         exchange; Hermiticity; negative exchange
@@ -407,3 +413,50 @@ def gw_iteration(tab, G, X, V, kgrid, with_ec=True):
     Wqt = w_to_qt(w_to_tau_phsym(tab, Ww))
     S = hermitize(sigma_rspace(G, X, Wqt, kgrid))
     return {"Pi_t": Pit, "Pi_w": Piw, "W_w": Ww, "W_qt": Wqt, "Sigma": S, "Ec": ec}
+
+
+def eps_inv_head(tab, Wqt, chi_bar, qcart, volume, mode, q_max=0.8, fit_order=2):
+    """(eps_tq (nt_half, nq), eps_t (nt_half)) of div_utils::eps_inv_head_t
+    (g0_div_utils.hpp:55-111, eval_eps_inv_q :180-214, helpers :264-322).
+    mode 0 ignore_g0 / 1 gygi: smallest nonzero |q|; 2 gygi_extrplt, 3 gygi_extrplt_2d."""
+    nq = Wqt.shape[0]
+    qa = np.sqrt(np.sum(np.asarray(qcart) ** 2, axis=1))
+    # sum_P cbar_P sum_Q W_PQ conj(cbar_Q)  (nda::blas::dot does not conjugate)
+    head = np.einsum("qp,qtpr,qr->tq", chi_bar, Wqt, chi_bar.conj())
+    eps = head * (qa ** 2 / (4 * np.pi) * volume)[None, :]
+    if nq == 1:
+        mode = 0
+    if mode in (2, 3):
+        sel, seen = [], []
+        for q in range(nq):
+            r = qa[q]
+            if 0.0 < r <= q_max and (mode != 3 or qcart[q][2] == 0.0) and r not in seen:
+                seen.append(r)
+                sel.append(q)
+        if len(sel) < 3:
+            return eps, eps[:, sel[0]].copy()
+        ew = tau_to_w_phsym(tab, np.ascontiguousarray(eps[:, sel]))
+        A = np.array([[qa[q] ** j for j in range(fit_order + 1)] for q in sel])
+        x = np.linalg.solve(A.T @ A, A.T @ ew.T)  # (fit_order+1, nw_half)
+        e0 = w_to_tau_phsym(tab, np.ascontiguousarray(x[0][:, None]))[:, 0]
+        return eps, e0
+    if nq == 1:
+        return eps, eps[:, 0].copy()
+    cand = [q for q in range(nq) if qa[q] > 0.0]
+    idx = min(cand, key=lambda q: (qa[q], q))
+    return eps, eps[:, idx].copy()
+
+
+def sigma_div_correction(Sigma, G, X, chi_head, madelung, eps_t, mode):
+    """Sigma + the Gygi correction of gw_t::Sigma_div_correction (thc_gw.icc:446-526)."""
+    nt, _, nk = Sigma.shape[:3]
+    if mode == 0 or nk == 1:
+        return Sigma.copy()
+    nth = (nt + 1) // 2
+    T = np.einsum("kpi,kpa,p->kia", X.conj(), X, chi_head.conj())
+    out = Sigma.copy()
+    for t in range(nt):
+        tp = t if t < nth else nt - t - 1
+        f = -madelung * eps_t[tp].real
+        out[t, 0] += f * np.einsum("kia,kab,kjb->kij", T, G[t, 0], T.conj())
+    return out

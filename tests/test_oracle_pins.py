@@ -494,3 +494,89 @@ def test_time_reversal_relations(orc, kgrid):
     # a system drawn without the symmetry does not have it
     s2 = make_system(kgrid, 6, 20, system_id=5)
     assert rel(s2.V[mk], s2.V.conj()) > 0.1
+
+
+# ---- long-wavelength (Gygi) correction ----
+
+def _qcart(kgrid, L=10.0):
+    """Cartesian q of a Gamma-centred grid, nearest images, cubic cell of side L (bohr)."""
+    out = []
+    for i in range(kgrid[0]):
+        for j in range(kgrid[1]):
+            for l in range(kgrid[2]):
+                f = np.array([i / kgrid[0], j / kgrid[1], l / kgrid[2]])
+                f = f - np.round(f)
+                out.append(2 * np.pi * f / L)
+    return np.array(out)
+
+
+def test_eps_inv_head_hermitian_and_identity(orc):
+    rng = np.random.default_rng(4)
+    nq, nth, n = 4, 3, 7
+    A = rng.standard_normal((nq, nth, n, n)) + 1j * rng.standard_normal((nq, nth, n, n))
+    W = A + np.conj(np.swapaxes(A, 2, 3))
+    cb = rng.standard_normal((nq, n)) + 1j * rng.standard_normal((nq, n))
+    q = rng.uniform(0.1, 0.5, (nq, 3))
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    eps, _ = orc.eps_inv_head(tab, W, cb, q, 2.0, 1)
+    assert np.max(np.abs(eps.imag)) < 1e-13 * np.max(np.abs(eps))
+    # the conjugate on the other side would not be real: the pin discriminates
+    wrong = np.einsum("qp,qtpr,qr->tq", cb.conj(), W, cb)
+    assert np.max(np.abs(np.einsum("qp,qtpr,qr->tq", cb, W, cb).imag)) > 1e-3
+    assert np.allclose(wrong.imag, 0, atol=1e-12 * np.max(np.abs(wrong)))  # also real, but differs:
+    assert np.max(np.abs(wrong - eps / ((np.sum(q**2, 1) / (4 * np.pi) * 2.0)[None]))) > 1e-3
+    # W = a I: eps = |q|^2/(4 pi) Vol a ||cbar||^2
+    W2 = np.broadcast_to(3.0 * np.eye(n), (nq, nth, n, n)).astype(complex)
+    eps2, _ = orc.eps_inv_head(tab, W2, cb, q, 2.0, 1)
+    want = np.sum(q**2, 1) / (4 * np.pi) * 2.0 * 3.0 * np.sum(np.abs(cb) ** 2, 1)
+    assert np.allclose(eps2, np.broadcast_to(want, (nth, nq)), rtol=1e-13)
+
+
+def test_eps_inv_head_q0_rules(orc):
+    """The fit reproduces a quadratic in |q| exactly (value at 0); the closest-q rule; the
+    first-q rule below 3 points."""
+    kgrid = (4, 4, 4)
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    nth = (tab["nt"] + 1) // 2
+    q = _qcart(kgrid)
+    qa = np.sqrt(np.sum(q**2, 1))
+    nq, n = len(q), 3
+    rng = np.random.default_rng(8)
+    # a PH-symmetric tau profile: a single bosonic pole, sampled on the half grid
+    x = tab["tau"][:nth] if "tau" in tab else np.linspace(0, 500, nth)
+    a = np.cosh(0.002 * (x - 500.0))
+    b, c = 0.3 * a, -0.7 * a
+    target = a[:, None] + b[:, None] * qa[None] + c[:, None] * qa[None] ** 2
+    fac = qa**2 / (4 * np.pi)
+    cb = np.zeros((nq, n), complex)
+    cb[:, 0] = 1.0
+    W = np.zeros((nq, nth, n, n), complex)
+    nz = qa > 0
+    W[nz, :, 0, 0] = (target[:, nz] / fac[nz][None]).T
+    eps, e0 = orc.eps_inv_head(tab, W, cb, q, 1.0, 2)
+    assert np.allclose(eps[:, nz], target[:, nz], rtol=1e-12)
+    assert np.max(np.abs(e0 - a)) < 1e-9 * np.max(np.abs(a))
+    _, e1 = orc.eps_inv_head(tab, W, cb, q, 1.0, 1)
+    assert np.allclose(e1, eps[:, int(np.argmin(np.where(nz, qa, np.inf)))])
+    _, e2 = orc.eps_inv_head(tab, W, cb, q, 1.0, 2, q_max=0.2)  # one distinct |q| (0.157)
+    first = [i for i in range(nq) if 0 < qa[i] <= 0.2][0]
+    assert np.allclose(e2, eps[:, first])
+
+
+def test_sigma_div_correction_closed_form(orc):
+    """X = [I; 0]: T = diag(conj h) and Delta_ij = f conj(h_i) G_ij h_j, f = -madelung Re eps(tau')."""
+    rng = np.random.default_rng(6)
+    nt, nk, no, nmu = 6, 3, 4, 7
+    X = np.zeros((nk, nmu, no), complex)
+    X[:, :no, :] = np.eye(no)
+    h = rng.standard_normal(nmu) + 1j * rng.standard_normal(nmu)
+    G = rng.standard_normal((nt, 1, nk, no, no)) + 1j * rng.standard_normal((nt, 1, nk, no, no))
+    S = rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)
+    eps_t = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+    out = orc.sigma_div_correction(S, G, X, h, 1.7, eps_t, 1)
+    for t in range(nt):
+        tp = t if t < 3 else nt - t - 1
+        f = -1.7 * eps_t[tp].real
+        want = S[t, 0] + f * np.conj(h[:no])[None, :, None] * G[t, 0] * h[:no][None, None, :]
+        assert np.allclose(out[t, 0], want, rtol=1e-13, atol=1e-14)
+    assert np.array_equal(orc.sigma_div_correction(S, G, X, h, 1.7, eps_t, 0), S)
