@@ -74,13 +74,15 @@ std::vector<cd> eps_inv_q0(const Tables& tab, const thcgw_dims& d, const std::ve
     for (int t = 0; t < nth; ++t) out[t] = eps[size_t(t) * nq + q];
   };
   if (mode == 2 || mode == 3) {
-    // filter_qpts: 0 < |q| <= q_max, distinct |q|, (2D: q_z == 0), in q order
+    // filter_qpts: 0 < |q| <= q_max, distinct |q|, (2D: q_z == 0), in q order.  |q| values
+    // within 1e-10 (relative) count as equal: images of equal length computed along
+    // different axes differ in the last bit, and keeping both would make the fit singular.
     std::vector<int> sel;
     std::vector<double> seen;
     for (int q = 0; q < nq; ++q) {
       const double r = qa[q];
-      if (r > 0.0 && r <= q_max && (mode != 3 || qcart[3 * q + 2] == 0.0) &&
-          std::find(seen.begin(), seen.end(), r) == seen.end()) {
+      const bool dup = std::any_of(seen.begin(), seen.end(), [r](double v) { return std::fabs(v - r) <= 1e-10 * std::max(v, r); });
+      if (r > 0.0 && r <= q_max && (mode != 3 || qcart[3 * q + 2] == 0.0) && !dup) {
         seen.push_back(r);
         sel.push_back(q);
       }

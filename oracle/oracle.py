@@ -430,7 +430,8 @@ def eps_inv_head(tab, Wqt, chi_bar, qcart, volume, mode, q_max=0.8, fit_order=2)
         sel, seen = [], []
         for q in range(nq):
             r = qa[q]
-            if 0.0 < r <= q_max and (mode != 3 or qcart[q][2] == 0.0) and r not in seen:
+            dup = any(abs(v - r) <= 1e-10 * max(v, r) for v in seen)  # equal lengths up to rounding
+            if 0.0 < r <= q_max and (mode != 3 or qcart[q][2] == 0.0) and not dup:
                 seen.append(r)
                 sel.append(q)
         if len(sel) < 3:

@@ -122,6 +122,32 @@ int main(int argc, char** argv) {
     save(dir + "/W_adapter.bin", W);
     const double es = reldiff(S, S2), ew = reldiff(W, W2), ee = std::abs(ec - ec2) / std::abs(ec2);
     std::printf("{\"sigma_rel\": %.3e, \"w_rel\": %.3e, \"ec_rel\": %.3e, \"ec\": %.15e}\n", es, ew, ee, ec);
+
+    // the same sequence with the long-wavelength treatment (gygi_extrplt), when its inputs are given
+    std::FILE* probe = std::fopen((dir + "/chi_bar.bin").c_str(), "rb");
+    if (probe) {
+      std::fclose(probe);
+      std::vector<cplx> cb = load<cplx>(dir + "/chi_bar.bin"), ch = load<cplx>(dir + "/chi_head.bin");
+      std::vector<double> q = load<double>(dir + "/qpts.bin"), par = load<double>(dir + "/gygi.bin");  // volume, madelung
+      cplx* dcb = dev_from(cb);
+      cplx* dch = dev_from(ch);
+      thcgw_cxx::THCView thc2 = thc;
+      thc2.chi_bar_qP = dcb;
+      thc2.chi_head_P = dch;
+      thc2.qpts_cart = q.data();
+      thc2.volume = par[0];
+      thc2.madelung = par[1];
+      thcgw_cxx::MBState mb2{dG, dS, dW};
+      thcgw_cxx::scr_coulomb_t scr2(ctx, nullptr, thcgw_cxx::gygi_extrplt);
+      thcgw_cxx::gw_t gw2(ctx, nullptr, thcgw_cxx::gygi_extrplt);
+      scr2.update_w(mb2, thc2, 1);
+      gw2.evaluate(mb2, thc2);
+      gw2.hermitize(mb2);
+      ok(hipDeviceSynchronize(), "sync");
+      save(dir + "/Sigma_gygi.bin", host_from(dS, nG));
+      save(dir + "/eps_head.bin", mb2.eps_inv_head);
+      for (cplx* p : {dcb, dch}) (void)hipFree(p);
+    }
     for (cplx* p : {dG, dX, dV, dS, dW, dS2, dW2}) (void)hipFree(p);
     return (es <= 1e-12 && ew <= 1e-12 && ee <= 1e-12) ? 0 : 1;
   } catch (const std::exception& e) {

@@ -29,6 +29,17 @@ def test_cpp_adapter_matches_oracle(orc, tmp_path):
     for name, arr in [("Ttw_ff", ia.Ttw_ff), ("Twt_ff", ia.Twt_ff), ("Ttw_bb", ia.Ttw_bb), ("Twt_bb", ia.Twt_bb),
                       ("T_beta", ia.T_beta_t_ff), ("G", G), ("X", s.X), ("V", s.V)]:
         np.ascontiguousarray(arr, np.complex128).tofile("%s/%s.bin" % (d, name))
+    # long-wavelength treatment inputs (gygi_extrplt): random head vectors, a 10-bohr cubic cell
+    rng = np.random.default_rng(21)
+    cb = rng.standard_normal((s.nk, nmu)) + 1j * rng.standard_normal((s.nk, nmu))
+    ch = rng.standard_normal(nmu) + 1j * rng.standard_normal(nmu)
+    f = np.array([[i / kgrid[0], j / kgrid[1], l / kgrid[2]] for i in range(kgrid[0]) for j in range(kgrid[1])
+                  for l in range(kgrid[2])])
+    q = 2 * np.pi * (f - np.round(f)) / 10.0
+    cb.tofile(d + "/chi_bar.bin")
+    ch.tofile(d + "/chi_head.bin")
+    q.astype(np.float64).tofile(d + "/qpts.bin")
+    np.array([1000.0, 2.3]).tofile(d + "/gygi.bin")
     r = subprocess.run([BIN, d], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
@@ -37,3 +48,10 @@ def test_cpp_adapter_matches_oracle(orc, tmp_path):
     W = np.fromfile(d + "/W_adapter.bin", np.complex128).reshape(ref["W_qt"].shape)
     assert np.max(np.abs(S - ref["Sigma"])) / np.max(np.abs(ref["Sigma"])) < 1e-10
     assert np.max(np.abs(W - ref["W_qt"])) / np.max(np.abs(ref["W_qt"])) < 1e-10
+    # update_w -> eps_inv_head, evaluate -> Sigma + correction, hermitize (G is Hermitian, so is the correction)
+    _, e0 = orc.eps_inv_head(tab, ref["W_qt"], cb, q, 1000.0, 2)
+    eh = np.fromfile(d + "/eps_head.bin", np.complex128)
+    assert np.max(np.abs(eh - e0)) / np.max(np.abs(e0)) < 1e-10
+    want = orc.sigma_div_correction(ref["Sigma"], G, s.X, ch, 2.3, e0, 2)
+    Sg = np.fromfile(d + "/Sigma_gygi.bin", np.complex128).reshape(want.shape)
+    assert np.max(np.abs(Sg - want)) / np.max(np.abs(want)) < 1e-10

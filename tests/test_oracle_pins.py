@@ -561,6 +561,13 @@ def test_eps_inv_head_q0_rules(orc):
     _, e2 = orc.eps_inv_head(tab, W, cb, q, 1.0, 2, q_max=0.2)  # one distinct |q| (0.157)
     first = [i for i in range(nq) if 0 < qa[i] <= 0.2][0]
     assert np.allclose(e2, eps[:, first])
+    # 3x3x1: |q| of (1/3, 0) and (2/3 - 1, 0) differ in the last bit; they are one length,
+    # so two distinct lengths remain and the first q is used, not a singular fit
+    q3 = _qcart((3, 3, 1))
+    W3 = np.zeros((9, nth, n, n), complex)
+    W3[:, :, 0, 0] = 1.0
+    eps3, e3 = orc.eps_inv_head(tab, W3, cb[:9], q3, 1.0, 2)
+    assert np.allclose(e3, eps3[:, 1])
 
 
 def test_sigma_div_correction_closed_form(orc):
