@@ -30,14 +30,15 @@ def torch():
     return _t
 
 
-def build(orc, torch, precision):
+def build(orc, torch, precision, trev=False):
     ia = IAFT.from_table(1000.0, TABLE)
     tab = orc.load_ir(1000.0, None, table=TABLE)
     out = []
     for sid, (kgrid, no, nmu) in enumerate(SYSTEMS):
-        s = make_system(kgrid, no, nmu, system_id=40 + sid)
+        s = make_system(kgrid, no, nmu, system_id=40 + sid, trev=trev)
         ctx = ThcGW(kgrid, no, nmu, ia)
         ctx.set_flag(7, precision)
+        ctx.set_flag(9, int(trev))
         G = torch.from_numpy(orc.update_G(tab, s.C, s.E)).cuda()
         X = torch.from_numpy(s.X).cuda()
         V = torch.from_numpy(s.V).cuda()
@@ -52,10 +53,12 @@ def eager(sys_):
     return S.cpu().numpy(), W.cpu().numpy(), ec
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_batch_graph_replays_match_eager(orc, torch, precision):
-    print("case: batch precision=%d systems=%s" % (precision, SYSTEMS), flush=True)
-    systems = build(orc, torch, precision)
+@pytest.mark.parametrize("precision,trev", [(0, False), (1, False), (0, True)])
+def test_batch_graph_replays_match_eager(orc, torch, precision, trev):
+    """With trev, every system is time-reversal symmetric and solves the irreducible q only
+    (flag 9), inside the captured graph too."""
+    print("case: batch precision=%d trev=%d systems=%s" % (precision, trev, SYSTEMS), flush=True)
+    systems = build(orc, torch, precision, trev)
     b = ThcBatch([x["ctx"] for x in systems], use_graph=True, tune=(precision != 0))
     args = [[x[k] for x in systems] for k in ("G", "X", "V", "Sig", "W")]
     try:
