@@ -933,7 +933,10 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     else if (flag == 2) ctx->pair_chunk = static_cast<int>(value);
     else if (flag == 3) ctx->dyson_chunk = static_cast<int>(value);
     else if (flag == 4) thcgw::gemm_profile_enable(value != 0);
-    else if (flag == 5) thcgw::gemm_set_tile_mode(static_cast<int>(value));
+    else if (flag == 5) {
+      THCGW_REQUIRE(value >= 0 && value <= 3, "flag 5 (GEMM tile) takes 0 (automatic) to 3");
+      thcgw::gemm_set_tile_mode(static_cast<int>(value));
+    }
     else if (flag == 8) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 8 (FP32 tile) takes 0, 1 or 2");
       thcgw::gemm_set_tile_mode32(static_cast<int>(value));

@@ -232,3 +232,28 @@ def test_kspace_variants(orc, torch, kgrid, no, nmu):
     Wqt = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=False)["W_qt"]
     SK = to_np(ctx.sigma_kspace(G, Wqt, s.X, hermitize=False))
     assert relmax(SK, orc.sigma_kspace(G, s.X, Wqt, s.kgrid)) < TOL
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3])
+def test_fp64_tile_shapes(orc, tile):
+    """Every complex-FP64 GEMM tile shape (flag 5) gives the oracle's iteration (1e-10)."""
+    import torch
+
+    kgrid, no, nmu, table = (2, 2, 2), 10, 150, "1e2.1e-15"
+    print("case: tile shape %d" % tile, flush=True)
+    s = make_system(kgrid, no, nmu, system_id=14)
+    tab = orc.load_ir(1000.0, None, table=table)
+    ctx = ThcGW(kgrid, no, nmu, IAFT.from_table(1000.0, table))
+    G = ctx.update_G(s.C, s.E, 0.0)
+    ref = orc.gw_iteration(tab, orc.update_G(tab, s.C, s.E), s.X, s.V, kgrid, with_ec=True)
+    try:
+        ctx.set_flag(5, tile)
+        for f6 in (0, 1):
+            ctx.set_flag(6, f6)
+            S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+            torch.cuda.synchronize()
+            assert np.max(np.abs(S.cpu().numpy() - ref["Sigma"])) < 1e-10 * np.max(np.abs(ref["Sigma"]))
+            assert np.max(np.abs(W.cpu().numpy() - ref["W_qt"])) < 1e-10 * np.max(np.abs(ref["W_qt"]))
+            assert abs(ec - ref["Ec"]) < 1e-10 * abs(ref["Ec"])
+    finally:
+        ctx.set_flag(5, 0)
