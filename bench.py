@@ -12,7 +12,9 @@ Forms (--forms, the first is the headline value):
              transforms, Cholesky factorisations and ln det;
   mixed_bf16 as mixed, with the FP32 products done as split-BF16 MFMA products;
   *_trev     the form with the Dyson step on the time-reversal irreducible q only (needs
-             --trev, which draws the system time-reversal symmetric).
+             --trev, which draws the system time-reversal symmetric);
+  *_c4       the form with the Dyson step on the q irreducible under a fourfold rotation
+             (needs --c4, which draws the system symmetric under it).
 Every further form is reported with its speedup over the first and its errors against it
 (W at q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
 
@@ -67,9 +69,11 @@ def batch_systems(n=24):
     return out
 
 
-# form: (flag 6 Dyson algorithm, flag 7 precision, flag 9 time reversal)
+# form: (flag 6 Dyson algorithm, flag 7 precision, symmetry: 0 none, 1 time reversal (flag 9),
+# 2 the C4 q-point map (thcgw_set_qsym))
 FORMS = {"plain": (0, 0, 0), "hermitian": (1, 0, 0), "mixed": (1, 1, 0), "mixed_bf16": (1, 2, 0),
-         "plain_trev": (0, 0, 1), "hermitian_trev": (1, 0, 1), "mixed_trev": (1, 1, 1)}
+         "plain_trev": (0, 0, 1), "hermitian_trev": (1, 0, 1), "mixed_trev": (1, 1, 1),
+         "plain_c4": (0, 0, 2), "hermitian_c4": (1, 0, 2), "mixed_c4": (1, 1, 2)}
 HA_PER_MEV = 1.0 / 27211.386245988
 
 
@@ -174,6 +178,7 @@ def main():
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
     ap.add_argument("--scgw-iters", type=int, default=10, help="cfg5: self-consistent iterations")
     ap.add_argument("--trev", action="store_true", help="draw the system time-reversal symmetric (for *_trev forms)")
+    ap.add_argument("--c4", action="store_true", help="draw the system symmetric under a fourfold rotation (*_c4 forms)")
     ap.add_argument("--krylov-max-iter", type=int, default=60, help="cfg5: CG iterations before the direct fallback")
     ap.add_argument("--forms", default="plain,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
@@ -182,8 +187,10 @@ def main():
     for f in forms:
         if f not in FORMS:
             ap.error("unknown form %r (choose from %s)" % (f, ", ".join(FORMS)))
-        if FORMS[f][2] and not args.trev:
+        if FORMS[f][2] == 1 and not args.trev:
             ap.error("form %r needs --trev" % f)
+        if FORMS[f][2] == 2 and not args.c4:
+            ap.error("form %r needs --c4" % f)
     if args.cpu_child:
         cpu_baseline_child(args.cpu_child)
         return
@@ -205,14 +212,21 @@ def main():
     log("bench: config %s kgrid=%s no=%d nmu=%d table=%s" % (args.config, kgrid, no, nmu, table))
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
-    s = make_system(kgrid, no, nmu, system_id=0, with_v=False, trev=args.trev)
+    if args.c4:
+        s = make_system(kgrid, no, nmu, system_id=0, c4=True)
+        s.Z = None
+    else:
+        s = make_system(kgrid, no, nmu, system_id=0, with_v=False, trev=args.trev)
     ia = IAFT.from_table(beta, table)
     ctx = ThcGW(kgrid, no, nmu, ia)
     ctx.set_flag(5, args.gemm_tile)
     ctx.set_flag(8, args.gemm_tile32)
-    Zd = torch.from_numpy(s.Z).cuda()
-    Vd = torch.matmul(Zd, Zd.conj().transpose(1, 2)) / nmu  # input preparation, V = Z Z^H / N_mu
-    del Zd
+    if s.V is not None:
+        Vd = torch.from_numpy(s.V).cuda()
+    else:
+        Zd = torch.from_numpy(s.Z).cuda()
+        Vd = torch.matmul(Zd, Zd.conj().transpose(1, 2)) / nmu  # input preparation, V = Z Z^H / N_mu
+        del Zd
     Xd = torch.from_numpy(s.X).cuda()
     Gd = ctx.update_G(s.C, s.E, s.mu)
     s.V_host2 = Vd[:2].cpu().numpy()
@@ -225,10 +239,17 @@ def main():
 
     stream = torch.cuda.current_stream()
 
-    def run_form(form, steps, warmup):
+    def set_form(form):
         ctx.set_flag(6, FORMS[form][0])
         ctx.set_flag(7, FORMS[form][1])
-        ctx.set_flag(9, FORMS[form][2])
+        ctx.set_flag(9, int(FORMS[form][2] == 1))
+        if FORMS[form][2] == 2:
+            ctx.set_qsym(*s.qsym)
+        else:
+            ctx.set_qsym()
+
+    def run_form(form, steps, warmup):
+        set_form(form)
         ec = None
         for i in range(warmup):
             _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
@@ -288,9 +309,7 @@ def main():
             log("bench: %s %.1f ms per step, %s" % (form, f_ms, json.dumps(
                 {k: v for k, v in other_forms[form].items() if k != "gemm_classes"})))
         del Sig_ref, W0_ref, H, Cd
-        ctx.set_flag(6, FORMS[forms[0]][0])
-        ctx.set_flag(7, FORMS[forms[0]][1])
-        ctx.set_flag(9, FORMS[forms[0]][2])
+        set_form(forms[0])
 
     if args.copy_calib:
         xa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -353,7 +372,7 @@ def main():
         "config": {"workload": "%s %s THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % (args.config, forms[0]),
                    "kgrid": list(kgrid), "nk": s.nk, "n_orb": no, "n_mu": nmu, "n_tau": ia.nt,
                    "n_iw_b": ia.nw_b, "ir_table": table, "beta": beta, "parallelism": "single GPU",
-                   "time_reversal_symmetric_system": bool(args.trev)},
+                   "time_reversal_symmetric_system": bool(args.trev), "c4_symmetric_system": bool(args.c4)},
         "Ec_Ha": ec,
         "form": forms[0],
         "speedup_vs_plain": 1.0 if forms[0] == "plain" else None,

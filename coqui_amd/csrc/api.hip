@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -50,6 +51,11 @@ struct thcgw_ctx {
   int nq_ir = 0;            // irreducible q (q <= -q) under time reversal
   int* d_trq = nullptr;     // irreducible q list (nq_ir) then the -q map (nk)
   int dy_nq = 0;            // V members of the Dyson chunk functions (0 = nk)
+  // q-point symmetry map (thcgw_set_qsym): W(q) = D_q P_q W^(src q) P_q^T D_q^*, W^ = W or conj W
+  int sym = 0;
+  int sym_nir = 0;          // irreducible q (src q == q)
+  int* d_sym = nullptr;     // src (nk), conj (nk), irreducible list (sym_nir), identity (nk), perm (nk * nmu)
+  c128* d_symph = nullptr;  // phases (nk * nmu)
   int vq() const { return dy_nq > 0 ? dy_nq : nk; }
   std::map<std::string, DevBuf> ws;
   std::string err;
@@ -104,6 +110,10 @@ struct thcgw_ctx {
     d_kmq = nullptr;
     if (d_trq) (void)hipFree(d_trq);
     d_trq = nullptr;
+    if (d_sym) (void)hipFree(d_sym);
+    d_sym = nullptr;
+    if (d_symph) (void)hipFree(d_symph);
+    d_symph = nullptr;
     d_tau = nullptr;
     d_Tplus = d_Tminus = d_Twt_ff = d_Ttw_ff = d_Tbeta = nullptr;
     d_wn_f = nullptr;
@@ -490,8 +500,18 @@ const int* trev_tables(thcgw_ctx* ctx) {
   return ctx->d_trq;
 }
 
+std::string sci(double v) {
+  char b[32];
+  std::snprintf(b, sizeof b, "%.3e", v);
+  return b;
+}
+
+// Tolerance of the symmetry checks of the input: 1e-9 of max |A| in FP64; the mixed forms
+// compute Pi from FP32 products, whose rounding breaks the relations at ~1e-7.
+double sym_tol(const thcgw_ctx* ctx) { return ctx->precision ? 1e-5 : 1e-9; }
+
 // Refuse input that is not time-reversal symmetric: max |A(-q) - conj A(q)| over the nq
-// matrices at A against 1e-9 of max |A|.
+// matrices at A against sym_tol of max |A|.
 void check_trev(thcgw_ctx* ctx, const c128* A, const char* what, hipStream_t s) {
   const int* mq = trev_tables(ctx) + ctx->nq_ir;
   const int nbx = 64, nq = ctx->nk;
@@ -505,9 +525,10 @@ void check_trev(thcgw_ctx* ctx, const c128* A, const char* what, hipStream_t s) 
     diff = std::max(diff, h[i]);
     mag = std::max(mag, h[i + 1]);
   }
-  if (!(diff <= 1e-9 * mag))
-    throw thcgw::Error(THCGW_ERR_ARG, std::string("time reversal (flag 9): ") + what + " is not symmetric, max |A(-q) - conj A(q)| = " +
-                                          std::to_string(diff) + " against max |A| = " + std::to_string(mag));
+  if (!(diff <= sym_tol(ctx) * mag))
+    throw thcgw::Error(THCGW_ERR_ARG, std::string("time reversal (flag 9): ") + what +
+                                          " is not symmetric, max |A(-q) - conj A(q)| = " + sci(diff) +
+                                          " against max |A| = " + sci(mag));
 }
 
 // Time-reversal symmetric input: Pi(-q) = conj Pi(q) and V(-q) = conj V(q), hence
@@ -548,6 +569,68 @@ void run_dyson_trev(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, dou
   }
 }
 
+// q-point symmetry (thcgw_set_qsym): the Dyson step runs on the irreducible q only, staged
+// like the time-reversal path; every other q of an orbit is then filled from its source,
+// W(q) = D P W^(src) P^T D^* (a unitary similarity, so Tr(V Pi) and the determinant carry
+// over, conjugated where W^ = conj W).
+void check_qsym(thcgw_ctx* ctx, const c128* A, const char* what, hipStream_t s) {
+  const int nq = ctx->nk, n = ctx->d.nmu, nbx = 64;
+  const int* src = ctx->d_sym;
+  const int* cj = src + nq;
+  const int* perm = cj + nq + ctx->sym_nir + nq;
+  double* d = ctx->buf<double>("sy_chk", static_cast<size_t>(2) * nq * nbx);
+  thcgw::qsym_diff(s, A, n, nq, src, cj, perm, ctx->d_symph, nbx, d);
+  std::vector<double> h(static_cast<size_t>(2) * nq * nbx);
+  THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  double diff = 0.0, mag = 0.0;
+  for (size_t i = 0; i < h.size(); i += 2) {
+    diff = std::max(diff, h[i]);
+    mag = std::max(mag, h[i + 1]);
+  }
+  if (!(diff <= sym_tol(ctx) * mag))
+    throw thcgw::Error(THCGW_ERR_ARG, std::string("q-point symmetry: ") + what + " does not follow the map, max deviation " +
+                                          sci(diff) + " against max |A| = " + sci(mag));
+}
+
+void run_dyson_sym(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* d_ldet, bool solve,
+                   bool pi_first, int* info_all, hipStream_t s) {
+  const int nqi = ctx->sym_nir, nq = ctx->nk, nw = ctx->nw_half, n = ctx->d.nmu;
+  const int* src = ctx->d_sym;
+  const int* cj = src + nq;
+  const int* d_qir = cj + nq;
+  const int* ident = d_qir + nqi;
+  const int* perm = ident + nq;
+  const long M = ctx->M;
+  if (!ctx->deferred) {
+    check_qsym(ctx, V, "V", s);
+    check_qsym(ctx, PiW, "Pi(iW_0)", s);
+  }
+  c128* Vir = ctx->buf<c128>("tr_v", static_cast<size_t>(nqi) * M);
+  thcgw::gather_ir(s, V, M, 0, 0, d_qir, nqi, nqi, Vir);
+  struct Restore {
+    thcgw_ctx* c;
+    ~Restore() { c->dy_nq = 0; }
+  } restore{ctx};
+  ctx->dy_nq = nqi;
+  const VFactors f = factor_dyson_v(ctx, Vir, s);
+  const int FB = std::max(1, choose_dyson_chunk(ctx, nw * nqi) / nqi);
+  const int cap = std::min(FB, nw) * nqi;
+  c128* S = ctx->buf<c128>("tr_s", static_cast<size_t>(cap) * M);
+  c128* trs = d_trace ? ctx->buf<c128>("tr_tr", static_cast<size_t>(cap)) : nullptr;
+  double* lds = d_ldet ? ctx->buf<double>("tr_ld", static_cast<size_t>(cap)) : nullptr;
+  int* infos = ctx->buf<int>("tr_info", static_cast<size_t>(cap));
+  for (int n0 = 0; n0 < nw; n0 += FB) {
+    const int cnt = std::min(FB, nw - n0) * nqi;
+    thcgw::gather_ir(s, PiW, M, n0, nq, d_qir, nqi, cnt, S);
+    dyson_matrices(ctx, S, Vir, f, cnt, trs, lds, solve, pi_first, infos, s);
+    if (solve) thcgw::scatter_ir(s, S, M, n0, nq, d_qir, ident, nqi, cnt, PiW);
+    thcgw::scatter_ir_terms(s, trs, lds, infos, n0, nq, d_qir, ident, nqi, cnt, d_trace, d_ldet, info_all);
+  }
+  if (solve) thcgw::qsym_fill(s, PiW, n, nq, nw, src, cj, perm, ctx->d_symph);
+  thcgw::qsym_fill_terms(s, nq, nw, src, cj, d_trace, d_ldet, info_all);
+}
+
 // In place Pi(iW) -> W(iW) - V; optional Tr(V Pi) and ln|Re det(I - V Pi)| per matrix
 // (device arrays of length nw_half*nk).  pi_first selects A = I - Pi V (thc_rpa.icc:141)
 // instead of I - V Pi, and solve = false stops after the determinant.  The Hermitian
@@ -559,7 +642,9 @@ void run_dyson(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, double* 
   const int nq = ctx->nk;
   const int total = ctx->nw_half * nq;
   int* info_all = ctx->buf<int>("dy_info", static_cast<size_t>(total));
-  if (ctx->trev) {
+  if (ctx->sym) {
+    run_dyson_sym(ctx, PiW, V, d_trace, d_ldet, solve, pi_first, info_all, s);
+  } else if (ctx->trev) {
     run_dyson_trev(ctx, PiW, V, d_trace, d_ldet, solve, pi_first, info_all, s);
   } else {
     const VFactors f = factor_dyson_v(ctx, V, s);
@@ -952,6 +1037,61 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
       if (ctx->trev) trev_tables(ctx);
     }
     else THCGW_REQUIRE(false, "unknown flag");
+  });
+}
+
+int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double* phase, const int* conj_) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    const int nq = ctx->nk, n = ctx->d.nmu;
+    if (ctx->d_sym) THCGW_HIP(hipFree(ctx->d_sym));
+    if (ctx->d_symph) THCGW_HIP(hipFree(ctx->d_symph));
+    ctx->d_sym = nullptr;
+    ctx->d_symph = nullptr;
+    ctx->sym = 0;
+    ctx->sym_nir = 0;
+    if (!src) return;  // clears the map
+    THCGW_REQUIRE(perm, "thcgw_set_qsym: perm is required with src");
+    std::vector<int> ir, cj(nq, 0), ident(nq);
+    for (int q = 0; q < nq; ++q) {
+      THCGW_REQUIRE(src[q] >= 0 && src[q] < nq && src[src[q]] == src[q],
+                    "thcgw_set_qsym: src[q] must be an irreducible q (src[src[q]] == src[q])");
+      if (conj_) cj[q] = conj_[q] != 0;
+      ident[q] = q;
+      std::vector<char> seen(n, 0);
+      for (int P = 0; P < n; ++P) {
+        const int t = perm[static_cast<size_t>(q) * n + P];
+        THCGW_REQUIRE(t >= 0 && t < n && !seen[t], "thcgw_set_qsym: perm[q] must be a permutation of 0..nmu-1");
+        seen[t] = 1;
+        if (src[q] == q) THCGW_REQUIRE(t == P, "thcgw_set_qsym: an irreducible q must map to itself");
+      }
+      if (src[q] == q) {
+        THCGW_REQUIRE(cj[q] == 0, "thcgw_set_qsym: an irreducible q cannot be conjugated");
+        if (phase)
+          for (int P = 0; P < n; ++P)
+            THCGW_REQUIRE(phase[2 * (static_cast<size_t>(q) * n + P)] == 1.0 && phase[2 * (static_cast<size_t>(q) * n + P) + 1] == 0.0,
+                          "thcgw_set_qsym: an irreducible q has unit phases");
+        ir.push_back(q);
+      } else if (phase) {
+        for (int P = 0; P < n; ++P) {
+          const double re = phase[2 * (static_cast<size_t>(q) * n + P)], im = phase[2 * (static_cast<size_t>(q) * n + P) + 1];
+          THCGW_REQUIRE(std::fabs(re * re + im * im - 1.0) < 1e-12, "thcgw_set_qsym: phases must have modulus 1");
+        }
+      }
+    }
+    std::vector<int> h(src, src + nq);
+    h.insert(h.end(), cj.begin(), cj.end());
+    h.insert(h.end(), ir.begin(), ir.end());
+    h.insert(h.end(), ident.begin(), ident.end());
+    h.insert(h.end(), perm, perm + static_cast<size_t>(nq) * n);
+    std::vector<std::complex<double>> ph(static_cast<size_t>(nq) * n, 1.0);
+    if (phase) std::memcpy(ph.data(), phase, sizeof(c128) * ph.size());
+    THCGW_HIP(hipMalloc(&ctx->d_sym, sizeof(int) * h.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_sym, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    THCGW_HIP(hipMalloc(&ctx->d_symph, sizeof(c128) * ph.size()));
+    THCGW_HIP(hipMemcpy(ctx->d_symph, ph.data(), sizeof(c128) * ph.size(), hipMemcpyHostToDevice));
+    ctx->sym_nir = static_cast<int>(ir.size());
+    ctx->sym = 1;
   });
 }
 

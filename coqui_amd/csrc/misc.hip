@@ -400,6 +400,78 @@ __global__ __launch_bounds__(T256) void scale_by_t_kernel(c128* A, const double*
   A[e] = c128{A[e].x * g, A[e].y * g};
 }
 
+
+// ---- q-point symmetry map: A[q][i][j] = ph[q][i] conj(ph[q][j]) A^[src q][perm[q][i]][perm[q][j]] ----
+__device__ __forceinline__ c128 qsym_value(const c128* __restrict__ A, long M, int n, int q, int sq, bool cj,
+                                           const int* __restrict__ perm, const c128* __restrict__ ph, long e) {
+  const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
+  const int* pq = perm + static_cast<long>(q) * n;
+  const c128 a = A[static_cast<long>(sq) * M + static_cast<long>(pq[i]) * n + pq[j]];
+  const c128 v{a.x, cj ? -a.y : a.y};
+  const c128 pi = ph[static_cast<long>(q) * n + i], pj = ph[static_cast<long>(q) * n + j];
+  const c128 w{pi.x * pj.x + pi.y * pj.y, pi.y * pj.x - pi.x * pj.y};  // ph_i conj(ph_j)
+  return c128{w.x * v.x - w.y * v.y, w.x * v.y + w.y * v.x};
+}
+
+// grid.y = nw * nq matrices (n, q); the irreducible q (src q == q) are left as they are
+__global__ __launch_bounds__(T256) void qsym_fill_kernel(c128* W, int n, int nq, const int* __restrict__ src,
+                                                         const int* __restrict__ cj, const int* __restrict__ perm,
+                                                         const c128* __restrict__ ph) {
+  const int nqw = blockIdx.y, q = nqw % nq, w = nqw / nq;
+  const int sq = src[q];
+  if (sq == q) return;
+  const long M = static_cast<long>(n) * n;
+  c128* base = W + static_cast<long>(w) * nq * M;  // the nq matrices of frequency w
+  c128* out = base + static_cast<long>(q) * M;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < M; e += static_cast<long>(gridDim.x) * T256)
+    out[e] = qsym_value(base, M, n, q, sq, cj[q] != 0, perm, ph, e);
+}
+
+__global__ void qsym_fill_terms_kernel(int nq, int total, const int* __restrict__ src, const int* __restrict__ cj,
+                                       c128* tr, double* ld, int* info) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= total) return;
+  const int q = b % nq, sq = src[q];
+  if (sq == q) return;
+  const int from = b - q + sq;
+  if (tr) tr[b] = c128{tr[from].x, cj[q] ? -tr[from].y : tr[from].y};
+  if (ld) ld[b] = ld[from];
+  info[b] = info[from];
+}
+
+// out[2 (q nbx + bx)] = max |A[q] - map(A[src q])|, out[... + 1] = max |A[q]|
+__global__ __launch_bounds__(T256) void qsym_diff_kernel(const c128* __restrict__ A, int n, const int* __restrict__ src,
+                                                         const int* __restrict__ cj, const int* __restrict__ perm,
+                                                         const c128* __restrict__ ph, double* __restrict__ out) {
+  __shared__ double r0[T256], r1[T256];
+  const int q = blockIdx.y, sq = src[q];
+  const long M = static_cast<long>(n) * n;
+  double d = 0.0, m = 0.0;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < M; e += static_cast<long>(gridDim.x) * T256) {
+    const c128 a = A[static_cast<long>(q) * M + e];
+    m = fmax(m, fmax(fabs(a.x), fabs(a.y)));
+    if (sq != q) {
+      const c128 v = qsym_value(A, M, n, q, sq, cj[q] != 0, perm, ph, e);
+      d = fmax(d, fmax(fabs(a.x - v.x), fabs(a.y - v.y)));
+    }
+  }
+  r0[threadIdx.x] = d;
+  r1[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = T256 / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r0[threadIdx.x] = fmax(r0[threadIdx.x], r0[threadIdx.x + w]);
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const long o = 2L * (static_cast<long>(q) * gridDim.x + blockIdx.x);
+    out[o] = r0[0];
+    out[o + 1] = r1[0];
+  }
+}
+
 }  // namespace
 
 void mo_diag(hipStream_t s, const c128* T1, const c128* C, c128* out, int nk, int no, long nb) {
@@ -543,6 +615,26 @@ void scale_by_t(hipStream_t s, c128* A, const double* f, long per_t, int nt) {
   const long total = per_t * nt;
   hipLaunchKernelGGL(scale_by_t_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, A, f,
                      per_t, total);
+  THCGW_HIP(hipGetLastError());
+}
+
+void qsym_fill(hipStream_t s, c128* W, int n, int nq, int nw, const int* src, const int* cj, const int* perm,
+               const c128* ph) {
+  const long M = static_cast<long>(n) * n;
+  hipLaunchKernelGGL(qsym_fill_kernel, dim3(slice_blocks(M), nw * nq), dim3(T256), 0, s, W, n, nq, src, cj, perm, ph);
+  THCGW_HIP(hipGetLastError());
+}
+
+void qsym_fill_terms(hipStream_t s, int nq, int nw, const int* src, const int* cj, c128* tr, double* ld, int* info) {
+  const int total = nq * nw;
+  hipLaunchKernelGGL(qsym_fill_terms_kernel, dim3((total + T256 - 1) / T256), dim3(T256), 0, s, nq, total, src, cj, tr,
+                     ld, info);
+  THCGW_HIP(hipGetLastError());
+}
+
+void qsym_diff(hipStream_t s, const c128* A, int n, int nq, const int* src, const int* cj, const int* perm,
+               const c128* ph, int nbx, double* out) {
+  hipLaunchKernelGGL(qsym_diff_kernel, dim3(nbx, nq), dim3(T256), 0, s, A, n, src, cj, perm, ph, out);
   THCGW_HIP(hipGetLastError());
 }
 

@@ -184,6 +184,14 @@ void scatter_ir(hipStream_t s, const c128* S, long M, int n0, int nq, const int*
 void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const int* infos, int n0, int nq,
                       const int* qir, const int* mq, int nqi, int cnt, c128* tr, double* ld, int* info);
 void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out);
+// q-point symmetry map: W[w][q] = ph_q conj(ph_q)^T o W^[w][src q](perm_q, perm_q) for the q with
+// src q != q (W^ = conj W where cj[q]); the Ec terms and flags likewise; and the check
+// max |A[q] - map(A[src q])| per (q, block) with max |A[q]|.
+void qsym_fill(hipStream_t s, c128* W, int n, int nq, int nw, const int* src, const int* cj, const int* perm,
+               const c128* ph);
+void qsym_fill_terms(hipStream_t s, int nq, int nw, const int* src, const int* cj, c128* tr, double* ld, int* info);
+void qsym_diff(hipStream_t s, const c128* A, int n, int nq, const int* src, const int* cj, const int* perm,
+               const c128* ph, int nbx, double* out);
 // Gygi correction: out[t nq + q] = sum_PQ cbar[q][P] W[q][t][P][Q] conj(cbar[q][Q]);
 // Y[k][P][a] = conj(h[P]) X[k][P][a]; A[t][...] *= f[t] (per_t elements per t).
 void prod_head(hipStream_t s, const c128* W, const c128* cbar, int n, int nt_half, int nq, c128* out);

@@ -21,6 +21,13 @@ Z are the complex conjugates of the point's own; at the points with -k = k (time
 invariant momenta) C is replaced by a real orthogonal matrix (QR of the real part) and X
 and Z by their real parts.  Then H(-k) = conj H(k), X(-k) = conj X(k), V(-q) = conj V(q),
 and so Pi(-q) = conj Pi(q), W(-q) = conj W(q) and Sigma(-k) = conj Sigma(k).
+
+With c4=True (kgrid[0] == kgrid[1]) the system has a fourfold rotation S(i, j, l) =
+(-j, i, l) acting on the THC points as a fixed permutation pi of order 4: on every orbit
+of S the first point is drawn freely (X and Z averaged over the orbit's stabiliser) and
+the point S^a k gets H(k), X(S^a k) = X(k)[pi_a] and Z(S^a k) = Z(k)[pi_a][:, pi_a], so
+V, Pi and W obey A(S^a q) = A(q)[pi_a][:, pi_a].  System.qsym is then (src, perm): the
+orbit representative of every q and the index map pi_a (nk x nmu).
 """
 from dataclasses import dataclass, field
 
@@ -41,6 +48,7 @@ class System:
     V: np.ndarray = field(repr=False)    # (nk, nmu, nmu), or None when built on the device
     Z: np.ndarray = field(repr=False)    # (nk, nmu, nmu) factor of V
     mu: float = 0.0
+    qsym: tuple = field(default=None, repr=False)  # (src (nk), perm (nk, nmu)) with c4=True
 
     @property
     def nk(self):
@@ -67,9 +75,48 @@ def minus_k(kgrid):
     return out
 
 
-def make_system(kgrid, no, nmu, system_id=0, with_v=True, trev=False):
+def c4_orbits(kgrid):
+    """For every k of the grid: (representative r, power a) with k = S^a r, S(i, j, l) = (-j, i, l)."""
+    n1, n2, n3 = kgrid
+    assert n1 == n2, "the fourfold rotation needs kgrid[0] == kgrid[1]"
+    idx = lambda i, j, l: (i * n2 + j) * n3 + l
+    nk = n1 * n2 * n3
+    rep = [-1] * nk
+    power = [0] * nk
+    for k in range(nk):
+        if rep[k] >= 0:
+            continue
+        i, j, l = k // (n2 * n3), (k // n3) % n2, k % n3
+        for a in range(4):
+            t = idx(i, j, l)
+            if rep[t] < 0:
+                rep[t], power[t] = k, a
+            i, j = (-j) % n1, i
+    return np.array(rep), np.array(power)
+
+
+def _perm_c4(nmu, rng):
+    """A permutation of order 4 (4-cycles, 2-cycles and fixed points) as index maps pi^a, a = 0..3."""
+    order = rng.permutation(nmu)
+    base = np.arange(nmu)
+    p = base.copy()
+    pos = 0
+    sizes = [4, 2, 1]
+    while pos < nmu:
+        c = sizes[rng.integers(0, 3)] if nmu - pos >= 4 else 1
+        cyc = order[pos:pos + c]
+        p[cyc] = np.roll(cyc, 1)
+        pos += c
+    maps = [base]
+    for _ in range(3):
+        maps.append(p[maps[-1]])
+    return maps
+
+
+def make_system(kgrid, no, nmu, system_id=0, with_v=True, trev=False, c4=False):
     """Build a synthetic system.  with_v=False skips V (the caller forms Z Z^H / N_mu itself);
-    trev=True makes it time-reversal symmetric (see the module docstring)."""
+    trev=True makes it time-reversal symmetric and c4=True symmetric under the fourfold
+    rotation (see the module docstring)."""
     kgrid = tuple(int(v) for v in kgrid)
     nk = int(np.prod(kgrid))
     seed = BASE_SEED + 1000 * int(system_id)
@@ -96,8 +143,31 @@ def make_system(kgrid, no, nmu, system_id=0, with_v=True, trev=False):
                 Z[k] = Z[k].real
             elif p > k:
                 C[p], E[p], X[p], Z[p] = C[k].conj(), E[k], X[k].conj(), Z[k].conj()
-    V = None
-    if with_v:
-        V = np.matmul(Z, np.conj(np.swapaxes(Z, 1, 2))) / nmu
+    qsym = None
+    if c4:
+        rep, power = c4_orbits(kgrid)
+        maps = _perm_c4(nmu, rng)
+        V0 = np.matmul(Z, np.conj(np.swapaxes(Z, 1, 2))) / nmu
+        Vs = np.empty_like(V0)
+        for r in sorted(set(rep.tolist())):
+            members = [k for k in range(nk) if rep[k] == r]
+            s_ = len(members)  # the stabiliser of r is generated by S^s_
+            stab = [maps[(s_ * b) % 4] for b in range(4 // s_)]
+            X[r] = sum(X[r][m] for m in stab) / len(stab)
+            X[r] /= np.linalg.norm(X[r], axis=1, keepdims=True)
+            Vs[r] = sum(V0[r][np.ix_(m, m)] for m in stab) / len(stab)
+            for k in members:
+                if k != r:
+                    m = maps[power[k]]
+                    C[k], E[k], X[k] = C[r], E[r], X[r][m]
+                    Vs[k] = Vs[r][np.ix_(m, m)]
+        perm = np.stack([maps[power[k]] for k in range(nk)]).astype(np.int32)
+        qsym = (rep.astype(np.int32), perm)
+        V = Vs if with_v else None
+        Z = None
+    else:
+        V = None
+        if with_v:
+            V = np.matmul(Z, np.conj(np.swapaxes(Z, 1, 2))) / nmu
     return System(kgrid=kgrid, no=no, nmu=nmu, seed=seed, C=C, E=np.ascontiguousarray(E), X=np.ascontiguousarray(X),
-                  V=V, Z=Z)
+                  V=V, Z=Z, qsym=qsym)

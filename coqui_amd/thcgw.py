@@ -28,6 +28,7 @@ EXPORTS = (
     "thcgw_mlp_create", "thcgw_mlp_destroy", "thcgw_mlp_last_error", "thcgw_mlp_num_params",
     "thcgw_mlp_get_params", "thcgw_mlp_set_params", "thcgw_mlp_forward", "thcgw_mlp_backward",
     "thcgw_mlp_adam_step", "thcgw_mlp_train_mse", "thcgw_eps_inv_head", "thcgw_sigma_div_correction",
+    "thcgw_set_qsym",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
@@ -105,6 +106,7 @@ def load_library(path=LIB_PATH):
         "thcgw_set_flag": (i, [p, i, l]),
         "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
         "thcgw_eps_inv_head": (i, [p, p, p, p, d, i, p, p, p]),
+        "thcgw_set_qsym": (i, [p, p, p, p, p]),
         "thcgw_sigma_div_correction": (i, [p, p, p, p, p, d, p, i, p]),
         "thcgw_lri_create": (i, [p, i, i, ctypes.c_ulonglong, ctypes.POINTER(p)]),
         "thcgw_lri_destroy": (None, [p]),
@@ -214,6 +216,16 @@ class ThcGW:
 
     def set_flag(self, flag, value):
         self._check(self.lib.thcgw_set_flag(self.h, int(flag), int(value)))
+
+    def set_qsym(self, src=None, perm=None, phase=None, conj=None):
+        """q-point symmetry map (thcgw_set_qsym); src=None clears it."""
+        if src is None:
+            self._check(self.lib.thcgw_set_qsym(self.h, None, None, None, None))
+            return
+        self._qsym = [np.ascontiguousarray(src, np.int32), np.ascontiguousarray(perm, np.int32),
+                      None if phase is None else np.ascontiguousarray(phase, np.complex128),
+                      None if conj is None else np.ascontiguousarray(conj, np.int32)]
+        self._check(self.lib.thcgw_set_qsym(self.h, *[None if a is None else _np_ptr(a) for a in self._qsym]))
 
     def profile_read(self):
         """{class: (ms, flops, launches)} of the GEMM launches recorded since the last read

@@ -312,6 +312,21 @@ int thcgw_mlp_backward(thcgw_mlp* m, int batch, const float* X, const float* dY,
 int thcgw_mlp_adam_step(thcgw_mlp* m, const float* grad, float lr, void* stream);
 int thcgw_mlp_train_mse(thcgw_mlp* m, int batch, const float* X, const float* T, float lr, float* loss, void* stream);
 
+/* q-point symmetry of the THC auxiliary basis (the IBZ reduction of scr_coulomb_t, which
+ * solves Dyson for the irreducible q only, src/methods/scr_coulomb/scr_coulomb_t.cpp:121-206
+ * over nqpts_ibz, with the basis rotations of thc_gw.icc:314-317 restricted to phased
+ * permutations of the THC points).  For every q: src[q] is the irreducible q it is
+ * generated from (src[src[q]] == src[q]); perm (nk x nmu, host) and phase (nk x nmu
+ * complex, host, may be NULL for 1) give
+ *   A(q)[P][Q] = phase[q][P] conj(phase[q][Q]) A^(src q)[perm[q][P]][perm[q][Q]]
+ * for A = V, Pi and W, where A^ = conj A if conj[q] != 0 (time reversal combined; conj may
+ * be NULL).  Irreducible q map to themselves with unit phases.  With a map set, the Dyson
+ * step (thcgw_dyson_w, thcgw_rpa_ec, thcgw_gw_iteration) solves the irreducible q only and
+ * fills the others; it refuses V or Pi(iW_0) that do not follow the map (1e-9 relative in FP64,
+ * 1e-5 in the mixed forms, whose Pi carries FP32 rounding).
+ * It takes precedence over flag 9.  src = NULL clears the map. */
+int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double* phase, const int* conj);
+
 /* Diagnostics (no counterpart in the reference): bytes of device workspace held by ctx,
  * control flags and GEMM launch timing.
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;

@@ -587,3 +587,27 @@ def test_sigma_div_correction_closed_form(orc):
         want = S[t, 0] + f * np.conj(h[:no])[None, :, None] * G[t, 0] * h[:no][None, None, :]
         assert np.allclose(out[t, 0], want, rtol=1e-13, atol=1e-14)
     assert np.array_equal(orc.sigma_div_correction(S, G, X, h, 1.7, eps_t, 0), S)
+
+
+@pytest.mark.parametrize("kgrid", [(4, 4, 1), (3, 3, 1), (2, 2, 2)])
+def test_c4_symmetry_relations(orc, kgrid):
+    """A system symmetric under the fourfold rotation, acting on the THC points as the
+    permutation pi_a, has Pi(S^a q) = Pi(q)[pi_a][:, pi_a] and the same for W: what lets the
+    GPU path solve Dyson on the irreducible q only (thcgw_set_qsym)."""
+    s = make_system(kgrid, 6, 20, system_id=7, c4=True)
+    src, perm = s.qsym
+    assert len(set(src.tolist())) < s.nk
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    G = orc.update_G(tab, s.C, s.E)
+    Piw = orc.tau_to_w_phsym(tab, orc.pi_rspace(G, s.X, kgrid))
+    W = orc.dyson(Piw, s.V)
+    for A in (Piw, W):
+        for q in range(s.nk):
+            m = perm[q]
+            B = A[:, src[q]][:, m][:, :, m]
+            assert np.max(np.abs(A[:, q] - B)) < 1e-12 * np.max(np.abs(A))
+    # a map that is off by one transposition breaks the relation
+    q = int(np.nonzero(src != np.arange(s.nk))[0][0])
+    m = perm[q].copy()
+    m[[0, 1]] = m[[1, 0]]
+    assert np.max(np.abs(W[:, q] - W[:, src[q]][:, m][:, :, m])) > 1e-3 * np.max(np.abs(W))
