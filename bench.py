@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--trev", action="store_true", help="draw the system time-reversal symmetric (for *_trev forms)")
     ap.add_argument("--c4", action="store_true", help="draw the system symmetric under a fourfold rotation (*_c4 forms)")
     ap.add_argument("--krylov-max-iter", type=int, default=60, help="cfg5: CG iterations before the direct fallback")
+    ap.add_argument("--compare-modes", type=int, default=1,
+                    help="cfg5: also solve the final state's Dyson step with every preconditioner (0 to skip)")
     ap.add_argument("--forms", default="plain,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
     args = ap.parse_args()
@@ -556,6 +558,27 @@ def run_scgw(args):
                     max_iter=args.krylov_max_iter, tol=1e-10, log=show("learned"))
     got = final_state(ctx, sc)
     kry = [r for r in hist_l if r["lri"] and r["lri"]["krylov_accepted"] + r["lri"]["fallbacks"] > 0]
+    # one more Dyson step at the final state with every preconditioner and the direct solve
+    modes = {}
+    if args.compare_modes:
+        a, w = sc._buffers()
+        ctx.pi(sc.G, sc.X, out=a)
+        ctx.tau_to_w(a, out=w)
+        Pi0 = w.clone()
+        for mode in ("direct", "none", "woodbury", "previous", "learned"):
+            w.copy_(Pi0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if mode == "direct":
+                st = lri.dyson(w, sc.V, krylov=False, keep_state=True)
+            else:
+                lri.set(mode, max_iter=args.krylov_max_iter, tol=1e-10)
+                st = lri.dyson(w, sc.V, krylov=True, keep_state=True)
+            torch.cuda.synchronize()
+            modes[mode] = {"s": round(time.perf_counter() - t0, 3), "cg_iterations_mean": round(st["iter_mean"], 2),
+                           "fallbacks": st["fallbacks"]}
+            log("bench: final-state Dyson, %s: %s" % (mode, modes[mode]))
+        del Pi0
     t_plain = sum(r["t_total"] for r in hist_p)
     t_learned = sum(r["t_total"] for r in hist_l)
     dqp = np.abs(got["E"] - ref["E"]) / HA_PER_MEV
@@ -583,6 +606,7 @@ def run_scgw(args):
         "krylov_accepted": sum(r["lri"]["krylov_accepted"] for r in kry),
         "resid_max": max([r["lri"]["resid_max"] for r in kry] or [0.0]),
         "training": getattr(sc, "train_info", None),
+        "final_state_dyson_by_mode": modes,
         "errors_vs_plain": {"qp_meV_max": float(dqp.max()), "qp_meV_median": float(np.median(dqp)),
                             "e_corr_gm_Ha": abs(got["e_corr"] - ref["e_corr"]), "mu_Ha": abs(got["mu"] - ref["mu"]),
                             "Sigma_rel": float((got["Sigma"] - ref["Sigma"]).abs().max() / ref["Sigma"].abs().max())},
