@@ -68,12 +68,13 @@ __global__ __launch_bounds__(CB* CB) void potrf_leaf_kernel(c128* A, long sA, in
   if (i < nb && k <= i) Ab[static_cast<long>(i) * lda + k] = S[i][k];
 }
 
-// X L^H = T for the lower non-unit nb x nb block L (nb <= 32); one thread per row of T.
+// X L^H = T for the lower non-unit nb x nb block L (nb <= CBL); one thread per row of T.
 // RC is the complex arithmetic type; T and L may be stored in another precision.
-template <typename RC, typename TT, typename TL>
+template <typename RC, typename TT, typename TL, int CBL = CB>
 __global__ __launch_bounds__(256) void trsm_rlc_leaf_kernel(const TL* L, long sL, int ldl, TT* T, long sT, int ldt,
                                                             int m, int nb) {
   using S = decltype(RC{}.x);
+  constexpr int CB = CBL;
   __shared__ RC Ls[CB][CB];
   const int b = blockIdx.y;
   const TL* Lb = L + static_cast<long>(b) * sL;
@@ -243,18 +244,21 @@ void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128*
 
 // The same in FP32 arithmetic with T stored as complex<float> and L as complex<double> or
 // complex<float>; gemm_flags selects the product of the updates (0: FP32 MFMA,
-// GEMM_SPLIT_BF16).
+// GEMM_SPLIT_BF16).  The leaves are 64 wide (64 complex<float> per thread), so that no
+// update product is narrower than the 64 x 64 GEMM tile.
+constexpr int CB32 = 64;
+
 template <typename TL>
 void trsm_rlc_f32(hipStream_t s, const TL* L, long sL, int ldl, c64* T, long sT, int ldt, int m, int nb, int batch,
                   int gemm_flags) {
   if (m <= 0 || nb <= 0 || batch <= 0) return;
-  if (nb <= CB) {
-    hipLaunchKernelGGL((trsm_rlc_leaf_kernel<c64, c64, TL>), dim3((m + 255) / 256, batch), dim3(256), 0, s, L, sL,
-                       ldl, T, sT, ldt, m, nb);
+  if (nb <= CB32) {
+    hipLaunchKernelGGL((trsm_rlc_leaf_kernel<c64, c64, TL, CB32>), dim3((m + 255) / 256, batch), dim3(256), 0, s, L,
+                       sL, ldl, T, sT, ldt, m, nb);
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(nb, CB), n2 = nb - n1;
+  const int n1 = split_point(nb, CB32), n2 = nb - n1;
   trsm_rlc_f32(s, L, sL, ldl, T, sT, ldt, m, n1, batch, gemm_flags);
   cgemm(s, OP_N, OP_C, m, n2, n1, kMinusOne, bp(T, sT), DT_C64, ldt, bp(L + static_cast<long>(n1) * ldl, sL),
         sizeof(TL) == sizeof(c128) ? DT_C128 : DT_C64, ldl, kOne, T + n1, DT_C64, sT, ldt, batch, Cplx{0.0, 0.0},
