@@ -42,11 +42,11 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # name: kgrid, N_orb, N_mu, IR table
-    "cfg2": ((4, 4, 4), 52, 800, "1e3.1e-15"),
-    "cfg1": ((2, 2, 2), 26, 200, "1e2.1e-15"),
-    "cfg3": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
-    "cfg4": None,  # batched: see batch_systems()
-    "cfg5": ((4, 4, 4), 78, 1200, "1e3.1e-15"),  # self-consistent GW, learned Dyson form: see run_scgw()
+    "k64-mu800": ((4, 4, 4), 52, 800, "1e3.1e-15"),
+    "k8-mu200": ((2, 2, 2), 26, 200, "1e2.1e-15"),
+    "k64-mu1600": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
+    "batch24": None,  # batched: see batch_systems()
+    "scgw-k64-mu1200": ((4, 4, 4), 78, 1200, "1e3.1e-15"),  # self-consistent GW, learned Dyson form: see run_scgw()
 }
 # The batched workload asks for N_tau = 60; the nearest shipped table (1e5.1e-06, 61 points)
 # makes the Pade continuation of the QP equation move by tens of meV under 1e-7 relative
@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="k64-mu800", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
@@ -176,12 +176,12 @@ def main():
     ap.add_argument("--gemm-tile32", type=int, default=0, help="FP32 GEMM tile shape: 0 64x64, 1 128x64, 2 128x128")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
-    ap.add_argument("--scgw-iters", type=int, default=10, help="cfg5: self-consistent iterations")
+    ap.add_argument("--scgw-iters", type=int, default=10, help="scgw-k64-mu1200: self-consistent iterations")
     ap.add_argument("--trev", action="store_true", help="draw the system time-reversal symmetric (for *_trev forms)")
     ap.add_argument("--c4", action="store_true", help="draw the system symmetric under a fourfold rotation (*_c4 forms)")
-    ap.add_argument("--krylov-max-iter", type=int, default=60, help="cfg5: CG iterations before the direct fallback")
+    ap.add_argument("--krylov-max-iter", type=int, default=60, help="scgw-k64-mu1200: CG iterations before the direct fallback")
     ap.add_argument("--compare-modes", type=int, default=1,
-                    help="cfg5: also solve the final state's Dyson step with every preconditioner (0 to skip)")
+                    help="scgw-k64-mu1200: also solve the final state's Dyson step with every preconditioner (0 to skip)")
     ap.add_argument("--forms", default="plain,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
     args = ap.parse_args()
@@ -205,9 +205,9 @@ def main():
 
     from coqui_amd import IAFT, ThcGW, make_system
 
-    if args.config == "cfg4":
+    if args.config == "batch24":
         return run_batch(args)
-    if args.config == "cfg5":
+    if args.config == "scgw-k64-mu1200":
         return run_scgw(args)
     kgrid, no, nmu, table = CONFIGS[args.config]
     beta = 1000.0
@@ -410,9 +410,9 @@ def run_batch(args):
     ia = IAFT.from_table(beta, BATCH_TABLE)
     specs = batch_systems()
     need = sum(int(np.prod(k)) * (ia.nt_half + 2) * nmu * nmu * 16 for k, no, nmu, _ in specs)
-    log("bench: cfg4 batch of %d systems, %.1f GB of per-system arrays" % (len(specs), need / 1e9))
+    log("bench: batch of %d systems, %.1f GB of per-system arrays" % (len(specs), need / 1e9))
     if need > 200e9:
-        print("bench.py: the cfg4 draw needs %.0f GB of per-system arrays" % (need / 1e9), file=sys.stderr)
+        print("bench.py: the batch draw needs %.0f GB of per-system arrays" % (need / 1e9), file=sys.stderr)
         sys.exit(2)
     sysd = []
     t0 = time.perf_counter()
@@ -496,7 +496,7 @@ def run_batch(args):
         "vs_baseline": None,
         "dtype": "c128 in/out, FP32 products, FP64 IR transforms and factorisations",
         "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
-        "config": {"workload": "cfg4 batched mixed-precision G0W0 iteration of %d systems" % len(specs),
+        "config": {"workload": "batched mixed-precision G0W0 iteration of %d systems" % len(specs),
                    "systems": [{"kgrid": list(k), "n_orb": no, "n_mu": nmu} for k, no, nmu, _ in specs],
                    "ir_table": BATCH_TABLE, "n_tau": ia.nt, "beta": beta, "parallelism": "single GPU"},
         "ms_per_step_one_by_one": round(ms_eager, 3),
@@ -522,12 +522,12 @@ def run_scgw(args):
 
     from coqui_amd import IAFT, LearnedDyson, ScGW, ThcGW, make_system
 
-    kgrid, no, nmu, table = CONFIGS["cfg5"]
+    kgrid, no, nmu, table = CONFIGS["scgw-k64-mu1200"]
     beta = 1000.0
     torch.cuda.set_device(0)
     ia = IAFT.from_table(beta, table)
     s = make_system(kgrid, no, nmu, system_id=0)
-    log("bench: cfg5 scGW kgrid=%s no=%d nmu=%d nt=%d, %d iterations" % (kgrid, no, nmu, ia.nt, args.scgw_iters))
+    log("bench: scGW kgrid=%s no=%d nmu=%d nt=%d, %d iterations" % (kgrid, no, nmu, ia.nt, args.scgw_iters))
 
     def final_state(ctx, sc):
         H = torch.from_numpy(s.H).cuda()
@@ -595,7 +595,7 @@ def run_scgw(args):
         "vs_baseline": None,
         "dtype": "c128 (FP64 Krylov and direct solves); network BF16 weights, FP32 accumulation",
         "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
-        "config": {"workload": "cfg5 self-consistent GW, learned Krylov Dyson form", "kgrid": list(kgrid),
+        "config": {"workload": "self-consistent GW (10 iterations), learned Krylov Dyson form", "kgrid": list(kgrid),
                    "n_orb": no, "n_mu": nmu, "ir_table": table, "n_tau": ia.nt, "beta": beta,
                    "iterations": args.scgw_iters, "rank": 64, "width": 512, "parallelism": "single GPU"},
         "plain_s_per_iteration": round(t_plain / len(hist_p), 4),

@@ -41,7 +41,7 @@ def system(orc, kgrid, no, nmu, table, sid=0):
 
 
 @pytest.fixture(scope="module")
-def cfg1(orc, torch):
+def nk8_nmu200(orc, torch):
     s, ia, tab, G = system(orc, (2, 2, 2), 26, 200, "1e2.1e-15")
     ref = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=True)
     ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
@@ -49,8 +49,8 @@ def cfg1(orc, torch):
     return dict(s=s, tab=tab, G=G, ref=ref, ctx=ctx)
 
 
-def test_hermitian_dyson_config1(cfg1, torch):
-    c = cfg1
+def test_hermitian_dyson_nk8_nmu200(nk8_nmu200, torch):
+    c = nk8_nmu200
     PiW = torch.from_numpy(c["ref"]["Pi_w"].copy()).cuda()
     c["ctx"].dyson_w(PiW, c["s"].V)
     assert relmax(to_np(PiW), c["ref"]["W_w"]) < TOL
@@ -59,8 +59,8 @@ def test_hermitian_dyson_config1(cfg1, torch):
     assert relmax(W, np.conj(np.swapaxes(W, -1, -2))) < 1e-14
 
 
-def test_hermitian_terms_and_rpa_ec(cfg1, torch):
-    c = cfg1
+def test_hermitian_terms_and_rpa_ec(nk8_nmu200, torch):
+    c = nk8_nmu200
     c["ctx"].set_flag(3, 7)  # ragged chunks
     try:
         PiW = torch.from_numpy(c["ref"]["Pi_w"].copy()).cuda()
@@ -82,8 +82,8 @@ def test_hermitian_terms_and_rpa_ec(cfg1, torch):
     assert abs(ec2 - c["ref"]["Ec"]) < TOL * abs(c["ref"]["Ec"])
 
 
-def test_hermitian_gw_iteration_config1(cfg1, torch):
-    c = cfg1
+def test_hermitian_gw_iteration_nk8_nmu200(nk8_nmu200, torch):
+    c = nk8_nmu200
     S, W, ec = c["ctx"].gw_iteration(c["G"], c["s"].X, c["s"].V, with_ec=True, hermitize=True)
     assert relmax(to_np(W), c["ref"]["W_qt"]) < TOL
     assert relmax(to_np(S), c["ref"]["Sigma"]) < TOL

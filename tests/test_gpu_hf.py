@@ -31,7 +31,7 @@ def density_matrix(s, beta=1000.0):
 
 
 @pytest.mark.parametrize("kgrid,no,nmu", [
-    ((2, 2, 2), 26, 200),     # config 1 sizes
+    ((2, 2, 2), 26, 200),     # N_k = 8, N_orb = 26, N_mu = 200
     ((4, 4, 4), 12, 70),      # lane-split N_k = 64 transform
     ((3, 3, 3), 9, 37),
     ((1, 1, 1), 7, 50),       # Gamma only

@@ -50,7 +50,7 @@ def system(orc, kgrid, no, nmu, table, sid=0):
 
 
 @pytest.fixture(scope="module")
-def cfg1(orc, torch):
+def nk8_nmu200(orc, torch):
     s, ia, tab, G = system(orc, (2, 2, 2), 26, 200, "1e2.1e-15")
     ref = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=True)
     ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
@@ -58,8 +58,8 @@ def cfg1(orc, torch):
 
 
 @pytest.mark.parametrize("prec", [1, 2])
-def test_mixed_stages_config1(cfg1, torch, prec):
-    c, s = cfg1, cfg1["s"]
+def test_mixed_stages_nk8_nmu200(nk8_nmu200, torch, prec):
+    c, s = nk8_nmu200, nk8_nmu200["s"]
     ctx, ref = c["ctx"], c["ref"]
     ctx.set_flag(7, prec)
     e_pi = relmax(to_np(ctx.pi(c["G"], s.X)), ref["Pi_t"])
@@ -72,8 +72,8 @@ def test_mixed_stages_config1(cfg1, torch, prec):
 
 
 @pytest.mark.parametrize("prec", [1, 2])
-def test_mixed_iteration_config1(cfg1, orc, torch, prec):
-    c, s = cfg1, cfg1["s"]
+def test_mixed_iteration_nk8_nmu200(nk8_nmu200, orc, torch, prec):
+    c, s = nk8_nmu200, nk8_nmu200["s"]
     c["ctx"].set_flag(7, prec)
     S, W, ec = c["ctx"].gw_iteration(c["G"], s.X, s.V, with_ec=True, hermitize=True)
     S, W = to_np(S), to_np(W)

@@ -41,23 +41,23 @@ def setup(orc, kgrid, no, nmu, table, sid=0, beta=1000.0):
 
 
 @pytest.fixture(scope="module")
-def cfg1(orc, torch):
-    """N_k = 8 (2x2x2), N_orb = 26, N_mu = 200, N_tau = 40, beta = 1000."""
+def nk8_nmu200(orc, torch):
+    """The N_k = 8 (2x2x2), N_orb = 26, N_mu = 200, N_tau = 40, beta = 1000 system."""
     s, ia, tab, G = setup(orc, (2, 2, 2), 26, 200, "1e2.1e-15")
     ref = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=True)
     ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
     return dict(s=s, ia=ia, tab=tab, G=G, ref=ref, ctx=ctx)
 
 
-def test_update_G(cfg1, torch):
-    c = cfg1
+def test_update_G(nk8_nmu200, torch):
+    c = nk8_nmu200
     G = c["ctx"].update_G(c["s"].C, c["s"].E, 0.0)
     torch.cuda.synchronize()
     assert relmax(to_np(G), c["G"]) < 1e-13
 
 
-def test_projections(cfg1, orc, torch):
-    c, s = cfg1, cfg1["s"]
+def test_projections(nk8_nmu200, orc, torch):
+    c, s = nk8_nmu200, nk8_nmu200["s"]
     rng = np.random.default_rng(1)
     O = rng.standard_normal((3, s.nk, s.no, s.no)) + 1j * rng.standard_normal((3, s.nk, s.no, s.no))
     P = c["ctx"].primary_to_aux(s.X, O)
@@ -67,31 +67,31 @@ def test_projections(cfg1, orc, torch):
     assert relmax(to_np(A), orc.aux_to_primary(s.X, Q)) < 1e-13
 
 
-def test_pi_config1(cfg1, torch):
-    c = cfg1
+def test_pi_nk8_nmu200(nk8_nmu200, torch):
+    c = nk8_nmu200
     P = c["ctx"].pi(c["G"], c["s"].X)
     assert relmax(to_np(P), c["ref"]["Pi_t"]) < TOL
 
 
-def test_tau_to_w_and_back(cfg1, orc, torch):
-    c = cfg1
+def test_tau_to_w_and_back(nk8_nmu200, orc, torch):
+    c = nk8_nmu200
     Pw = c["ctx"].tau_to_w(c["ref"]["Pi_t"])
     assert relmax(to_np(Pw), c["ref"]["Pi_w"]) < TOL
     Wqt = c["ctx"].w_to_tau(c["ref"]["W_w"])
     assert relmax(to_np(Wqt), c["ref"]["W_qt"]) < TOL
 
 
-def test_dyson_config1(cfg1, torch):
-    c = cfg1
+def test_dyson_nk8_nmu200(nk8_nmu200, torch):
+    c = nk8_nmu200
     PiW = torch.from_numpy(c["ref"]["Pi_w"].copy()).cuda()
     c["ctx"].dyson_w(PiW, c["s"].V)
     assert relmax(to_np(PiW), c["ref"]["W_w"]) < TOL
 
 
-def test_dyson_chunked_and_terms(cfg1, orc, torch):
+def test_dyson_chunked_and_terms(nk8_nmu200, orc, torch):
     """Small Dyson batches (chunk of 5 matrices) give the same W; Tr and ln det terms
     reproduce the oracle's RPA energy."""
-    c = cfg1
+    c = nk8_nmu200
     c["ctx"].set_flag(3, 5)
     try:
         PiW = torch.from_numpy(c["ref"]["Pi_w"].copy()).cuda()
@@ -111,8 +111,8 @@ def test_dyson_chunked_and_terms(cfg1, orc, torch):
     assert abs(ec - c["ref"]["Ec"]) < TOL * abs(c["ref"]["Ec"])
 
 
-def test_sigma_config1(cfg1, orc, torch):
-    c = cfg1
+def test_sigma_nk8_nmu200(nk8_nmu200, orc, torch):
+    c = nk8_nmu200
     S = c["ctx"].sigma(c["G"], c["ref"]["W_qt"], c["s"].X, hermitize=True)
     assert relmax(to_np(S), c["ref"]["Sigma"]) < TOL
     S0 = c["ctx"].sigma(c["G"], c["ref"]["W_qt"], c["s"].X, hermitize=False)
@@ -120,14 +120,14 @@ def test_sigma_config1(cfg1, orc, torch):
     assert relmax(to_np(S0), ref0) < TOL
 
 
-def test_rpa_ec_config1(cfg1, torch):
-    c = cfg1
+def test_rpa_ec_nk8_nmu200(nk8_nmu200, torch):
+    c = nk8_nmu200
     ec = c["ctx"].rpa_ec(c["ref"]["Pi_w"], c["s"].V)
     assert abs(ec - c["ref"]["Ec"]) < TOL * abs(c["ref"]["Ec"])
 
 
-def test_gw_iteration_config1(cfg1, torch):
-    c = cfg1
+def test_gw_iteration_nk8_nmu200(nk8_nmu200, torch):
+    c = nk8_nmu200
     S, W, ec = c["ctx"].gw_iteration(c["G"], c["s"].X, c["s"].V, with_ec=True, hermitize=True)
     assert relmax(to_np(S), c["ref"]["Sigma"]) < TOL
     assert relmax(to_np(W), c["ref"]["W_qt"]) < TOL
@@ -135,8 +135,8 @@ def test_gw_iteration_config1(cfg1, torch):
     print("Ec gpu %.15e oracle %.15e" % (ec, c["ref"]["Ec"]))
 
 
-def test_qp_and_gm_config1(cfg1, orc, torch):
-    c, s = cfg1, cfg1["s"]
+def test_qp_and_gm_nk8_nmu200(nk8_nmu200, orc, torch):
+    c, s = nk8_nmu200, nk8_nmu200["s"]
     H = s.H
     Sig = c["ref"]["Sigma"]
     E_gpu = c["ctx"].solve_qp(Sig, s.C, H, s.E, mu=0.0, nfit=18, eta=1e-4, tol=1e-8)
