@@ -11,6 +11,9 @@ Forms (--forms, the first is the headline value):
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
              transforms, Cholesky factorisations and ln det;
   mixed_bf16 as mixed, with the FP32 products done as split-BF16 MFMA products;
+  resolvent  FP64 throughout, Dyson by the resolvent form W + V = (V^-1 - Pi)^-1 (one
+             Cholesky factorisation, triangular inverse and triangular product per matrix);
+  mixed_resolvent  FP32 projections and k-grid transforms, FP64 resolvent Dyson step;
   *_trev     the form with the Dyson step on the time-reversal irreducible q only (needs
              --trev, which draws the system time-reversal symmetric);
   *_c4       the form with the Dyson step on the q irreducible under a fourfold rotation
@@ -72,8 +75,11 @@ def batch_systems(n=24):
 # form: (flag 6 Dyson algorithm, flag 7 precision, symmetry: 0 none, 1 time reversal (flag 9),
 # 2 the C4 q-point map (thcgw_set_qsym))
 FORMS = {"plain": (0, 0, 0), "hermitian": (1, 0, 0), "mixed": (1, 1, 0), "mixed_bf16": (1, 2, 0),
+         "resolvent": (2, 0, 0), "mixed_resolvent": (2, 1, 0),
          "plain_trev": (0, 0, 1), "hermitian_trev": (1, 0, 1), "mixed_trev": (1, 1, 1),
-         "plain_c4": (0, 0, 2), "hermitian_c4": (1, 0, 2), "mixed_c4": (1, 1, 2)}
+         "resolvent_trev": (2, 0, 1), "mixed_resolvent_trev": (2, 1, 1),
+         "plain_c4": (0, 0, 2), "hermitian_c4": (1, 0, 2), "mixed_c4": (1, 1, 2),
+         "resolvent_c4": (2, 0, 2), "mixed_resolvent_c4": (2, 1, 2)}
 HA_PER_MEV = 1.0 / 27211.386245988
 
 
@@ -183,7 +189,8 @@ def main():
     ap.add_argument("--compare-modes", type=int, default=1,
                     help="scgw-k64-mu1200: also solve the final state's Dyson step with every preconditioner (0 to skip)")
     ap.add_argument("--forms", default="plain,mixed",
-                    help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed)")
+                    help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed, "
+                         "mixed_bf16, resolvent, mixed_resolvent, and their _trev / _c4 variants)")
     args = ap.parse_args()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     for f in forms:

@@ -62,7 +62,8 @@ struct thcgw_ctx {
   bool force_fallback = false;
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
-  int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky)
+  int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky),
+                        // 2 = resolvent (V^-1 - Pi)^-1
   int precision = 0;    // 0 = FP64 throughout, 1 = mixed (FP32 products, FP64 factorisations and IR),
                         // 2 = mixed with split-BF16 products in place of FP32 MFMA
   int tile32 = -1;      // FP32 tile shape chosen for this context (-1: process default)
@@ -427,23 +428,64 @@ bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv
   return true;
 }
 
-// Factors of V for the Hermitian form (V members: vq()).  hpd is false when the LU form is
-// selected or some V_q is not positive definite.
+// One chunk of the resolvent form: W + V = (I - V Pi)^-1 V = (V^-1 - Pi)^-1 = A^-1, with
+// A = V^-1 - Pi = L_A L_A^H (Hermitian positive definite for Pi <= 0), A^-1 = Z Z^H with
+// Z = L_A^-H, and ln det(I - V Pi) = ln det V + ln det A.  V^-1 and ln det V are formed once
+// per q (factor_dyson_v).  Per matrix this is n^3/3 complex multiply-adds for the Cholesky
+// factorisation together with the inverse of its factor (cholinv_batched) and n^3/6 for
+// the triangular product, all in FP64.  Returns false, leaving the chunk's Pi slots
+// untouched, when some A is not positive definite.
+bool dyson_chunk_inv(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Vi, const double* ldv, int b0, int cb,
+                     c128* d_trace, double* d_ldet, bool solve, c128* Ab, c128* Z, int* info, hipStream_t s) {
+  const int n = ctx->d.nmu, nq = ctx->vq();
+  const long M = ctx->M;
+  c128* P = PiW + static_cast<long>(b0) * M;
+  if (d_trace) thcgw::trace_vp(s, V, M, nq, b0, P, M, n, d_trace + b0, cb);
+  thcgw::resolvent_lower(s, Vi, M, nq, b0, P, Ab, M, n, cb);
+  thcgw::cholinv_batched(s, Ab, M, Z, M, n, info, cb);
+  if (!ctx->deferred && any_nonzero(info, cb, s)) return false;
+  if (d_ldet) thcgw::chol_logdet(s, Ab, M, n, d_ldet + b0, cb, ldv, nq, b0);
+  if (solve) {
+    // W = Z Z^H - V from the lower tiles; Z is upper triangular, so the sum starts at the row
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(Z, M), n, thcgw::bp(Z, M), n, kZero, P, M, n,
+                 cb, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW | thcgw::GEMM_HERM_SUB,
+                 thcgw::bp(V, M, nq, 1, b0));
+  }
+  return true;
+}
+
+// Factors of V for the Hermitian and resolvent forms (V members: vq()).  hpd is false when
+// the LU form is selected or some V_q is not positive definite; inv selects the resolvent
+// form, for which Vi holds the lower triangles of V_q^-1 and ldv ln det V_q.
 struct VFactors {
   bool hpd = false;
+  bool inv = false;
   c128* Lv = nullptr;
   c64* Lv64 = nullptr;
+  c128* Vi = nullptr;
+  double* ldv = nullptr;
 };
 
 VFactors factor_dyson_v(thcgw_ctx* ctx, const c128* V, hipStream_t s) {
   VFactors f;
   const bool mixed = ctx->precision != 0;
-  if (!(mixed || ctx->dyson_algo == 1)) return f;
-  const int nq = ctx->vq();
+  if (!(mixed || ctx->dyson_algo != 0)) return f;
+  const int nq = ctx->vq(), n = ctx->d.nmu;
   const long M = ctx->M;
   f.Lv = ctx->buf<c128>("dy_lv", static_cast<size_t>(nq) * M);
   f.hpd = factor_v(ctx, V, f.Lv, s);
-  if (f.hpd && mixed) {
+  if (f.hpd && ctx->dyson_algo == 2) {
+    // V^-1 = T T^H with T = L^-H (upper triangular): lower tiles, K from the row
+    f.inv = true;
+    f.ldv = ctx->buf<double>("dy_ldv", static_cast<size_t>(nq));
+    thcgw::chol_logdet(s, f.Lv, M, n, f.ldv, nq);
+    c128* T = ctx->buf<c128>("dy_vt", static_cast<size_t>(nq) * M);
+    thcgw::tri_inverse_conj(s, f.Lv, M, n, T, M, n, n, nq);
+    f.Vi = f.Lv;  // L is no longer needed
+    f.Lv = nullptr;
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, n, n, n, kOne, thcgw::bp(T, M), n, thcgw::bp(T, M), n, kZero, f.Vi, M, n,
+                 nq, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW);
+  } else if (f.hpd && mixed) {
     f.Lv64 = ctx->buf<c64>("dy_lv64", static_cast<size_t>(nq) * M);
     thcgw::to_c64(s, f.Lv, f.Lv64, static_cast<long>(nq) * M);
   }
@@ -465,7 +507,10 @@ void dyson_matrices(thcgw_ctx* ctx, c128* PiW, const c128* V, const VFactors& f,
   int* ipiv = nullptr;
   for (int b0 = 0; b0 < total; b0 += CH) {
     const int cb = std::min(CH, total - b0);
-    if (f.hpd && mixed) {
+    if (f.inv) {
+      if (!Z) Z = ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M);
+      if (dyson_chunk_inv(ctx, PiW, V, f.Vi, f.ldv, b0, cb, d_trace, d_ldet, solve, LU, Z, info_all + b0, s)) continue;
+    } else if (f.hpd && mixed) {
       if (!Z64) Z64 = ctx->buf<c64>("dy_z64", static_cast<size_t>(CH) * M);
       if (!Mb64) Mb64 = ctx->buf<c64>("dy_m64", static_cast<size_t>(CH) * M);
       if (dyson_chunk_hpd_f32(ctx, PiW, V, f.Lv64, b0, cb, d_trace, d_ldet, solve, LU, Z64, Mb64, info_all + b0, s))
@@ -828,7 +873,7 @@ void enqueue_iteration(thcgw_ctx* ctx, const c128* G, const c128* X, const c128*
 // After a deferred enqueue has completed: Cholesky / LU flags of the Dyson step.
 void check_deferred(thcgw_ctx* ctx, hipStream_t s) {
   const int total = ctx->nw_half * ctx->nk, nq = ctx->nk;
-  if (ctx->precision != 0 || ctx->dyson_algo == 1) {
+  if (ctx->precision != 0 || ctx->dyson_algo != 0) {
     std::vector<int> vi(nq);
     THCGW_HIP(hipMemcpyAsync(vi.data(), ctx->buf<int>("dy_vinfo", static_cast<size_t>(nq)), sizeof(int) * nq,
                              hipMemcpyDeviceToHost, s));
@@ -1026,7 +1071,7 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 8 (FP32 tile) takes 0, 1 or 2");
       thcgw::gemm_set_tile_mode32(static_cast<int>(value));
     } else if (flag == 6) {
-      THCGW_REQUIRE(value == 0 || value == 1, "flag 6 (Dyson algorithm) takes 0 (LU) or 1 (Hermitian)");
+      THCGW_REQUIRE(value >= 0 && value <= 2, "flag 6 (Dyson algorithm) takes 0 (LU), 1 (Hermitian) or 2 (resolvent)");
       ctx->dyson_algo = static_cast<int>(value);
     } else if (flag == 7) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 7 (precision) takes 0 (FP64), 1 (mixed) or 2 (mixed, split BF16)");

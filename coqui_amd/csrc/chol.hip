@@ -17,6 +17,7 @@
 // A22 <- A22 - A21 A21^H (lower tiles only), factor A22.  Leaves of order <= 32 are
 // factored in LDS by one workgroup per matrix.  Only the lower triangle of a factored
 // matrix is defined.
+#include "lri.hpp"
 #include "thcgw_internal.hpp"
 
 #include <cmath>
@@ -66,6 +67,76 @@ __global__ __launch_bounds__(CB* CB) void potrf_leaf_kernel(c128* A, long sA, in
     __syncthreads();
   }
   if (i < nb && k <= i) Ab[static_cast<long>(i) * lda + k] = S[i][k];
+}
+
+// The potrf leaf followed by the inverse of its factor: the nb x nb (nb <= 32) diagonal
+// block at [j0][j0] is factored in place (A = L L^H, lower), and the block of Z at
+// [j0][j0] receives L^-H (upper triangular, zeros below the diagonal).  Column j of L^-1 is
+// formed by forward substitution by thread j.
+__global__ __launch_bounds__(CB* CB) void cholinv_leaf_kernel(c128* A, long sA, int lda, c128* Z, long sZ, int ldz,
+                                                              int j0, int nb, int* info) {
+  __shared__ c128 S[CB][CB + 1];
+  __shared__ c128 R[CB][CB + 1];
+  __shared__ double s_inv;
+  const int b = blockIdx.x;
+  c128* Ab = A + static_cast<long>(b) * sA + static_cast<long>(j0) * lda + j0;
+  c128* Zb = Z + static_cast<long>(b) * sZ + static_cast<long>(j0) * ldz + j0;
+  const int t = threadIdx.x;
+  const int i = t / CB, k = t % CB;
+  S[i][k] = (i < nb && k <= i) ? Ab[static_cast<long>(i) * lda + k] : c128{0.0, 0.0};
+  R[i][k] = c128{0.0, 0.0};
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    if (t == 0) {
+      double d = S[j][j].x;
+      if (!(d > 0.0)) {
+        if (info[b] == 0) info[b] = j0 + j + 1;
+        d = 1.0;
+      }
+      const double l = sqrt(d);
+      S[j][j] = c128{l, 0.0};
+      s_inv = 1.0 / l;
+    }
+    __syncthreads();
+    if (k == j && i > j && i < nb) {
+      const double r = s_inv;
+      S[i][j] = c128{S[i][j].x * r, S[i][j].y * r};
+    }
+    __syncthreads();
+    if (i > j && k > j && k <= i && i < nb) {
+      const c128 u = cmulc(S[i][j], S[k][j]);
+      S[i][k] = c128{S[i][k].x - u.x, S[i][k].y - u.y};
+    }
+    __syncthreads();
+  }
+  if (i < nb && k <= i) Ab[static_cast<long>(i) * lda + k] = S[i][k];
+  if (t < nb) {  // column t of L^-1: R[r][t] = -(sum_{c=t}^{r-1} L[r][c] R[c][t]) / L[r][r]
+    const int j = t;
+    R[j][j] = c128{1.0 / S[j][j].x, 0.0};
+    for (int r = j + 1; r < nb; ++r) {
+      c128 acc{0.0, 0.0};
+      for (int c = j; c < r; ++c) {
+        const c128 u = cmul(S[r][c], R[c][j]);
+        acc.x += u.x;
+        acc.y += u.y;
+      }
+      const double d = -1.0 / S[r][r].x;
+      R[r][j] = c128{acc.x * d, acc.y * d};
+    }
+  }
+  __syncthreads();
+  if (i < nb && k < nb) {  // Z[i][k] = conj(L^-1[k][i]) on and above the diagonal
+    const c128 v = R[k][i];
+    Zb[static_cast<long>(i) * ldz + k] = (k >= i) ? c128{v.x, -v.y} : c128{0.0, 0.0};
+  }
+}
+
+// Zero a rows x cols block (leading dimension ld) of each matrix.
+__global__ __launch_bounds__(256) void zero_block_kernel(c128* A, long sA, int ld, int rows, int cols) {
+  const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= static_cast<long>(rows) * cols) return;
+  const int r = static_cast<int>(e / cols), c = static_cast<int>(e % cols);
+  A[static_cast<long>(blockIdx.y) * sA + static_cast<long>(r) * ld + c] = c128{0.0, 0.0};
 }
 
 // X L^H = T for the lower non-unit nb x nb block L (nb <= CBL); one thread per row of T.
@@ -136,8 +207,9 @@ __global__ __launch_bounds__(256) void zero_upper_kernel(c128* A, long sA, int n
   if (c > r) A[static_cast<long>(blockIdx.y) * sA + e] = c128{0.0, 0.0};
 }
 
-// ldet[b] = 2 sum_j ln Re L[j][j]
-__global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL, int n, double* ldet) {
+// ldet[b] = 2 sum_j ln Re L[j][j] (+ add[(b + aoff) % amod] when add is given)
+__global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL, int n, double* ldet,
+                                                          const double* add, int amod, int aoff) {
   __shared__ double red[256];
   const c128* Lb = L + static_cast<long>(blockIdx.x) * sL;
   double acc = 0.0;
@@ -148,7 +220,50 @@ __global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) ldet[blockIdx.x] = 2.0 * red[0];
+  if (threadIdx.x == 0)
+    ldet[blockIdx.x] = 2.0 * red[0] + (add ? add[(static_cast<int>(blockIdx.x) + aoff) % amod] : 0.0);
+}
+
+// Lower triangle of A[b] = Vi[(b + voff) % vmod] - P[b] (the diagonal included); the strict
+// upper triangle of A is not written.
+__global__ __launch_bounds__(256) void resolvent_lower_kernel(const c128* Vi, long sV, int vmod, int voff,
+                                                              const c128* P, c128* A, long sA, int n) {
+  const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  const long nn = static_cast<long>(n) * n;
+  if (e >= nn) return;
+  const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
+  if (c > r) return;
+  const int b = blockIdx.y;
+  const c128 v = Vi[static_cast<long>((b + voff) % vmod) * sV + e];
+  const c128 p = P[static_cast<long>(b) * sA + e];
+  A[static_cast<long>(b) * sA + e] = c128{v.x - p.x, v.y - p.y};
+}
+
+// tr[b] = sum_e conj(V[e]) P[e] over the n x n elements of V[(b + voff) % vmod] and P[b]:
+// Tr(V P) for Hermitian V.  One workgroup per matrix.
+__global__ __launch_bounds__(256) void trace_vp_kernel(const c128* V, long sV, int vmod, int voff, const c128* P,
+                                                       long sP, long nn, c128* tr) {
+  __shared__ double red[2][256];
+  const int b = blockIdx.x;
+  const c128* Vb = V + static_cast<long>((b + voff) % vmod) * sV;
+  const c128* Pb = P + static_cast<long>(b) * sP;
+  double ar = 0.0, ai = 0.0;
+  for (long e = threadIdx.x; e < nn; e += 256) {
+    const c128 v = Vb[e], p = Pb[e];
+    ar += v.x * p.x + v.y * p.y;
+    ai += v.x * p.y - v.y * p.x;
+  }
+  red[0][threadIdx.x] = ar;
+  red[1][threadIdx.x] = ai;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tr[b] = c128{red[0][0], red[1][0]};
 }
 
 // W <- H - V where H is the Hermitian matrix whose lower triangle W holds on entry.
@@ -222,7 +337,61 @@ void potrf_rec(hipStream_t s, c128* A, long sA, int lda, int j0, int n, int* inf
   potrf_rec(s, A, sA, lda, j0 + n1, n2, info, batch);
 }
 
+// Cholesky factorisation with the inverse of the factor, every off-diagonal step a GEMM:
+// for the diagonal block [j0, j0 + n) split as n1 + n2,
+//   (L11, Z11) of A11;  U = L21^H = Z11^H A21^H -> A12 (the unused upper block of A);
+//   A22 -= U^H U (lower tiles);  (L22, Z22) of A22;
+//   Z21 <- (U Z22)^H = Z22^H U^H (scratch in the zero lower block of Z);
+//   Z12 = -Z11 (U Z22) = -Z11 Z21^H;  Z21 <- 0,
+// which is Z = L^-H = [[Z11, -Z11 L21^H Z22], [0, Z22]].  n^3/3 complex multiply-adds, as
+// for a factorisation plus a triangular inverse, without the triangular solves' narrow
+// updates.  On exit A holds L on and below the diagonal (L21 is not stored: L^H's blocks
+// are left in the upper triangle) and Z holds L^-H with zeros below the diagonal.
+void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int ldz, int j0, int n, int* info,
+                 int batch) {
+  if (n <= CB) {
+    hipLaunchKernelGGL(cholinv_leaf_kernel, dim3(batch), dim3(CB * CB), 0, s, A, sA, lda, Z, sZ, ldz, j0, n, info);
+    THCGW_HIP(hipGetLastError());
+    return;
+  }
+  const int n1 = split_point(n, CB), n2 = n - n1;
+  cholinv_rec(s, A, sA, lda, Z, sZ, ldz, j0, n1, info, batch);
+  c128* A11 = A + static_cast<long>(j0) * lda + j0;
+  c128* A12 = A11 + n1;
+  c128* A21 = A11 + static_cast<long>(n1) * lda;
+  c128* A22 = A21 + n1;
+  c128* Z11 = Z + static_cast<long>(j0) * ldz + j0;
+  c128* Z12 = Z11 + n1;
+  c128* Z21 = Z11 + static_cast<long>(n1) * ldz;
+  c128* Z22 = Z21 + n1;
+  const Cplx zero{0.0, 0.0};
+  zgemm(s, OP_C, OP_C, n1, n2, n1, kOne, bp(Z11, sZ), ldz, bp(A21, sA), lda, zero, A12, sA, lda, batch, zero,
+        GEMM_TRI_KTO_ROW);
+  zgemm(s, OP_C, OP_N, n2, n2, n1, kMinusOne, bp(A12, sA), lda, bp(A12, sA), lda, kOne, A22, sA, lda, batch, zero,
+        GEMM_TRI_LOWER);
+  cholinv_rec(s, A, sA, lda, Z, sZ, ldz, j0 + n1, n2, info, batch);
+  zgemm(s, OP_C, OP_C, n2, n1, n2, kOne, bp(Z22, sZ), ldz, bp(A12, sA), lda, zero, Z21, sZ, ldz, batch, zero,
+        GEMM_TRI_KTO_ROW);
+  zgemm(s, OP_N, OP_C, n1, n2, n1, kMinusOne, bp(Z11, sZ), ldz, bp(Z21, sZ), ldz, zero, Z12, sZ, ldz, batch, zero,
+        GEMM_TRI_KFROM_ROW);
+  const long nb = static_cast<long>(n2) * n1;
+  hipLaunchKernelGGL(zero_block_kernel, dim3(static_cast<unsigned>((nb + 255) / 256), batch), dim3(256), 0, s, Z21, sZ,
+                     ldz, n2, n1);
+  THCGW_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+// A = L L^H in place (lower; the strict upper triangle is overwritten with scratch) and
+// Z = L^-H (upper, zeros below the diagonal) for `batch` Hermitian n x n matrices, both
+// packed (leading dimension n).  info as for potrf_batched.
+void cholinv_batched(hipStream_t s, c128* A, long sA, c128* Z, long sZ, int n, int* info, int batch) {
+  THCGW_REQUIRE(n >= 1, "cholinv: order must be positive");
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(fill_int0_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, info, batch);
+  THCGW_HIP(hipGetLastError());
+  cholinv_rec(s, A, sA, n, Z, sZ, n, 0, n, info, batch);
+}
 
 // X L^H = T, L lower non-unit nb x nb, T m x nb; T is overwritten with X.
 void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* T, long sT, int ldt, int m, int nb,
@@ -240,6 +409,38 @@ void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128*
   zgemm(s, OP_N, OP_C, m, n2, n1, kMinusOne, bp(T, sT), ldt, bp(L + static_cast<long>(n1) * ldl, sL), ldl, kOne,
         T + n1, sT, ldt, batch);
   trsm_right_lower_conj(s, L + static_cast<long>(n1) * ldl + n1, sL, ldl, T + n1, sT, ldt, m, n2, batch);
+}
+
+namespace {
+// Column block c0 .. c0+nb-1 of X L^H = T for upper triangular T (T[r][c] = 0 for r > c):
+// X is upper triangular as well, so only rows 0 .. c0+nb-1 of the block can be nonzero, and
+// only those rows are solved and updated.  T and L point at column c0 and at L[c0][c0].
+void trsm_rlc_utri(hipStream_t s, const c128* L, long sL, int ldl, c128* T, long sT, int ldt, int c0, int nb,
+                   int batch) {
+  const int m = c0 + nb;
+  if (nb <= CB) {
+    hipLaunchKernelGGL((trsm_rlc_leaf_kernel<c128, c128, c128>), dim3((m + 255) / 256, batch), dim3(256), 0, s, L, sL,
+                       ldl, T, sT, ldt, m, nb);
+    THCGW_HIP(hipGetLastError());
+    return;
+  }
+  const int n1 = split_point(nb, CB), n2 = nb - n1;
+  trsm_rlc_utri(s, L, sL, ldl, T, sT, ldt, c0, n1, batch);
+  // T2 -= X1 (L21)^H over the rows where X1 can be nonzero
+  zgemm(s, OP_N, OP_C, c0 + n1, n2, n1, kMinusOne, bp(T, sT), ldt, bp(L + static_cast<long>(n1) * ldl, sL), ldl, kOne,
+        T + n1, sT, ldt, batch);
+  trsm_rlc_utri(s, L + static_cast<long>(n1) * ldl + n1, sL, ldl, T + n1, sT, ldt, c0 + n1, n2, batch);
+}
+}  // namespace
+
+// Z = L^-H (upper triangular) for lower non-unit n x n L: the triangular solve of
+// Z L^H = I restricted to the nonzero rows of each column block (n^3/6 complex
+// multiply-adds instead of n^3/2 for a full right-hand side).  Z is overwritten.
+void tri_inverse_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* Z, long sZ, int ldz, int n, int batch) {
+  if (n <= 0 || batch <= 0) return;
+  THCGW_REQUIRE(ldz == n, "tri_inverse_conj: Z must be packed (ldz = n)");
+  set_identity(s, Z, sZ, n, batch);
+  trsm_rlc_utri(s, L, sL, ldl, Z, sZ, ldz, 0, n, batch);
 }
 
 // The same in FP32 arithmetic with T stored as complex<float> and L as complex<double> or
@@ -295,9 +496,27 @@ void zero_upper(hipStream_t s, c128* A, long sA, int n, int batch) {
   THCGW_HIP(hipGetLastError());
 }
 
-void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch) {
+void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch, const double* add, int amod,
+                 int aoff) {
   if (batch <= 0) return;
-  hipLaunchKernelGGL(chol_logdet_kernel, dim3(batch), dim3(256), 0, s, L, sL, n, ldet);
+  hipLaunchKernelGGL(chol_logdet_kernel, dim3(batch), dim3(256), 0, s, L, sL, n, ldet, add, amod, aoff);
+  THCGW_HIP(hipGetLastError());
+}
+
+void resolvent_lower(hipStream_t s, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A, long sA, int n,
+                     int batch) {
+  if (batch <= 0 || n <= 0) return;
+  const long nn = static_cast<long>(n) * n;
+  hipLaunchKernelGGL(resolvent_lower_kernel, dim3(static_cast<unsigned>((nn + 255) / 256), batch), dim3(256), 0, s, Vi,
+                     sV, vmod, voff, P, A, sA, n);
+  THCGW_HIP(hipGetLastError());
+}
+
+void trace_vp(hipStream_t s, const c128* V, long sV, int vmod, int voff, const c128* P, long sP, int n, c128* tr,
+              int batch) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(trace_vp_kernel, dim3(batch), dim3(256), 0, s, V, sV, vmod, voff, P, sP,
+                     static_cast<long>(n) * n, tr);
   THCGW_HIP(hipGetLastError());
 }
 

@@ -71,9 +71,13 @@ inline BatchPtr bp(const c64* p, long stride, int mod = 1 << 30, int div = 1, in
 //   GEMM_TRI_KFROM_ROW the K sum of a tile starts at its first row (op(A) is upper triangular,
 //                      e.g. A = L^H with L lower); the skipped operand entries must be zero.
 //   GEMM_TRI_KFROM_COL the K sum of a tile starts at its first column (op(B) lower triangular).
+//   GEMM_TRI_KTO_ROW   the K sum of a tile ends after its last row (op(A) lower triangular).
+//   GEMM_TRI_KTO_COL   the K sum of a tile ends after its last column (op(B) upper triangular).
 constexpr int GEMM_TRI_LOWER = 1;
 constexpr int GEMM_TRI_KFROM_ROW = 2;
 constexpr int GEMM_TRI_KFROM_COL = 4;
+constexpr int GEMM_TRI_KTO_ROW = 512;
+constexpr int GEMM_TRI_KTO_COL = 1024;
 // cgemm only: accumulate the FP32 MFMA partial sums of every K chunk in FP64.
 constexpr int GEMM_FOLD64 = 8;
 // cgemm only: split-BF16 products (three bf16 parts per value, six part products, FP32
@@ -149,7 +153,18 @@ void trsm_right_lower_conj_f32(hipStream_t s, const c128* L, long sL, int ldl, c
 void trsm_right_lower_conj_f32(hipStream_t s, const c64* L, long sL, int ldl, c64* T, long sT, int ldt, int m,
                                int nb, int batch, int gemm_flags = 0);
 void zero_upper(hipStream_t s, c128* A, long sA, int n, int batch);
-void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch);
+void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch, const double* add = nullptr,
+                 int amod = 1, int aoff = 0);
+// A = L L^H (lower) and Z = L^-H (upper) by GEMM-only recursion (A and Z packed)
+void cholinv_batched(hipStream_t s, c128* A, long sA, c128* Z, long sZ, int n, int* info, int batch);
+// Z = L^-H for lower non-unit L (Z packed, ldz = n)
+void tri_inverse_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* Z, long sZ, int ldz, int n, int batch);
+// lower triangle of A[b] = Vi[(b + voff) % vmod] - P[b]
+void resolvent_lower(hipStream_t s, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A, long sA, int n,
+                     int batch);
+// tr[b] = Tr(V[(b + voff) % vmod] P[b]) for Hermitian V
+void trace_vp(hipStream_t s, const c128* V, long sV, int vmod, int voff, const c128* P, long sP, int n, c128* tr,
+              int batch);
 void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch);
 
 // misc.hip

@@ -179,6 +179,9 @@ void gemm_kernel(Args p) {
   if (p.tri & GEMM_TRI_KFROM_ROW) k_lo = m0;
   if (p.tri & GEMM_TRI_KFROM_COL) k_lo = n0 > k_lo ? n0 : k_lo;
   k_lo = (k_lo / BK) * BK;
+  int k_hi = p.K;
+  if ((p.tri & GEMM_TRI_KTO_ROW) && m0 + BM < k_hi) k_hi = m0 + BM;
+  if ((p.tri & GEMM_TRI_KTO_COL) && n0 + BN < k_hi) k_hi = n0 + BN;
 
   const TA* A = batch_ptr<TA>(p.A, b);
   const TB* B = batch_ptr<TB>(p.B, b);
@@ -208,7 +211,7 @@ void gemm_kernel(Args p) {
       }
   }
 
-  const int nk = (p.K + BK - 1) / BK;
+  const int nk = (k_hi + BK - 1) / BK;
   const int kt0 = k_lo / BK;
   RC ra[2 * IOA::P], rb[2 * IOB::P];
   if (kt0 < nk) {
@@ -458,6 +461,9 @@ __global__ __launch_bounds__(NT) void gemm_bf16x6_kernel(Args p) {
   if (p.tri & GEMM_TRI_KFROM_ROW) k_lo = m0;
   if (p.tri & GEMM_TRI_KFROM_COL) k_lo = n0 > k_lo ? n0 : k_lo;
   k_lo = (k_lo / XBK) * XBK;
+  int k_hi = p.K;
+  if ((p.tri & GEMM_TRI_KTO_ROW) && m0 + 64 < k_hi) k_hi = m0 + 64;
+  if ((p.tri & GEMM_TRI_KTO_COL) && n0 + 64 < k_hi) k_hi = n0 + 64;
 
   const TA* A = batch_ptr<TA>(p.A, b);
   const TB* B = batch_ptr<TB>(p.B, b);
@@ -481,7 +487,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16x6_kernel(Args p) {
       }
     }
 
-  const int nk = (p.K + XBK - 1) / XBK;
+  const int nk = (k_hi + XBK - 1) / XBK;
   const int kt0 = k_lo / XBK;
   float2 va[8], vb[8];
   if (kt0 < nk) {
@@ -704,7 +710,8 @@ Args make_args(int M, int N, int K, Cplx alpha, BatchPtr A, long lda, BatchPtr B
 
 // Modelled flops; triangular structure (square operands) halves the work per flag.
 double model_flops(int M, int N, int K, int batch, int tri) {
-  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL;  // the other flags do not change the work
+  // the other flags do not change the work
+  tri &= GEMM_TRI_LOWER | GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL | GEMM_TRI_KTO_ROW | GEMM_TRI_KTO_COL;
   double flops = 8.0 * double(M) * N * K * batch;
   if ((tri & GEMM_TRI_LOWER) && (tri & (GEMM_TRI_KFROM_ROW | GEMM_TRI_KFROM_COL))) flops /= 6.0;
   else if (tri) flops /= 2.0;

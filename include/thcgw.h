@@ -338,6 +338,14 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           default), 1 = Hermitian form V = L L^H, M = I - L^H Pi L = L_M L_M^H (two
  *           Cholesky factorisations, triangular products; valid for Hermitian positive
  *           definite V and Hermitian negative semidefinite Pi, with per-chunk fallback to LU);
+ *           2 = resolvent form W + V = (V^-1 - Pi)^-1: V_q^-1 and ln det V_q once per q, then
+ *           per matrix one Cholesky factorisation of A = V^-1 - Pi, the triangular inverse
+ *           L_A^-H and the product L_A^-H L_A^-1, all FP64 (n^3/2 complex multiply-adds
+ *           against 7n^3/3 for LU), with ln det(I - V Pi) = ln det V + ln det A.  Same
+ *           validity and fallback as 1; its rounding error grows with the condition number
+ *           of V (W to ~1e-9 relative for cond(V) ~ 1e7).  With flag 7 = 1 or 2 the Dyson
+ *           step stays in this FP64 form and only the projections and k-grid transforms run
+ *           in FP32;
  *   flag 8: FP32 GEMM tile shape (process-wide; products without FP64 chunk accumulation):
  *           0 = 64x64, 1 = 128x64, 2 = 128x128;
  *   flag 7: precision: 0 = FP64 throughout (default); 1 = mixed: the projections X G X^H and

@@ -304,6 +304,34 @@ def test_dyson_closed_form_and_lapack(orc):
             assert rel(W[n, q], A @ s.V[q]) < 1e-13
 
 
+def test_dyson_resolvent_identity(orc):
+    """For Hermitian positive definite V and Hermitian negative semidefinite Pi, the oracle's
+    W (LU of I - V Pi) equals (V^-1 - Pi)^-1 - V, and its ln det term equals
+    ln det V + ln det(V^-1 - Pi): the identities of the GPU resolvent form (flag 6 = 2).
+    A transposed product (I - Pi V)^-1 V would differ at O(1)."""
+    tab = orc.load_ir(1000.0, None, table="1e2.1e-15")
+    s = make_system((2, 1, 1), 4, 10, system_id=16)
+    rng = np.random.default_rng(9)
+    nwh = (tab["nw_b"] + 1) // 2
+    B = rng.standard_normal((nwh, 2, 10, 10)) + 1j * rng.standard_normal((nwh, 2, 10, 10))
+    Pi = -(B @ np.conj(np.swapaxes(B, -1, -2))) * 0.05
+    W = orc.dyson(Pi, s.V)
+    _, ew = orc.rpa_ec(tab, Pi, s.V, return_ew=True)
+    nwb = tab["nw_b"]
+    ref = np.zeros(nwb, complex)
+    for n in range(nwh):
+        for q in range(2):
+            Vi = np.linalg.inv(s.V[q])
+            assert rel(W[n, q], np.linalg.inv(Vi - Pi[n, q]) - s.V[q]) < 1e-12
+            assert rel(np.linalg.solve(np.eye(10) - Pi[n, q] @ s.V[q], s.V[q]) - s.V[q], W[n, q]) > 1e-3
+            ld = np.linalg.slogdet(s.V[q])[1] + np.linalg.slogdet(Vi - Pi[n, q])[1]
+            v = np.sum(np.conj(s.V[q]) * Pi[n, q]) + ld
+            ref[nwb // 2 + n] += v
+            if n:
+                ref[nwb // 2 - n] += v
+    assert np.max(np.abs(ew - ref)) < 1e-11
+
+
 # ----------------------------------------------------------------- Sigma
 @pytest.mark.parametrize("nt_table", ["1e2.1e-15", "1e5.1e-06"])
 def test_sigma_rspace_vs_kspace(orc, nt_table):
