@@ -188,6 +188,8 @@ def main():
     ap.add_argument("--krylov-max-iter", type=int, default=60, help="scgw-k64-mu1200: CG iterations before the direct fallback")
     ap.add_argument("--compare-modes", type=int, default=1,
                     help="scgw-k64-mu1200: also solve the final state's Dyson step with every preconditioner (0 to skip)")
+    ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
+                    help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
     ap.add_argument("--forms", default="plain,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed, "
                          "mixed_bf16, resolvent, mixed_resolvent, and their _trev / _c4 variants)")
@@ -405,9 +407,12 @@ def main():
 
 
 def run_batch(args):
-    """Batched form: the 24 systems of batch_systems() in mixed precision, one iteration of
-    all of them per step, replayed from one captured HIP graph; also timed one system after
-    another without the graph, and checked against a plain FP64 pass of the same batch."""
+    """Batched form: the 24 systems of batch_systems() in mixed precision (FP32 projections
+    and k-grid transforms; Dyson step by the FP64 resolvent form, or with --batch-dyson
+    hermitian by the FP32 Hermitian form), one iteration of all of them per step, replayed
+    from one captured HIP graph; also timed one system after another without the graph, and
+    against a plain FP64 pass of the same batch (LU Dyson, one system after another), which
+    is also the reference of the errors."""
     import torch
 
     from coqui_amd import IAFT, ThcBatch, ThcGW, make_system
@@ -438,7 +443,7 @@ def run_batch(args):
     lists = [[d[k] for d in sysd] for k in ("G", "X", "V", "Sig", "W")]
     b = ThcBatch([d["ctx"] for d in sysd], use_graph=True, tune=True)  # from here on the scratch is shared
 
-    # plain FP64 reference pass, one system after another
+    # plain FP64 reference pass, one system after another; then the same pass timed
     ref = []
     for d in sysd:
         d["ctx"].set_flag(6, 0)
@@ -449,10 +454,19 @@ def run_batch(args):
         Cd = torch.from_numpy(d["s"].C).cuda()
         E = d["ctx"].solve_qp(S, Cd, H, d["s"].E, mu=d["s"].mu)
         ref.append((S.clone(), W[0].clone(), ec, E))
-    for d in sysd:
-        d["ctx"].set_flag(7, 1)  # mixed precision
-
     stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for d in sysd:
+        d["ctx"].gw_iteration(d["G"], d["X"], d["V"], with_ec=True, hermitize=True, Sigma=d["Sig"], W_qt=d["W"])
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = ev0.elapsed_time(ev1)
+    log("bench: plain FP64 batch pass %.1f ms" % ms_plain)
+    for d in sysd:
+        d["ctx"].set_flag(6, 2 if args.batch_dyson == "resolvent" else 1)
+        d["ctx"].set_flag(7, 1)  # mixed precision
 
     def timed(steps, graph):
         b_set = b.lib.thcgw_batch_set(b.h, int(graph), 1)
@@ -501,13 +515,18 @@ def run_batch(args):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128 in/out, FP32 products, FP64 IR transforms and factorisations",
+        "dtype": ("c128 in/out, FP32 projections and k-grid transforms, FP64 IR transforms and resolvent Dyson"
+                  if args.batch_dyson == "resolvent" else
+                  "c128 in/out, FP32 products, FP64 IR transforms and factorisations"),
         "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
-        "config": {"workload": "batched mixed-precision G0W0 iteration of %d systems" % len(specs),
+        "config": {"workload": "batched mixed-precision G0W0 iteration of %d systems (%s Dyson)" % (
+                       len(specs), args.batch_dyson),
                    "systems": [{"kgrid": list(k), "n_orb": no, "n_mu": nmu} for k, no, nmu, _ in specs],
                    "ir_table": BATCH_TABLE, "n_tau": ia.nt, "beta": beta, "parallelism": "single GPU"},
         "ms_per_step_one_by_one": round(ms_eager, 3),
         "graph_speedup": round(ms_eager / ms_graph, 3),
+        "ms_plain_fp64_one_by_one": round(ms_plain, 3),
+        "speedup_vs_plain": round(ms_plain / ms_graph, 3),
         "max_errors_vs_plain_fp64": errs,
     }
     print(json.dumps(out), flush=True)

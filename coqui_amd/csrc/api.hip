@@ -440,8 +440,8 @@ bool dyson_chunk_inv(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Vi, c
   const int n = ctx->d.nmu, nq = ctx->vq();
   const long M = ctx->M;
   c128* P = PiW + static_cast<long>(b0) * M;
-  if (d_trace) thcgw::trace_vp(s, V, M, nq, b0, P, M, n, d_trace + b0, cb);
-  thcgw::resolvent_lower(s, Vi, M, nq, b0, P, Ab, M, n, cb);
+  c128* part = d_trace ? ctx->buf<c128>("dy_trp", static_cast<size_t>(cb) * thcgw::resolvent_prep_parts(n)) : nullptr;
+  thcgw::resolvent_prep(s, V, Vi, M, nq, b0, P, Ab, M, n, d_trace ? d_trace + b0 : nullptr, part, cb);
   thcgw::cholinv_batched(s, Ab, M, Z, M, n, info, cb);
   if (!ctx->deferred && any_nonzero(info, cb, s)) return false;
   if (d_ldet) thcgw::chol_logdet(s, Ab, M, n, d_ldet + b0, cb, ldv, nq, b0);

@@ -224,34 +224,59 @@ __global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL
     ldet[blockIdx.x] = 2.0 * red[0] + (add ? add[(static_cast<int>(blockIdx.x) + aoff) % amod] : 0.0);
 }
 
-// Lower triangle of A[b] = Vi[(b + voff) % vmod] - P[b] (the diagonal included); the strict
-// upper triangle of A is not written.
-__global__ __launch_bounds__(256) void resolvent_lower_kernel(const c128* Vi, long sV, int vmod, int voff,
-                                                              const c128* P, c128* A, long sA, int n) {
-  const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
-  const long nn = static_cast<long>(n) * n;
-  if (e >= nn) return;
-  const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
-  if (c > r) return;
+// One pass over the n x n elements of P[b], V[m] and Vi[m] (m = (b + voff) % vmod): the lower
+// triangle of A[b] = Vi[m] - P[b], and, when part is given, per workgroup the partial sum of
+// conj(V[e]) P[e] (Tr(V P) for Hermitian V) at part[b * gridDim.x + blockIdx.x].  Each
+// workgroup covers RP_ELEMS consecutive elements.
+constexpr int RP_ELEMS = 4096;
+__global__ __launch_bounds__(256) void resolvent_prep_kernel(const c128* V, const c128* Vi, long sV, int vmod,
+                                                             int voff, const c128* P, c128* A, long sA, int n,
+                                                             c128* part) {
+  __shared__ double red[2][256];
   const int b = blockIdx.y;
-  const c128 v = Vi[static_cast<long>((b + voff) % vmod) * sV + e];
-  const c128 p = P[static_cast<long>(b) * sA + e];
-  A[static_cast<long>(b) * sA + e] = c128{v.x - p.x, v.y - p.y};
+  const long nn = static_cast<long>(n) * n;
+  const long m = static_cast<long>((b + voff) % vmod) * sV;
+  const c128* Pb = P + static_cast<long>(b) * sA;
+  c128* Ab = A + static_cast<long>(b) * sA;
+  const long e0 = static_cast<long>(blockIdx.x) * RP_ELEMS;
+  const long e1 = e0 + RP_ELEMS < nn ? e0 + RP_ELEMS : nn;
+  double ar = 0.0, ai = 0.0;
+  for (long e = e0 + threadIdx.x; e < e1; e += 256) {
+    const c128 p = Pb[e];
+    if (part) {
+      const c128 v = V[m + e];
+      ar += v.x * p.x + v.y * p.y;
+      ai += v.x * p.y - v.y * p.x;
+    }
+    const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
+    if (c <= r) {
+      const c128 w = Vi[m + e];
+      Ab[e] = c128{w.x - p.x, w.y - p.y};
+    }
+  }
+  if (!part) return;
+  red[0][threadIdx.x] = ar;
+  red[1][threadIdx.x] = ai;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[static_cast<long>(b) * gridDim.x + blockIdx.x] = c128{red[0][0], red[1][0]};
 }
 
-// tr[b] = sum_e conj(V[e]) P[e] over the n x n elements of V[(b + voff) % vmod] and P[b]:
-// Tr(V P) for Hermitian V.  One workgroup per matrix.
-__global__ __launch_bounds__(256) void trace_vp_kernel(const c128* V, long sV, int vmod, int voff, const c128* P,
-                                                       long sP, long nn, c128* tr) {
+// tr[b] = sum of the np partial sums part[b * np ..]; one workgroup per matrix.
+__global__ __launch_bounds__(256) void sum_parts_kernel(const c128* part, int np, c128* tr) {
   __shared__ double red[2][256];
   const int b = blockIdx.x;
-  const c128* Vb = V + static_cast<long>((b + voff) % vmod) * sV;
-  const c128* Pb = P + static_cast<long>(b) * sP;
   double ar = 0.0, ai = 0.0;
-  for (long e = threadIdx.x; e < nn; e += 256) {
-    const c128 v = Vb[e], p = Pb[e];
-    ar += v.x * p.x + v.y * p.y;
-    ai += v.x * p.y - v.y * p.x;
+  for (int i = threadIdx.x; i < np; i += 256) {
+    const c128 v = part[static_cast<long>(b) * np + i];
+    ar += v.x;
+    ai += v.y;
   }
   red[0][threadIdx.x] = ar;
   red[1][threadIdx.x] = ai;
@@ -503,21 +528,23 @@ void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int
   THCGW_HIP(hipGetLastError());
 }
 
-void resolvent_lower(hipStream_t s, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A, long sA, int n,
-                     int batch) {
-  if (batch <= 0 || n <= 0) return;
-  const long nn = static_cast<long>(n) * n;
-  hipLaunchKernelGGL(resolvent_lower_kernel, dim3(static_cast<unsigned>((nn + 255) / 256), batch), dim3(256), 0, s, Vi,
-                     sV, vmod, voff, P, A, sA, n);
-  THCGW_HIP(hipGetLastError());
+int resolvent_prep_parts(int n) {
+  return static_cast<int>((static_cast<long>(n) * n + RP_ELEMS - 1) / RP_ELEMS);
 }
 
-void trace_vp(hipStream_t s, const c128* V, long sV, int vmod, int voff, const c128* P, long sP, int n, c128* tr,
-              int batch) {
-  if (batch <= 0) return;
-  hipLaunchKernelGGL(trace_vp_kernel, dim3(batch), dim3(256), 0, s, V, sV, vmod, voff, P, sP,
-                     static_cast<long>(n) * n, tr);
+void resolvent_prep(hipStream_t s, const c128* V, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A,
+                    long sA, int n, c128* tr, c128* part, int batch) {
+  if (batch <= 0 || n <= 0) return;
+  THCGW_REQUIRE(!tr || part, "resolvent_prep: the trace needs the partial-sum buffer");
+  THCGW_REQUIRE(batch <= 65535, "resolvent_prep: at most 65535 matrices per launch");
+  const int np = resolvent_prep_parts(n);
+  hipLaunchKernelGGL(resolvent_prep_kernel, dim3(np, batch), dim3(256), 0, s, V, Vi, sV, vmod, voff, P, A, sA, n,
+                     tr ? part : nullptr);
   THCGW_HIP(hipGetLastError());
+  if (tr) {
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(batch), dim3(256), 0, s, part, np, tr);
+    THCGW_HIP(hipGetLastError());
+  }
 }
 
 void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch) {

@@ -159,12 +159,11 @@ void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int
 void cholinv_batched(hipStream_t s, c128* A, long sA, c128* Z, long sZ, int n, int* info, int batch);
 // Z = L^-H for lower non-unit L (Z packed, ldz = n)
 void tri_inverse_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* Z, long sZ, int ldz, int n, int batch);
-// lower triangle of A[b] = Vi[(b + voff) % vmod] - P[b]
-void resolvent_lower(hipStream_t s, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A, long sA, int n,
-                     int batch);
-// tr[b] = Tr(V[(b + voff) % vmod] P[b]) for Hermitian V
-void trace_vp(hipStream_t s, const c128* V, long sV, int vmod, int voff, const c128* P, long sP, int n, c128* tr,
-              int batch);
+// lower triangle of A[b] = Vi[m] - P[b] and, with tr, tr[b] = Tr(V[m] P[b]) for Hermitian V
+// (m = (b + voff) % vmod) in one pass; part holds resolvent_prep_parts(n) values per matrix
+int resolvent_prep_parts(int n);
+void resolvent_prep(hipStream_t s, const c128* V, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A,
+                    long sA, int n, c128* tr, c128* part, int batch);
 void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch);
 
 // misc.hip
