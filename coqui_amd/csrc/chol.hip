@@ -364,14 +364,14 @@ void potrf_rec(hipStream_t s, c128* A, long sA, int lda, int j0, int n, int* inf
 
 // Cholesky factorisation with the inverse of the factor, every off-diagonal step a GEMM:
 // for the diagonal block [j0, j0 + n) split as n1 + n2,
-//   (L11, Z11) of A11;  U = L21^H = Z11^H A21^H -> A12 (the unused upper block of A);
-//   A22 -= U^H U (lower tiles);  (L22, Z22) of A22;
-//   Z21 <- (U Z22)^H = Z22^H U^H (scratch in the zero lower block of Z);
-//   Z12 = -Z11 (U Z22) = -Z11 Z21^H;  Z21 <- 0,
+//   (L11, Z11) of A11;  L21 = A21 L11^-H = A21 Z11 -> Z21 (scratch in the zero lower block of Z);
+//   A22 -= L21 L21^H (lower tiles);  (L22, Z22) of A22;
+//   T = L21^H Z22 -> A12 (the unused upper block of A);  Z12 = -Z11 T;  Z21 <- 0,
 // which is Z = L^-H = [[Z11, -Z11 L21^H Z22], [0, Z22]].  n^3/3 complex multiply-adds, as
 // for a factorisation plus a triangular inverse, without the triangular solves' narrow
-// updates.  On exit A holds L on and below the diagonal (L21 is not stored: L^H's blocks
-// are left in the upper triangle) and Z holds L^-H with zeros below the diagonal.
+// updates; the four products are N,N / N,C / C,N / N,N with triangular-K flags.  On exit A
+// holds L on and below the diagonal except L21 (the diagonal blocks are exact; only their
+// diagonal is read afterwards) and Z holds L^-H with zeros below the diagonal.
 void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int ldz, int j0, int n, int* info,
                  int batch) {
   if (n <= CB) {
@@ -390,14 +390,14 @@ void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int
   c128* Z21 = Z11 + static_cast<long>(n1) * ldz;
   c128* Z22 = Z21 + n1;
   const Cplx zero{0.0, 0.0};
-  zgemm(s, OP_C, OP_C, n1, n2, n1, kOne, bp(Z11, sZ), ldz, bp(A21, sA), lda, zero, A12, sA, lda, batch, zero,
-        GEMM_TRI_KTO_ROW);
-  zgemm(s, OP_C, OP_N, n2, n2, n1, kMinusOne, bp(A12, sA), lda, bp(A12, sA), lda, kOne, A22, sA, lda, batch, zero,
+  zgemm(s, OP_N, OP_N, n2, n1, n1, kOne, bp(A21, sA), lda, bp(Z11, sZ), ldz, zero, Z21, sZ, ldz, batch, zero,
+        GEMM_TRI_KTO_COL);
+  zgemm(s, OP_N, OP_C, n2, n2, n1, kMinusOne, bp(Z21, sZ), ldz, bp(Z21, sZ), ldz, kOne, A22, sA, lda, batch, zero,
         GEMM_TRI_LOWER);
   cholinv_rec(s, A, sA, lda, Z, sZ, ldz, j0 + n1, n2, info, batch);
-  zgemm(s, OP_C, OP_C, n2, n1, n2, kOne, bp(Z22, sZ), ldz, bp(A12, sA), lda, zero, Z21, sZ, ldz, batch, zero,
-        GEMM_TRI_KTO_ROW);
-  zgemm(s, OP_N, OP_C, n1, n2, n1, kMinusOne, bp(Z11, sZ), ldz, bp(Z21, sZ), ldz, zero, Z12, sZ, ldz, batch, zero,
+  zgemm(s, OP_C, OP_N, n1, n2, n2, kOne, bp(Z21, sZ), ldz, bp(Z22, sZ), ldz, zero, A12, sA, lda, batch, zero,
+        GEMM_TRI_KTO_COL);
+  zgemm(s, OP_N, OP_N, n1, n2, n1, kMinusOne, bp(Z11, sZ), ldz, bp(A12, sA), lda, zero, Z12, sZ, ldz, batch, zero,
         GEMM_TRI_KFROM_ROW);
   const long nb = static_cast<long>(n2) * n1;
   hipLaunchKernelGGL(zero_block_kernel, dim3(static_cast<unsigned>((nb + 255) / 256), batch), dim3(256), 0, s, Z21, sZ,
