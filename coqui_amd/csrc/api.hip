@@ -295,6 +295,10 @@ void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t 
 // Pi(iW_n, q, PQ) = sum_i T+[n][i] Pi(tau_i, q, PQ): one GEMM (nw_half x nt_half) x (nt_half x N_q N_mu^2).
 void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
   const long cols = ctx->M * ctx->nk;
+  if (thcgw::ir_transform_fits(ctx->nw_half, ctx->nt_half)) {
+    thcgw::ir_transform(s, ctx->d_Tplus, ctx->nw_half, ctx->nt_half, Pit, cols, 0, Piw, cols, 0, cols, 1);
+    return;
+  }
   THCGW_REQUIRE(cols < (1L << 31), "tau_to_w: N_q N_mu^2 too large for one GEMM");
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, ctx->nw_half, static_cast<int>(cols), ctx->nt_half, kOne,
                thcgw::bp(ctx->d_Tplus, 0), ctx->nt_half, thcgw::bp(Pit, 0), cols, kZero, Piw, 0, cols, 1);
@@ -304,6 +308,11 @@ void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
 // whose output rows land at stride N_mu^2 inside the q block of W_qtPQ.
 void run_w_to_tau(thcgw_ctx* ctx, const c128* Ww, c128* Wqt, hipStream_t s) {
   const long M = ctx->M, nk = ctx->nk;
+  if (thcgw::ir_transform_fits(ctx->nt_half, ctx->nw_half)) {
+    thcgw::ir_transform(s, ctx->d_Tminus, ctx->nt_half, ctx->nw_half, Ww, nk * M, M, Wqt, M, ctx->nt_half * M, M,
+                        ctx->nk);
+    return;
+  }
   THCGW_REQUIRE(M < (1L << 31), "w_to_tau: N_mu^2 too large for one GEMM");
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, ctx->nt_half, static_cast<int>(M), ctx->nw_half, kOne,
                thcgw::bp(ctx->d_Tminus, 0), ctx->nw_half, thcgw::bp(Ww, M), nk * M, kZero, Wqt, ctx->nt_half * M, M,

@@ -153,6 +153,8 @@ def test_qp_and_gm_nk8_nmu200(nk8_nmu200, orc, torch):
     ((4, 4, 4), "1e5.1e-06", 7, 24),      # N_k = 64 (lane-split DFT), odd nt = 61
     ((1, 1, 1), "1e2.1e-15", 5, 70),      # Gamma only
     ((2, 3, 1), "1e2.1e-15", 6, 20),      # non-cubic grid: generic transform kernel
+    ((2, 2, 2), "1e3.1e-15", 6, 24),      # 72 tau points: tau <-> iW MFMA kernel with 3 row blocks
+    ((1, 1, 1), "1e4.1e-15", 5, 20),      # 104 tau points: tau <-> iW by the GEMM (T has 53 rows)
 ])
 def test_iteration_shapes(orc, torch, kgrid, table, no, nmu):
     s, ia, tab, G = setup(orc, kgrid, no, nmu, table, sid=3)
