@@ -190,7 +190,7 @@ def main():
                     help="scgw-k64-mu1200: also solve the final state's Dyson step with every preconditioner (0 to skip)")
     ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
                     help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
-    ap.add_argument("--forms", default="plain,mixed",
+    ap.add_argument("--forms", default="plain,mixed_resolvent,mixed",
                     help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed, "
                          "mixed_bf16, resolvent, mixed_resolvent, and their _trev / _c4 variants)")
     args = ap.parse_args()
@@ -259,7 +259,20 @@ def main():
         else:
             ctx.set_qsym()
 
+    first_form = [True]
+
     def run_form(form, steps, warmup):
+        # every form after the first gets a fresh context, so that no form runs with the
+        # workspace of the forms before it held on the device (the Dyson chunk size follows
+        # the free memory)
+        nonlocal ctx
+        if not first_form[0]:
+            ctx.close()
+            torch.cuda.synchronize()
+            ctx = ThcGW(kgrid, no, nmu, ia)
+            ctx.set_flag(5, args.gemm_tile)
+            ctx.set_flag(8, args.gemm_tile32)
+        first_form[0] = False
         set_form(form)
         ec = None
         for i in range(warmup):

@@ -71,8 +71,12 @@ class context {
   context(const context&) = delete;
   context& operator=(const context&) = delete;
   thcgw_ctx* get() const { return ctx_; }
-  // thcgw_set_flag (e.g. 6: Hermitian Dyson form, 7: precision, 9: time-reversal input)
+  // thcgw_set_flag (e.g. 6: Dyson form, 7: precision, 9: time-reversal input)
   void set_flag(int flag, long value) { check(ctx_, thcgw_set_flag(ctx_, flag, value)); }
+  // flag 6: LU with partial pivoting (the reference's algorithm), the two-Cholesky
+  // Hermitian form, or the resolvent form (V^-1 - Pi)^-1
+  enum dyson_form { dyson_lu = 0, dyson_hermitian = 1, dyson_resolvent = 2 };
+  void set_dyson_form(dyson_form f) { set_flag(6, f); }
   const thcgw_dims& dims() const { return dims_; }
   int nk() const { return dims_.kgrid[0] * dims_.kgrid[1] * dims_.kgrid[2]; }
   int nt_half() const { return (dims_.nt + 1) / 2; }

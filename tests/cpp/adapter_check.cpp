@@ -148,6 +148,20 @@ int main(int argc, char** argv) {
       save(dir + "/eps_head.bin", mb2.eps_inv_head);
       for (cplx* p : {dcb, dch}) (void)hipFree(p);
     }
+    // the plain sequence again with the resolvent Dyson form (flag 6 = 2)
+    ctx.set_dyson_form(thcgw_cxx::context::dyson_resolvent);
+    thcgw_cxx::MBState mb3{dG, dS, dW};
+    thcgw_cxx::scr_coulomb_t scr3(ctx);
+    thcgw_cxx::gw_t gw3(ctx);
+    scr3.update_w(mb3, thc, 0);
+    gw3.evaluate(mb3, thc);
+    gw3.hermitize(mb3);
+    const double ec3 = gw3.rpa_energy(dG, thc);
+    ok(hipDeviceSynchronize(), "sync");
+    save(dir + "/Sigma_resolvent.bin", host_from(dS, nG));
+    save(dir + "/W_resolvent.bin", host_from(dW, nW));
+    save(dir + "/ec_resolvent.bin", std::vector<double>{ec3});
+    ctx.set_dyson_form(thcgw_cxx::context::dyson_lu);
     for (cplx* p : {dG, dX, dV, dS, dW, dS2, dW2}) (void)hipFree(p);
     return (es <= 1e-12 && ew <= 1e-12 && ee <= 1e-12) ? 0 : 1;
   } catch (const std::exception& e) {

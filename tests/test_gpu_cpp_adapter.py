@@ -55,3 +55,11 @@ def test_cpp_adapter_matches_oracle(orc, tmp_path):
     want = orc.sigma_div_correction(ref["Sigma"], G, s.X, ch, 2.3, e0, 2)
     Sg = np.fromfile(d + "/Sigma_gygi.bin", np.complex128).reshape(want.shape)
     assert np.max(np.abs(Sg - want)) / np.max(np.abs(want)) < 1e-10
+    # set_dyson_form(dyson_resolvent): the resolvent form, to 1e-10 + cond(V) eps
+    tol = 1e-10 + max(np.linalg.cond(v) for v in s.V) * np.finfo(float).eps
+    Sr = np.fromfile(d + "/Sigma_resolvent.bin", np.complex128).reshape(ref["Sigma"].shape)
+    Wr = np.fromfile(d + "/W_resolvent.bin", np.complex128).reshape(ref["W_qt"].shape)
+    er = np.fromfile(d + "/ec_resolvent.bin", np.float64)[0]
+    assert np.max(np.abs(Sr - ref["Sigma"])) / np.max(np.abs(ref["Sigma"])) < tol
+    assert np.max(np.abs(Wr - ref["W_qt"])) / np.max(np.abs(ref["W_qt"])) < tol
+    assert abs(er - ref["Ec"]) < 1e-10 * abs(ref["Ec"])
