@@ -379,7 +379,9 @@ void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(n, CB), n2 = n - n1;
+  // above order 128 the split is a multiple of the 64-wide GEMM tile, so that no product of
+  // the recursion has a half-filled tile row or column at the split
+  const int n1 = split_point(n, n > 128 ? 64 : CB), n2 = n - n1;
   cholinv_rec(s, A, sA, lda, Z, sZ, ldz, j0, n1, info, batch);
   c128* A11 = A + static_cast<long>(j0) * lda + j0;
   c128* A12 = A11 + n1;
