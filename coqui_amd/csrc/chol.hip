@@ -345,13 +345,17 @@ int split_point(int n, int unit) {
   return h < unit ? unit : h;
 }
 
+// Above order 128 the recursions split at multiples of the 64-wide GEMM tile, so that no
+// product of the recursion has a half-filled tile row or column at the split.
+int tile_split(int n, int unit) { return split_point(n, n > 128 ? 64 : unit); }
+
 void potrf_rec(hipStream_t s, c128* A, long sA, int lda, int j0, int n, int* info, int batch) {
   if (n <= CB) {
     hipLaunchKernelGGL(potrf_leaf_kernel, dim3(batch), dim3(CB * CB), 0, s, A, sA, lda, j0, n, info);
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(n, CB), n2 = n - n1;
+  const int n1 = tile_split(n, CB), n2 = n - n1;
   potrf_rec(s, A, sA, lda, j0, n1, info, batch);
   c128* A11 = A + static_cast<long>(j0) * lda + j0;
   c128* A21 = A11 + static_cast<long>(n1) * lda;
@@ -379,9 +383,7 @@ void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int
     THCGW_HIP(hipGetLastError());
     return;
   }
-  // above order 128 the split is a multiple of the 64-wide GEMM tile, so that no product of
-  // the recursion has a half-filled tile row or column at the split
-  const int n1 = split_point(n, n > 128 ? 64 : CB), n2 = n - n1;
+  const int n1 = tile_split(n, CB), n2 = n - n1;
   cholinv_rec(s, A, sA, lda, Z, sZ, ldz, j0, n1, info, batch);
   c128* A11 = A + static_cast<long>(j0) * lda + j0;
   c128* A12 = A11 + n1;
@@ -430,7 +432,7 @@ void trsm_right_lower_conj(hipStream_t s, const c128* L, long sL, int ldl, c128*
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(nb, CB), n2 = nb - n1;
+  const int n1 = tile_split(nb, CB), n2 = nb - n1;
   trsm_right_lower_conj(s, L, sL, ldl, T, sT, ldt, m, n1, batch);
   // T2 -= X1 (L21)^H
   zgemm(s, OP_N, OP_C, m, n2, n1, kMinusOne, bp(T, sT), ldt, bp(L + static_cast<long>(n1) * ldl, sL), ldl, kOne,
@@ -451,7 +453,7 @@ void trsm_rlc_utri(hipStream_t s, const c128* L, long sL, int ldl, c128* T, long
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(nb, CB), n2 = nb - n1;
+  const int n1 = tile_split(nb, CB), n2 = nb - n1;
   trsm_rlc_utri(s, L, sL, ldl, T, sT, ldt, c0, n1, batch);
   // T2 -= X1 (L21)^H over the rows where X1 can be nonzero
   zgemm(s, OP_N, OP_C, c0 + n1, n2, n1, kMinusOne, bp(T, sT), ldt, bp(L + static_cast<long>(n1) * ldl, sL), ldl, kOne,

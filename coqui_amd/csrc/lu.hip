@@ -274,6 +274,10 @@ int split_point(int n, int unit) {
   return h < unit ? unit : h;
 }
 
+// Above order 128 the recursions split at multiples of the 64-wide GEMM tile, so that no
+// update product has a half-filled tile row or column at the split.
+int tile_split(int n, int unit) { return split_point(n, n > 128 ? 64 : unit); }
+
 }  // namespace
 
 // ---- triangular solves (batched, strides between matrices) ----------------------
@@ -287,7 +291,7 @@ void trsm_right_upper_unit(hipStream_t s, const c128* U, long sU, int ldu, c128*
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(nb, LB), n2 = nb - n1;
+  const int n1 = tile_split(nb, LB), n2 = nb - n1;
   trsm_right_upper_unit(s, U, sU, ldu, T, sT, ldt, m, n1, batch);
   // T2 -= X1 U12
   zgemm(s, OP_N, OP_N, m, n2, n1, kMinusOne, bp(T, sT), ldt, bp(U + n1, sU), ldu, kOne, T + n1, sT, ldt, batch);
@@ -304,7 +308,7 @@ void trsm_left_lower(hipStream_t s, const c128* L, long sL, int ldl, c128* B, lo
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(n, LB), n2 = n - n1;
+  const int n1 = tile_split(n, LB), n2 = n - n1;
   trsm_left_lower(s, L, sL, ldl, B, sB, ldb, n1, ncol, batch);
   zgemm(s, OP_N, OP_N, n2, ncol, n1, kMinusOne, bp(L + static_cast<long>(n1) * ldl, sL), ldl, bp(B, sB), ldb, kOne,
         B + static_cast<long>(n1) * ldb, sB, ldb, batch);
@@ -322,7 +326,7 @@ void trsm_left_upper_unit(hipStream_t s, const c128* U, long sU, int ldu, c128* 
     THCGW_HIP(hipGetLastError());
     return;
   }
-  const int n1 = split_point(n, LB), n2 = n - n1;
+  const int n1 = tile_split(n, LB), n2 = n - n1;
   trsm_left_upper_unit(s, U + static_cast<long>(n1) * ldu + n1, sU, ldu, B + static_cast<long>(n1) * ldb, sB, ldb, n2,
                        ncol, batch);
   zgemm(s, OP_N, OP_N, n1, ncol, n2, kMinusOne, bp(U + n1, sU), ldu, bp(B + static_cast<long>(n1) * ldb, sB), ldb,
@@ -358,7 +362,7 @@ void getrf_rec(hipStream_t s, c128* A, long sA, int lda, int n, int j0, int w, i
     panel(s, A, sA, lda, n, j0, w, ipiv, info, batch);
     return;
   }
-  const int w1 = split_point(w, pw), w2 = w - w1;
+  const int w1 = tile_split(w, pw), w2 = w - w1;
   getrf_rec(s, A, sA, lda, n, j0, w1, ipiv, info, batch);
   laswp(s, A, sA, lda, n, j0 + w1, j0 + w, j0, j0 + w1, ipiv, batch);
   c128* diag = A + static_cast<long>(j0) * lda + j0;
