@@ -12,7 +12,8 @@ Forms (--forms, the first is the headline value):
              transforms, Cholesky factorisations and ln det;
   mixed_bf16 as mixed, with the FP32 products done as split-BF16 MFMA products;
   resolvent  FP64 throughout, Dyson by the resolvent form W + V = (V^-1 - Pi)^-1 (one
-             Cholesky factorisation, triangular inverse and triangular product per matrix);
+             Cholesky factorisation, triangular inverse and triangular product per matrix),
+             and the projections of the Hermitian G on their lower triangle (flag 10);
   mixed_resolvent  FP32 projections and k-grid transforms, FP64 resolvent Dyson step;
   *_trev     the form with the Dyson step on the time-reversal irreducible q only (needs
              --trev, which draws the system time-reversal symmetric);
@@ -253,6 +254,7 @@ def main():
     def set_form(form):
         ctx.set_flag(6, FORMS[form][0])
         ctx.set_flag(7, FORMS[form][1])
+        ctx.set_flag(10, int(FORMS[form][0] == 2))  # the resolvent forms also project G as Hermitian
         ctx.set_flag(9, int(FORMS[form][2] == 1))
         if FORMS[form][2] == 2:
             ctx.set_qsym(*s.qsym)

@@ -62,6 +62,7 @@ struct thcgw_ctx {
   bool force_fallback = false;
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
+  int herm_g = 0;       // flag 10: G(tau, k) Hermitian, its projections formed on the lower triangle
   int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky),
                         // 2 = resolvent (V^-1 - Pi)^-1
   int precision = 0;    // 0 = FP64 throughout, 1 = mixed (FP32 products, FP64 factorisations and IR),
@@ -188,14 +189,17 @@ size_t free_device_bytes() {
 }
 
 // O_out[b,k] = X_k O_in[b,k] X_k^H for nbk = nb*nk matrices (thc_solver_comm.hpp:422,427).
-void project_p2a(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, c128* Oout, long nbk, c128* Abuf) {
+// herm: O_in is Hermitian, so is O_out; its lower tiles are formed and mirrored.
+void project_p2a(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, c128* Oout, long nbk, c128* Abuf,
+                 bool herm = false) {
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(X, sX, nk), no,
                thcgw::bp(Oin, static_cast<long>(no) * no), no, kZero, Abuf, sX, no, static_cast<int>(nbk));
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_C, nmu, nmu, no, kOne, thcgw::bp(Abuf, sX), no, thcgw::bp(X, sX, nk), no,
-               kZero, Oout, ctx->M, nmu, static_cast<int>(nbk));
+               kZero, Oout, ctx->M, nmu, static_cast<int>(nbk), kZero,
+               herm ? (thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB) : 0);
 }
 
 // O_out[b,k] = X_k^H O_in[b,k] X_k (thc_solver_comm.hpp:560,563).
@@ -217,8 +221,10 @@ const c64* x_c64(thcgw_ctx* ctx, const c128* X, hipStream_t s) {
   return X64;
 }
 
-// O_out[b,k] = X_k O_in[b,k] X_k^H with O_in complex<double>, result complex<float>.
-void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* Oin, c64* Oout, long nbk, c64* Abuf) {
+// O_out[b,k] = X_k O_in[b,k] X_k^H with O_in complex<double>, result complex<float>; herm as
+// for project_p2a.
+void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* Oin, c64* Oout, long nbk, c64* Abuf,
+                     bool herm = false) {
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
@@ -227,7 +233,7 @@ void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* 
                static_cast<int>(nbk), kZero, ctx->pf());
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, nmu, nmu, no, kOne, thcgw::bp(Abuf, sX), DT_C64, no,
                thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C64, ctx->M, nmu, static_cast<int>(nbk), kZero,
-               ctx->pf());
+               ctx->pf() | (herm ? (thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB) : 0));
 }
 
 // O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>.
@@ -251,6 +257,28 @@ int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
   return std::max(1, std::min(c, ctx->nt_half));
 }
 
+std::string sci(double v);
+
+// Flag 10 refuses G(tau, k) that is not Hermitian: max |G - G^H| over all (tau, k) against
+// 1e-12 of max |G| (G built from a Hermitian H(k) is Hermitian to rounding).
+void check_herm_g(thcgw_ctx* ctx, const c128* G, hipStream_t s) {
+  if (!ctx->herm_g || ctx->deferred) return;
+  const int nbx = 256;
+  double* d = ctx->buf<double>("hg_chk", static_cast<size_t>(2) * nbx);
+  thcgw::herm_diff(s, G, ctx->d.no, static_cast<long>(ctx->d.nt) * ctx->nk, nbx, d);
+  std::vector<double> h(static_cast<size_t>(2) * nbx);
+  THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  double diff = 0.0, mag = 0.0;
+  for (size_t i = 0; i < h.size(); i += 2) {
+    diff = std::max(diff, h[i]);
+    mag = std::max(mag, h[i + 1]);
+  }
+  if (!(diff <= 1e-12 * mag))
+    throw thcgw::Error(THCGW_ERR_ARG, "flag 10 (Hermitian G): G is not Hermitian, max |G - G^H| = " + sci(diff) +
+                                          " against max |G| = " + sci(mag));
+}
+
 void run_pi_f32(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
@@ -263,8 +291,8 @@ void run_pi_f32(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStrea
   const double scale = -2.0 / (double(nk) * nk);
   for (int i0 = 0; i0 < nth; i0 += P) {
     const int c = std::min(P, nth - i0);
-    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
-    project_p2a_f32(ctx, s, X64, G + (nt - i0 - c) * gs, GN, static_cast<long>(c) * nk, Ab);
+    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    project_p2a_f32(ctx, s, X64, G + (nt - i0 - c) * gs, GN, static_cast<long>(c) * nk, Ab, ctx->herm_g);
     for (int j = 0; j < c; ++j)
       thcgw::pi_kfft_f32(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
                          GN + static_cast<long>(c - 1 - j) * nk * M, Pi + static_cast<long>(i0 + j) * nk * M, M, scale);
@@ -272,6 +300,7 @@ void run_pi_f32(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStrea
 }
 
 void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
+  check_herm_g(ctx, G, s);
   if (ctx->precision != 0) return run_pi_f32(ctx, G, X, Pi, s);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
@@ -283,9 +312,9 @@ void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t 
   const double scale = -2.0 / (double(nk) * nk);  // sp = -2 for ns = 1 (rpa_pi.icc:46)
   for (int i0 = 0; i0 < nth; i0 += P) {
     const int c = std::min(P, nth - i0);
-    project_p2a(ctx, s, X, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
+    project_p2a(ctx, s, X, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g);
     // G(beta - tau_i) = G[nt-1-i] for i in [i0, i0+c): tau slots [nt-i0-c, nt-i0), reversed
-    project_p2a(ctx, s, X, G + (nt - i0 - c) * gs, GN, static_cast<long>(c) * nk, Ab);
+    project_p2a(ctx, s, X, G + (nt - i0 - c) * gs, GN, static_cast<long>(c) * nk, Ab, ctx->herm_g);
     for (int j = 0; j < c; ++j)
       thcgw::pi_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
                      GN + static_cast<long>(c - 1 - j) * nk * M, Pi + static_cast<long>(i0 + j) * nk * M, M, scale);
@@ -756,8 +785,9 @@ void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X
   for (int i0 = 0; i0 < nth; i0 += P) {
     const int c = std::min(P, nth - i0);
     const int c2 = std::max(0, std::min(c, nt / 2 - i0));
-    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
-    if (c2 > 0) project_p2a_f32(ctx, s, X64, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab);
+    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    if (c2 > 0)
+      project_p2a_f32(ctx, s, X64, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab, ctx->herm_g);
     for (int j = 0; j < c; ++j) {
       const int i = i0 + j;
       const bool second = j < c2;
@@ -774,6 +804,7 @@ void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X
 }
 
 void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm, hipStream_t s) {
+  check_herm_g(ctx, G, s);
   if (ctx->precision != 0) return run_sigma_f32(ctx, G, Wqt, X, Sig, herm, s);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const long M = ctx->M;
@@ -789,8 +820,8 @@ void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c1
     const int c = std::min(P, nth - i0);
     // second pass only for i < nt/2 (thc_gw.icc:53-62 writes Sigma(beta - tau_i) at nt-1-i)
     const int c2 = std::max(0, std::min(c, nt / 2 - i0));
-    project_p2a(ctx, s, X, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab);
-    if (c2 > 0) project_p2a(ctx, s, X, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab);
+    project_p2a(ctx, s, X, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    if (c2 > 0) project_p2a(ctx, s, X, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab, ctx->herm_g);
     for (int j = 0; j < c; ++j) {
       const int i = i0 + j;
       const bool second = j < c2;
@@ -1085,6 +1116,9 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     } else if (flag == 7) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 7 (precision) takes 0 (FP64), 1 (mixed) or 2 (mixed, split BF16)");
       ctx->precision = static_cast<int>(value);
+    } else if (flag == 10) {
+      THCGW_REQUIRE(value == 0 || value == 1, "flag 10 (Hermitian G) takes 0 or 1");
+      ctx->herm_g = static_cast<int>(value);
     } else if (flag == 9) {
       THCGW_REQUIRE(value == 0 || value == 1, "flag 9 (time reversal) takes 0 or 1");
       ctx->trev = static_cast<int>(value);

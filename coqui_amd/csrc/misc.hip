@@ -315,6 +315,37 @@ __global__ void scatter_ir_terms_kernel(const c128* __restrict__ trs, const doub
 }
 
 // out[2 (q nbx + bx)] = max |A[mq(q)] - conj A[q]|, out[... + 1] = max |A[q]| over the block's slice
+// out[2 b'] = max |A_ij - conj(A_ji)|, out[2 b' + 1] = max |A_ij| over the elements that block
+// b' visits (grid stride over all nb matrices of order n).
+__global__ __launch_bounds__(T256) void herm_diff_kernel(const c128* __restrict__ A, int n, long nb,
+                                                         double* __restrict__ out) {
+  __shared__ double r0[T256], r1[T256];
+  const long nn = static_cast<long>(n) * n, total = nn * nb;
+  double d = 0.0, m = 0.0;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < total;
+       e += static_cast<long>(gridDim.x) * T256) {
+    const long b = e / nn;
+    const int ij = static_cast<int>(e - b * nn), i = ij / n, j = ij % n;
+    const c128 u = A[e], v = A[b * nn + static_cast<long>(j) * n + i];
+    d = fmax(d, fmax(fabs(u.x - v.x), fabs(u.y + v.y)));
+    m = fmax(m, fmax(fabs(u.x), fabs(u.y)));
+  }
+  r0[threadIdx.x] = d;
+  r1[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = T256 / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      r0[threadIdx.x] = fmax(r0[threadIdx.x], r0[threadIdx.x + w]);
+      r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2L * blockIdx.x] = r0[0];
+    out[2L * blockIdx.x + 1] = r1[0];
+  }
+}
+
 __global__ __launch_bounds__(T256) void trev_diff_kernel(const c128* __restrict__ A, long M, const int* __restrict__ mq,
                                                          double* __restrict__ out) {
   __shared__ double r0[T256], r1[T256];
@@ -635,6 +666,11 @@ void qsym_fill_terms(hipStream_t s, int nq, int nw, const int* src, const int* c
 void qsym_diff(hipStream_t s, const c128* A, int n, int nq, const int* src, const int* cj, const int* perm,
                const c128* ph, int nbx, double* out) {
   hipLaunchKernelGGL(qsym_diff_kernel, dim3(nbx, nq), dim3(T256), 0, s, A, n, src, cj, perm, ph, out);
+  THCGW_HIP(hipGetLastError());
+}
+
+void herm_diff(hipStream_t s, const c128* A, int n, long nb, int nbx, double* out) {
+  hipLaunchKernelGGL(herm_diff_kernel, dim3(nbx), dim3(T256), 0, s, A, n, nb, out);
   THCGW_HIP(hipGetLastError());
 }
 

@@ -87,8 +87,8 @@ constexpr int GEMM_SPLIT_BF16 = 16;
 // (0 = the process default, see gemm_set_tile_mode32).
 constexpr int GEMM_TILE32_SHIFT = 6;
 // With GEMM_TRI_LOWER on a square product X = alpha op(A) op(B) that is Hermitian: write
-// C = X - H on both triangles from the lower tiles (H complex<double>, same layout as C;
-// no beta, gamma or split BF16).
+// C = X - H on both triangles from the lower tiles (H complex<double>, same layout as C, or
+// none for C = X; no beta, gamma or split BF16).
 constexpr int GEMM_HERM_SUB = 256;
 
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I   (row-major, M x N)
@@ -204,6 +204,8 @@ void scatter_ir(hipStream_t s, const c128* S, long M, int n0, int nq, const int*
 void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const int* infos, int n0, int nq,
                       const int* qir, const int* mq, int nqi, int cnt, c128* tr, double* ld, int* info);
 void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out);
+// per block b' < nbx: max |A_ij - conj(A_ji)| and max |A_ij| over nb matrices of order n
+void herm_diff(hipStream_t s, const c128* A, int n, long nb, int nbx, double* out);
 // q-point symmetry map: W[w][q] = ph_q conj(ph_q)^T o W^[w][src q](perm_q, perm_q) for the q with
 // src q != q (W^ = conj W where cj[q]); the Ec terms and flags likewise; and the check
 // max |A[q] - map(A[src q])| per (q, block) with max |A[q]|.

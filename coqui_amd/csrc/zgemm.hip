@@ -304,12 +304,13 @@ void gemm_kernel(Args p) {
           using S = decltype(TC{}.x);
           if (p.tri & GEMM_HERM_SUB) {
             // lower element (r >= c) of a Hermitian product: C = X - H on both triangles
+            // (C = X when no H is given)
             if (r >= c) {
-              const c128* Hb = batch_ptr<c128>(p.H, b);
-              const c128 h = Hb[static_cast<long>(r) * p.ldc + c];
+              const c128* Hb = p.H.p ? batch_ptr<c128>(p.H, b) : nullptr;
+              const c128 h = Hb ? Hb[static_cast<long>(r) * p.ldc + c] : c128{0.0, 0.0};
               *dst = TC{static_cast<S>(outr - h.x), static_cast<S>(outi - h.y)};
               if (r > c) {
-                const c128 ht = Hb[static_cast<long>(c) * p.ldc + r];
+                const c128 ht = Hb ? Hb[static_cast<long>(c) * p.ldc + r] : c128{0.0, 0.0};
                 C[static_cast<long>(c) * p.ldc + r] = TC{static_cast<S>(outr - ht.x), static_cast<S>(-outi - ht.y)};
               }
             }
@@ -681,7 +682,7 @@ namespace {
 
 Args make_args(int M, int N, int K, Cplx alpha, BatchPtr A, long lda, BatchPtr B, long ldb, Cplx beta, void* C,
                long strideC, long ldc, int batch, Cplx gamma, int tri, BatchPtr hsub) {
-  THCGW_REQUIRE(!(tri & GEMM_HERM_SUB) || (hsub.p && M == N && beta.re == 0.0 && beta.im == 0.0 &&
+  THCGW_REQUIRE(!(tri & GEMM_HERM_SUB) || (M == N && beta.re == 0.0 && beta.im == 0.0 &&
                                            gamma.re == 0.0 && gamma.im == 0.0 && (tri & GEMM_TRI_LOWER) &&
                                            !(tri & GEMM_SPLIT_BF16)),
                 "gemm: GEMM_HERM_SUB needs a square lower product without beta, gamma or split BF16");
@@ -785,7 +786,8 @@ void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
            int tri, BatchPtr hsub) {
   THCGW_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch >= 0, "cgemm: negative size");
   if (M == 0 || N == 0 || batch == 0) return;
-  THCGW_REQUIRE(!(tri & GEMM_HERM_SUB) || tc == DT_C128, "cgemm: GEMM_HERM_SUB writes complex<double>");
+  THCGW_REQUIRE(!(tri & GEMM_HERM_SUB) || tc == DT_C128 || !hsub.p,
+                "cgemm: GEMM_HERM_SUB with a subtracted matrix writes complex<double>");
   const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri, hsub);
   const int cls = ((tri & GEMM_SPLIT_BF16) ? 13 : 9) + (opA == OP_C ? 2 : 0) + (opB == OP_C ? 1 : 0);
   timed(s, model_flops(M, N, K, batch, tri), cls, [&] {

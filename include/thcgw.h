@@ -346,6 +346,10 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           of V (W to ~1e-9 relative for cond(V) ~ 1e7).  With flag 7 = 1 or 2 the Dyson
  *           step stays in this FP64 form and only the projections and k-grid transforms run
  *           in FP32;
+ *   flag 10: G(tau, k) is Hermitian (as for any G built from a Hermitian H(k) and Fermi
+ *           occupations): the projections X G X^H for chi0 and Sigma form their lower tiles
+ *           and mirror them, half the work of that product.  thcgw_pi, thcgw_sigma and
+ *           thcgw_gw_iteration refuse G with max |G - G^H| > 1e-12 max |G| (THCGW_ERR_ARG);
  *   flag 8: FP32 GEMM tile shape (process-wide; products without FP64 chunk accumulation):
  *           0 = 64x64, 1 = 128x64, 2 = 128x128;
  *   flag 7: precision: 0 = FP64 throughout (default); 1 = mixed: the projections X G X^H and

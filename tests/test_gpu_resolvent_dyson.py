@@ -188,3 +188,31 @@ def test_flag6_range():
 
     with pytest.raises(ThcgwError):
         ctx.set_flag(6, 3)
+
+
+def test_hermitian_g_projections(orc, torch):
+    """Flag 10: the projections of a Hermitian G form their lower tiles and mirror them.
+    The iteration equals the full-product one to rounding (1e-12), in FP64 and with the
+    FP32 projections, and G that is not Hermitian is refused."""
+    from coqui_amd import ThcgwError
+
+    s, ia, tab, G = system(orc, (2, 2, 2), 10, 72, "1e2.1e-15", sid=6)
+    ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
+    for prec in (0, 1):
+        ctx.set_flag(7, prec)
+        ctx.set_flag(10, 0)
+        S0, W0, e0 = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+        S0, W0 = to_np(S0), to_np(W0)
+        ctx.set_flag(10, 1)
+        S1, W1, e1 = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+        S1, W1 = to_np(S1), to_np(W1)
+        print("prec %d: W %.1e Sigma %.1e Ec %.1e" % (prec, relmax(W1, W0), relmax(S1, S0), abs(e1 - e0)))
+        tol = 1e-12 if prec == 0 else 1e-5
+        assert relmax(W1, W0) < tol and relmax(S1, S0) < tol and abs(e1 - e0) < tol * abs(e0)
+    ctx.set_flag(7, 0)
+    Gbad = G.copy()
+    Gbad[3, 0, 1, 0, 1] += 1e-3  # one element breaks G = G^H
+    with pytest.raises(ThcgwError):
+        ctx.gw_iteration(Gbad, s.X, s.V, with_ec=True)
+    ctx.set_flag(10, 0)
+    ctx.gw_iteration(Gbad, s.X, s.V, with_ec=True)  # accepted without the flag
