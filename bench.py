@@ -481,6 +481,7 @@ def run_batch(args):
     log("bench: plain FP64 batch pass %.1f ms" % ms_plain)
     for d in sysd:
         d["ctx"].set_flag(6, 2 if args.batch_dyson == "resolvent" else 1)
+        d["ctx"].set_flag(10, int(args.batch_dyson == "resolvent"))  # Hermitian G projections
         d["ctx"].set_flag(7, 1)  # mixed precision
 
     def timed(steps, graph):

@@ -38,6 +38,7 @@ def build(orc, torch, precision, trev=False, dyson=0):
         s = make_system(kgrid, no, nmu, system_id=40 + sid, trev=trev)
         ctx = ThcGW(kgrid, no, nmu, ia)
         ctx.set_flag(6, dyson)
+        ctx.set_flag(10, int(dyson == 2))  # the resolvent batch also projects G as Hermitian
         ctx.set_flag(7, precision)
         ctx.set_flag(9, int(trev))
         G = torch.from_numpy(orc.update_G(tab, s.C, s.E)).cuda()
