@@ -224,10 +224,11 @@ __global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL
     ldet[blockIdx.x] = 2.0 * red[0] + (add ? add[(static_cast<int>(blockIdx.x) + aoff) % amod] : 0.0);
 }
 
-// One pass over the n x n elements of P[b], V[m] and Vi[m] (m = (b + voff) % vmod): the lower
-// triangle of A[b] = Vi[m] - P[b], and, when part is given, per workgroup the partial sum of
-// conj(V[e]) P[e] (Tr(V P) for Hermitian V) at part[b * gridDim.x + blockIdx.x].  Each
-// workgroup covers RP_ELEMS consecutive elements.
+// One pass over the lower triangles of P[b], V[m] and Vi[m] (m = (b + voff) % vmod): the
+// lower triangle of A[b] = Vi[m] - P[b], and, when part is given, per workgroup the partial
+// sum of Tr(V P) for Hermitian V and P, sum_i V_ii P_ii + 2 sum_{i>j} Re(conj(V_ij) P_ij)
+// (real), at part[b * gridDim.x + blockIdx.x].  Each workgroup covers RP_ELEMS consecutive
+// elements; the form reads only these lower triangles.
 constexpr int RP_ELEMS = 4096;
 __global__ __launch_bounds__(256) void resolvent_prep_kernel(const c128* V, const c128* Vi, long sV, int vmod,
                                                              int voff, const c128* P, c128* A, long sA, int n,
@@ -242,17 +243,16 @@ __global__ __launch_bounds__(256) void resolvent_prep_kernel(const c128* V, cons
   const long e1 = e0 + RP_ELEMS < nn ? e0 + RP_ELEMS : nn;
   double ar = 0.0, ai = 0.0;
   for (long e = e0 + threadIdx.x; e < e1; e += 256) {
+    const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
+    if (c > r) continue;
     const c128 p = Pb[e];
     if (part) {
       const c128 v = V[m + e];
-      ar += v.x * p.x + v.y * p.y;
-      ai += v.x * p.y - v.y * p.x;
+      const double re = v.x * p.x + v.y * p.y;  // Re(conj(v) p)
+      ar += (c == r) ? re : 2.0 * re;
     }
-    const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
-    if (c <= r) {
-      const c128 w = Vi[m + e];
-      Ab[e] = c128{w.x - p.x, w.y - p.y};
-    }
+    const c128 w = Vi[m + e];
+    Ab[e] = c128{w.x - p.x, w.y - p.y};
   }
   if (!part) return;
   red[0][threadIdx.x] = ar;
