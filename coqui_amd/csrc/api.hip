@@ -213,6 +213,21 @@ void project_a2p(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, 
                kZero, Oout, static_cast<long>(no) * no, no, static_cast<int>(nbk));
 }
 
+// The same for Hermitian O_in, which is overwritten: with O_in = L + L^H (L its lower
+// triangle with the diagonal halved), X^H O_in X = Y + Y^H for Y = X^H L X, whose first
+// product has a lower triangular operand (half the work of the full one).
+void project_a2p_herm(thcgw_ctx* ctx, hipStream_t s, const c128* X, c128* Oin, c128* Oout, long nbk, c128* Bbuf) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
+  const long sX = static_cast<long>(nmu) * no;
+  THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
+  thcgw::lower_half(s, Oin, nmu, nbk);
+  thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X, sX, nk), no, thcgw::bp(Oin, ctx->M), nmu,
+               kZero, Bbuf, sX, nmu, static_cast<int>(nbk), kZero, thcgw::GEMM_TRI_KFROM_COL);
+  thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), nmu, thcgw::bp(X, sX, nk), no,
+               kZero, Oout, static_cast<long>(no) * no, no, static_cast<int>(nbk));
+  thcgw::hermitize(s, Oout, no, nbk, 2.0);
+}
+
 // Mixed precision: X rounded to complex<float> once per call, FP32 products.
 const c64* x_c64(thcgw_ctx* ctx, const c128* X, hipStream_t s) {
   const long n = static_cast<long>(ctx->nk) * ctx->d.nmu * ctx->d.no;
@@ -236,17 +251,21 @@ void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* 
                ctx->pf() | (herm ? (thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB) : 0));
 }
 
-// O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>.
-void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c64* Oin, c128* Oout, long nbk, c64* Bbuf) {
+// O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>; herm as
+// for project_a2p_herm (O_in Hermitian, overwritten).
+void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, c64* Oin, c128* Oout, long nbk, c64* Bbuf,
+                     bool herm = false) {
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
+  if (herm) thcgw::lower_half(s, Oin, nmu, nbk);
   thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
-               thcgw::bp(Oin, ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu, static_cast<int>(nbk), kZero,
-               ctx->pf());
+               thcgw::bp(static_cast<const c64*>(Oin), ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu,
+               static_cast<int>(nbk), kZero, ctx->pf() | (herm ? thcgw::GEMM_TRI_KFROM_COL : 0));
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), DT_C64, nmu,
                thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C128, static_cast<long>(no) * no, no,
                static_cast<int>(nbk), kZero, ctx->pf());
+  if (herm) thcgw::hermitize(s, Oout, no, nbk, 2.0);
 }
 
 int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
@@ -797,10 +816,12 @@ void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X
                             SP + static_cast<long>(j) * nk * M,
                             second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale);
     }
-    project_a2p_f32(ctx, s, X64, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);
-    if (c2 > 0) project_a2p_f32(ctx, s, X64, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
+    project_a2p_f32(ctx, s, X64, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    if (c2 > 0)
+      project_a2p_f32(ctx, s, X64, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab, ctx->herm_g);
   }
-  if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
+  // with flag 10 Sigma_PQ is Hermitian and so is every back-projection (Y + Y^H)
+  if (herm && !ctx->herm_g) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
 }
 
 void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm, hipStream_t s) {
@@ -830,10 +851,16 @@ void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c1
                         static_cast<long>(nth) * M, SP + static_cast<long>(j) * nk * M,
                         second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale);
     }
-    project_a2p(ctx, s, X, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);
-    if (c2 > 0) project_a2p(ctx, s, X, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
+    if (ctx->herm_g) {
+      project_a2p_herm(ctx, s, X, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);
+      if (c2 > 0) project_a2p_herm(ctx, s, X, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
+    } else {
+      project_a2p(ctx, s, X, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);
+      if (c2 > 0) project_a2p(ctx, s, X, SN, Sig + (nt - i0 - c2) * gs, static_cast<long>(c2) * nk, Ab);
+    }
   }
-  if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
+  // with flag 10 Sigma_PQ is Hermitian and so is every back-projection (Y + Y^H)
+  if (herm && !ctx->herm_g) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
 }
 
 // ---- k-space (convolution) variants of chi0 and Sigma ----------------------------------

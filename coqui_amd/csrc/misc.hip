@@ -29,8 +29,9 @@ __global__ __launch_bounds__(T256) void scale_mo_kernel(const c128* C, const dou
   CG[e] = c128{c.x * g, c.y * g};
 }
 
-// A <- (A + A^H)/2 for each n x n matrix (scf_common.hpp:70-86); thread per (i, j), i <= j.
-__global__ __launch_bounds__(T256) void hermitize_kernel(c128* A, int n, long nb) {
+// A <- scale (A + A^H)/2 for each n x n matrix (scf_common.hpp:70-86 with scale = 1); thread
+// per (i, j), i <= j.
+__global__ __launch_bounds__(T256) void hermitize_kernel(c128* A, int n, long nb, double scale) {
   const long per = static_cast<long>(n) * n;
   const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
   if (e >= per * nb) return;
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(T256) void hermitize_kernel(c128* A, int n, long nb
   if (i > j) return;
   c128* Ab = A + b * per;
   const c128 x = Ab[static_cast<long>(i) * n + j], y = Ab[static_cast<long>(j) * n + i];
-  const c128 h{0.5 * (x.x + y.x), 0.5 * (x.y - y.y)};
+  const c128 h{0.5 * scale * (x.x + y.x), 0.5 * scale * (x.y - y.y)};
   Ab[static_cast<long>(i) * n + j] = h;
   Ab[static_cast<long>(j) * n + i] = c128{h.x, -h.y};
 }
@@ -526,13 +527,40 @@ void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, 
   THCGW_HIP(hipGetLastError());
 }
 
-void hermitize(hipStream_t s, c128* A, int n, long nb) {
+void hermitize(hipStream_t s, c128* A, int n, long nb, double scale) {
   const long total = static_cast<long>(n) * n * nb;
   if (total == 0) return;
   hipLaunchKernelGGL(hermitize_kernel, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, A, n,
-                     nb);
+                     nb, scale);
   THCGW_HIP(hipGetLastError());
 }
+
+// Lower half of a Hermitian matrix, L with A = L + L^H: the strict upper triangle set to zero
+// and the diagonal halved, for nb n x n matrices in place.
+template <typename T>
+__global__ __launch_bounds__(T256) void lower_half_kernel(T* A, int n, long nb) {
+  const long per = static_cast<long>(n) * n;
+  const long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x;
+  if (e >= per * nb) return;
+  const int i = static_cast<int>((e % per) / n), j = static_cast<int>(e % n);
+  if (i > j) return;
+  using S = decltype(T{}.x);
+  const T x = A[e];
+  A[e] = (i == j) ? T{static_cast<S>(0.5) * x.x, static_cast<S>(0.5) * x.y} : T{0, 0};
+}
+
+template <typename T>
+void lower_half_t(hipStream_t s, T* A, int n, long nb) {
+  const long total = static_cast<long>(n) * n * nb;
+  if (total == 0) return;
+  THCGW_REQUIRE((total + T256 - 1) / T256 < (1L << 31), "lower_half: too many elements");
+  hipLaunchKernelGGL(lower_half_kernel<T>, dim3(static_cast<unsigned>((total + T256 - 1) / T256)), dim3(T256), 0, s, A,
+                     n, nb);
+  THCGW_HIP(hipGetLastError());
+}
+
+void lower_half(hipStream_t s, c128* A, int n, long nb) { lower_half_t(s, A, n, nb); }
+void lower_half(hipStream_t s, c64* A, int n, long nb) { lower_half_t(s, A, n, nb); }
 
 void one_minus_trace(hipStream_t s, const c128* A, long sA, int n, c128* trace, int batch) {
   hipLaunchKernelGGL(one_minus_trace_kernel, dim3(batch), dim3(T256), 0, s, A, sA, n, trace);

@@ -175,7 +175,10 @@ void ir_transform(hipStream_t s, const c128* T, int R, int I, const c128* in, lo
 // misc.hip
 void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, double mu, double beta, int nt, int nk,
               int no, c128* CG);
-void hermitize(hipStream_t s, c128* A, int n, long nb);
+void hermitize(hipStream_t s, c128* A, int n, long nb, double scale = 1.0);
+// L with A = L + L^H in place (strict upper triangle zeroed, diagonal halved)
+void lower_half(hipStream_t s, c128* A, int n, long nb);
+void lower_half(hipStream_t s, c64* A, int n, long nb);
 void one_minus_trace(hipStream_t s, const c128* A, long sA, int n, c128* trace, int batch);
 void log_re_det(hipStream_t s, const c128* A, long sA, int n, const int* ipiv, double* ldet, int batch);
 // W = P Z - V: rows of Z in the order that undoes the LU column swaps, minus V (nullable).

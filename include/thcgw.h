@@ -348,7 +348,9 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           in FP32;
  *   flag 10: G(tau, k) is Hermitian (as for any G built from a Hermitian H(k) and Fermi
  *           occupations): the projections X G X^H for chi0 and Sigma form their lower tiles
- *           and mirror them, half the work of that product.  thcgw_pi, thcgw_sigma and
+ *           and mirror them, half the work of that product, and Sigma's back-projection
+ *           X^H S X of the then Hermitian S is formed as Y + Y^H, Y = X^H L X with L the
+ *           lower half of S (Sigma is returned Hermitian).  thcgw_pi, thcgw_sigma and
  *           thcgw_gw_iteration refuse G with max |G - G^H| > 1e-12 max |G| (THCGW_ERR_ARG);
  *   flag 8: FP32 GEMM tile shape (process-wide; products without FP64 chunk accumulation):
  *           0 = 64x64, 1 = 128x64, 2 = 128x128;
