@@ -213,14 +213,16 @@ void project_a2p(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, 
                kZero, Oout, static_cast<long>(no) * no, no, static_cast<int>(nbk));
 }
 
-// The same for Hermitian O_in, which is overwritten: with O_in = L + L^H (L its lower
-// triangle with the diagonal halved), X^H O_in X = Y + Y^H for Y = X^H L X, whose first
-// product has a lower triangular operand (half the work of the full one).
-void project_a2p_herm(thcgw_ctx* ctx, hipStream_t s, const c128* X, c128* Oin, c128* Oout, long nbk, c128* Bbuf) {
+// The same for a Hermitian O = L + L^H given by its lower half L (the lower triangle with the
+// diagonal halved, zeros above it within the diagonal 128 x 128 blocks): X^H O X = Y + Y^H for
+// Y = X^H L X, whose first product has a lower triangular operand (half the work of the
+// full one).
+void project_a2p_herm(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Lin, c128* Oout, long nbk,
+                      c128* Bbuf) {
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
-  thcgw::lower_half(s, Oin, nmu, nbk);
+  const c128* Oin = Lin;
   thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X, sX, nk), no, thcgw::bp(Oin, ctx->M), nmu,
                kZero, Bbuf, sX, nmu, static_cast<int>(nbk), kZero, thcgw::GEMM_TRI_KFROM_COL);
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), nmu, thcgw::bp(X, sX, nk), no,
@@ -251,16 +253,15 @@ void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* 
                ctx->pf() | (herm ? (thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB) : 0));
 }
 
-// O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>; herm as
-// for project_a2p_herm (O_in Hermitian, overwritten).
-void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, c64* Oin, c128* Oout, long nbk, c64* Bbuf,
+// O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>; herm: O_in
+// holds the lower half of a Hermitian matrix, as for project_a2p_herm.
+void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c64* Oin, c128* Oout, long nbk, c64* Bbuf,
                      bool herm = false) {
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
-  if (herm) thcgw::lower_half(s, Oin, nmu, nbk);
   thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
-               thcgw::bp(static_cast<const c64*>(Oin), ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu,
+               thcgw::bp(Oin, ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu,
                static_cast<int>(nbk), kZero, ctx->pf() | (herm ? thcgw::GEMM_TRI_KFROM_COL : 0));
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), DT_C64, nmu,
                thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C128, static_cast<long>(no) * no, no,
@@ -814,7 +815,8 @@ void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X
                             second ? GN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr,
                             Wqt + static_cast<long>(i) * M, static_cast<long>(nth) * M,
                             SP + static_cast<long>(j) * nk * M,
-                            second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale);
+                            second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale,
+                            ctx->herm_g ? nmu : 0);
     }
     project_a2p_f32(ctx, s, X64, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab, ctx->herm_g);
     if (c2 > 0)
@@ -849,7 +851,8 @@ void run_sigma(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c1
       thcgw::sigma_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP + static_cast<long>(j) * nk * M,
                         second ? GN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, Wqt + static_cast<long>(i) * M,
                         static_cast<long>(nth) * M, SP + static_cast<long>(j) * nk * M,
-                        second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale);
+                        second ? SN + static_cast<long>(c2 - 1 - j) * nk * M : nullptr, M, scale,
+                        ctx->herm_g ? nmu : 0);
     }
     if (ctx->herm_g) {
       project_a2p_herm(ctx, s, X, SP, Sig + i0 * gs, static_cast<long>(c) * nk, Ab);

@@ -194,16 +194,35 @@ __global__ __launch_bounds__(KT) void pi_kfft_kernel(const TG* __restrict__ Gp, 
 }
 
 // S(:, col) = scale * F+[ F-'[G] o F-'[W] ] for up to two G arrays sharing W; scale = -1/N_k^2.
+// herm_n > 0 (Hermitian S, order herm_n): only the lower half L of S = L + L^H is written: the
+// columns above the diagonal are not formed (zeros are written only inside the diagonal
+// 128 x 128 blocks, the part that a product with K from the tile column reads with tiles up
+// to 128 wide) and the diagonal is halved.
 template <int N1, int N2, int N3, int S, typename RC, typename TG, typename TW, typename TO>
 __global__ __launch_bounds__(KT) void sigma_kfft_kernel(const TG* __restrict__ G1, const TG* __restrict__ G2,
                                                         const TW* __restrict__ W, long w_kstride,
                                                         TO* __restrict__ S1, TO* __restrict__ S2, long M,
-                                                        double scale) {
+                                                        double scale, int herm_n) {
   constexpr int L = (N1 / S) * N2 * N3;
   const long gid = static_cast<long>(blockIdx.x) * KT + threadIdx.x;
   const long col = gid / S;
   const int h = static_cast<int>(gid % S);
   if (col >= M) return;
+  bool half = false;
+  if (herm_n > 0) {  // a column's S lanes share (P, Q) and take the same branch
+    const int P = static_cast<int>(col / herm_n), Q = static_cast<int>(col % herm_n);
+    if (Q > P) {
+      if ((Q >> 7) == (P >> 7)) {
+        RC z[L];
+#pragma unroll
+        for (int e = 0; e < L; ++e) z[e] = RC{0, 0};
+        store_col<N1, N2, N3, S>(z, S1, M, col, h);
+        if (G2 != nullptr) store_col<N1, N2, N3, S>(z, S2, M, col, h);
+      }
+      return;
+    }
+    half = (P == Q);
+  }
   RC w[L], g[L];
   load_col<N1, N2, N3, S>(w, W, w_kstride, col, h);
   dft3<N1, N2, N3, S, -1>(w, h);
@@ -219,6 +238,10 @@ __global__ __launch_bounds__(KT) void sigma_kfft_kernel(const TG* __restrict__ G
 #pragma unroll
     for (int e = 0; e < L; ++e) g[e] = cmul(g[e], w[e]);
     dft3<N1, N2, N3, S, +1>(g, h);
+    if (half) {
+#pragma unroll
+      for (int e = 0; e < L; ++e) g[e] = cscale(g[e], static_cast<scal_t<RC>>(0.5));
+    }
     store_col<N1, N2, N3, S>(g, O, M, col, h);
   }
 }
@@ -299,11 +322,11 @@ void launch_pi(hipStream_t s, const TG* Gp, const TG* Gn, TO* Pi, long M, double
 
 template <int N1, int N2, int N3, int S, typename RC, typename TG, typename TW, typename TO>
 void launch_sigma(hipStream_t s, const TG* G1, const TG* G2, const TW* W, long wks, TO* S1, TO* S2, long M,
-                  double scale) {
+                  double scale, int herm_n) {
   const long threads = M * S;
   const unsigned grid = static_cast<unsigned>((threads + KT - 1) / KT);
   hipLaunchKernelGGL((sigma_kfft_kernel<N1, N2, N3, S, RC, TG, TW, TO>), dim3(grid), dim3(KT), 0, s, G1, G2, W, wks,
-                     S1, S2, M, scale);
+                     S1, S2, M, scale, herm_n);
 }
 
 int grid_code(const int* kg) {
@@ -343,19 +366,24 @@ void pi_kfft_t(hipStream_t s, const int* kgrid, bool force_fallback, const TG* G
 
 template <typename RC, typename TG, typename TW, typename TO>
 void sigma_kfft_t(hipStream_t s, const int* kgrid, bool force_fallback, const TG* G1, const TG* G2, const TW* W,
-                  long w_kstride, TO* S1, TO* S2, long M, double scale) {
+                  long w_kstride, TO* S1, TO* S2, long M, double scale, int herm_n) {
   const int code = force_fallback ? 0 : grid_code(kgrid);
   switch (code) {
-    case 1: launch_sigma<1, 1, 1, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
-    case 2: launch_sigma<2, 2, 2, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
-    case 3: launch_sigma<3, 3, 3, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
-    case 4: launch_sigma<4, 4, 4, 4, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale); break;
+    case 1: launch_sigma<1, 1, 1, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale, herm_n); break;
+    case 2: launch_sigma<2, 2, 2, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale, herm_n); break;
+    case 3: launch_sigma<3, 3, 3, 1, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale, herm_n); break;
+    case 4: launch_sigma<4, 4, 4, 4, RC>(s, G1, G2, W, w_kstride, S1, S2, M, scale, herm_n); break;
     default: {
       const int nk = kgrid[0] * kgrid[1] * kgrid[2];
       THCGW_REQUIRE(nk <= FB_NK, "k-grid not supported by the fused transform kernels");
       const unsigned grid = static_cast<unsigned>((M + FB_T - 1) / FB_T);
       hipLaunchKernelGGL((kfft_fallback_kernel<TG, TW, TO>), dim3(grid), dim3(FB_T), 0, s, 1, kgrid[0], kgrid[1],
                          kgrid[2], G1, G2, W, w_kstride, S1, S2, M, scale);
+      THCGW_HIP(hipGetLastError());
+      if (herm_n > 0) {  // the lower half of the full result
+        lower_half(s, S1, herm_n, nk);
+        if (G2 != nullptr) lower_half(s, S2, herm_n, nk);
+      }
     }
   }
   THCGW_HIP(hipGetLastError());
@@ -369,8 +397,8 @@ void pi_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G
 }
 
 void sigma_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G1, const c128* G2,
-                const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale) {
-  sigma_kfft_t<c128>(s, kgrid, force_fallback, G1, G2, W, w_kstride, S1, S2, M, scale);
+                const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale, int herm_n) {
+  sigma_kfft_t<c128>(s, kgrid, force_fallback, G1, G2, W, w_kstride, S1, S2, M, scale, herm_n);
 }
 
 void pi_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* Gp, const c64* Gn, c128* Pi,
@@ -379,8 +407,8 @@ void pi_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64
 }
 
 void sigma_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* G1, const c64* G2,
-                    const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale) {
-  sigma_kfft_t<c64>(s, kgrid, force_fallback, G1, G2, W, w_kstride, S1, S2, M, scale);
+                    const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale, int herm_n) {
+  sigma_kfft_t<c64>(s, kgrid, force_fallback, G1, G2, W, w_kstride, S1, S2, M, scale, herm_n);
 }
 
 }  // namespace thcgw

@@ -120,13 +120,14 @@ void gemm_profile_read(double* ms, double* flops, long* launches);
 bool kfft_supported(const int* kgrid);
 void pi_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* Gp, const c128* Gn, c128* Pi,
              long M, double scale);
+// herm_n > 0: S Hermitian of order herm_n, only its lower half L (S = L + L^H) is written
 void sigma_kfft(hipStream_t s, const int* kgrid, bool force_fallback, const c128* G1, const c128* G2,
-                const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale);
+                const c128* W, long w_kstride, c128* S1, c128* S2, long M, double scale, int herm_n = 0);
 // FP32 arithmetic (the generic-grid fallback computes in FP64), complex<float> G and Sigma.
 void pi_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* Gp, const c64* Gn, c128* Pi,
                  long M, double scale);
 void sigma_kfft_f32(hipStream_t s, const int* kgrid, bool force_fallback, const c64* G1, const c64* G2,
-                    const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale);
+                    const c128* W, long w_kstride, c64* S1, c64* S2, long M, double scale, int herm_n = 0);
 
 // kconv.hip: k-space convolutions (chi0 uses the [q][k] table, Sigma the [k][q] one)
 std::vector<int> kminus_table(const int* kgrid);  // t[a * nk + b] = index of k_a - k_b

@@ -191,22 +191,27 @@ def test_flag6_range():
 
 
 def test_hermitian_g_projections(orc, torch):
-    """Flag 10: the projections of a Hermitian G form their lower tiles and mirror them.
-    The iteration equals the full-product one to rounding (1e-12), in FP64 and with the
-    FP32 projections, and G that is not Hermitian is refused."""
+    """Flag 10: the projections of a Hermitian G form their lower tiles and mirror them, and
+    Sigma's back-projection uses the lower half of Sigma_PQ.  The iteration equals the
+    full-product one to rounding (1e-12 in FP64, 1e-5 with the FP32 projections) with every
+    GEMM tile width, and G that is not Hermitian is refused."""
     from coqui_amd import ThcgwError
 
-    s, ia, tab, G = system(orc, (2, 2, 2), 10, 72, "1e2.1e-15", sid=6)
+    s, ia, tab, G = system(orc, (2, 2, 2), 10, 200, "1e2.1e-15", sid=6)
     ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
-    for prec in (0, 1):
+    # every GEMM tile width: flag 5 (FP64) 0 = automatic, 3 = 128 x 128; flag 8 (FP32) 0 = 64, 2 = 128
+    for prec, tile_flag, tile in ((0, 5, 0), (0, 5, 3), (1, 8, 0), (1, 8, 2)):
         ctx.set_flag(7, prec)
+        ctx.set_flag(tile_flag, tile)
         ctx.set_flag(10, 0)
         S0, W0, e0 = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
         S0, W0 = to_np(S0), to_np(W0)
         ctx.set_flag(10, 1)
         S1, W1, e1 = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
         S1, W1 = to_np(S1), to_np(W1)
-        print("prec %d: W %.1e Sigma %.1e Ec %.1e" % (prec, relmax(W1, W0), relmax(S1, S0), abs(e1 - e0)))
+        ctx.set_flag(tile_flag, 0)
+        print("prec %d tile %d: W %.1e Sigma %.1e Ec %.1e" % (prec, tile, relmax(W1, W0), relmax(S1, S0),
+                                                              abs(e1 - e0)))
         tol = 1e-12 if prec == 0 else 1e-5
         assert relmax(W1, W0) < tol and relmax(S1, S0) < tol and abs(e1 - e0) < tol * abs(e0)
     ctx.set_flag(7, 0)
