@@ -121,10 +121,11 @@ def run_cpu_baseline(s, ia, G_pair):
         return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def cpu_baseline(s, ia, G_host_pair, budget_note):
-    """Oracle (C++/OpenMP FP64 restatement of the reference) on a bounded sample:
-    one tau slice of chi0, one tau pair of Sigma, two Dyson and two Ec matrices, one q
-    block of each imaginary-axis transform; extrapolated to a full iteration."""
+def cpu_baseline(s, ia, G_host_pair, budget_note, reps=2):
+    """Oracle (C++/OpenMP FP64 restatement of the reference) on a bounded sample, each part
+    timed `reps` times (mean): one tau slice of chi0, one tau pair of Sigma, 4 reps Dyson and
+    4 reps Ec matrices, 2 reps q blocks of each imaginary-axis transform; extrapolated to a
+    full iteration."""
     from oracle import oracle
 
     tab = oracle.load_ir(ia.beta, None, table=ia.name)
@@ -133,39 +134,41 @@ def cpu_baseline(s, ia, G_host_pair, budget_note):
     nth, nwh = ia.nt_half, ia.nw_half
     rng = np.random.default_rng(0)
     t = {}
-    t0 = time.perf_counter()
-    Pi1 = oracle.pi_rspace(G_host_pair, s.X, s.kgrid)  # nt = 2 -> one tau slice
-    t["pi_slice"] = time.perf_counter() - t0
+    sample_s = 0.0
+
+    def timed(key, fn, count):
+        nonlocal sample_s
+        out = None
+        t0 = time.perf_counter()
+        for _ in range(count):
+            out = fn()
+        dt = time.perf_counter() - t0
+        sample_s += dt
+        t[key] = dt / count
+        return out
+
+    Pi1 = timed("pi_slice", lambda: oracle.pi_rspace(G_host_pair, s.X, s.kgrid), reps)  # nt = 2 -> one tau slice
     Wqt = np.ascontiguousarray(np.swapaxes(Pi1, 0, 1))  # (nk, 1, nmu, nmu): any W of the right shape
-    t0 = time.perf_counter()
-    oracle.sigma_rspace(G_host_pair, s.X, Wqt, s.kgrid)  # both passes of one tau pair
-    t["sigma_pair"] = time.perf_counter() - t0
+    timed("sigma_pair", lambda: oracle.sigma_rspace(G_host_pair, s.X, Wqt, s.kgrid), reps)  # both passes of one pair
     B = rng.standard_normal((1, 2, nmu, nmu)) + 1j * rng.standard_normal((1, 2, nmu, nmu))
     Pi = -(B @ np.conj(np.swapaxes(B, -1, -2))) / nmu * 0.05
     V2 = s.V_host2
-    t0 = time.perf_counter()
-    oracle.dyson(Pi, V2)
-    t["dyson_2"] = time.perf_counter() - t0
+    timed("dyson_2", lambda: oracle.dyson(Pi, V2), 2 * reps)
     tab2 = dict(tab, nw_b=1, Ttw_bb=np.ascontiguousarray(tab["Ttw_bb"][:, :1]))  # one iW_n: two (q, iW) matrices
-    t0 = time.perf_counter()
-    oracle.rpa_ec(tab2, Pi, V2)
-    t["ec_2"] = time.perf_counter() - t0
+    timed("ec_2", lambda: oracle.rpa_ec(tab2, Pi, V2), 2 * reps)
     Xt = rng.standard_normal((nth, 1, M)) + 0j
-    t0 = time.perf_counter()
-    Xw = oracle.tau_to_w_phsym(tab, Xt)
-    oracle.w_to_tau_phsym(tab, Xw)
-    t["ft_q"] = time.perf_counter() - t0
+    timed("ft_q", lambda: oracle.w_to_tau_phsym(tab, oracle.tau_to_w_phsym(tab, Xt)), 2 * reps)
     total = (t["pi_slice"] * nth + (t["dyson_2"] + t["ec_2"]) / 2.0 * nwh * nk + t["ft_q"] * nk
              + t["sigma_pair"] * nth)
-    sample_s = sum(t.values())
     return {
         "value": total,
         "unit": "s",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": ("oracle on one tau slice of chi0, one tau pair of Sigma, 2 Dyson + 2 Ec (q, iW) matrices "
-                   "and one q block of tau<->iW, timed %.1f s and extrapolated to %d tau slices, %d (q, iW) "
-                   "matrices, %d q blocks; %s" % (sample_s, nth, nwh * nk, nk, budget_note)),
+        "sample": ("oracle on one tau slice of chi0 and one tau pair of Sigma (%d times each), 2 Dyson + 2 Ec "
+                   "(q, iW) matrices and one q block of tau<->iW (%d times each), timed %.1f s in all and "
+                   "extrapolated from the mean times to %d tau slices, %d (q, iW) matrices, %d q blocks; %s"
+                   % (reps, 2 * reps, sample_s, nth, nwh * nk, nk, budget_note)),
         "sample_times_s": {k: round(v, 3) for k, v in t.items()},
     }
 
