@@ -228,44 +228,64 @@ __global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL
 // lower triangle of A[b] = Vi[m] - P[b], and, when part is given, per workgroup the partial
 // sum of Tr(V P) for Hermitian V and P, sum_i V_ii P_ii + 2 sum_{i>j} Re(conj(V_ij) P_ij)
 // (real), at part[b * gridDim.x + blockIdx.x].  Each workgroup covers RP_ELEMS consecutive
-// elements; the form reads only these lower triangles.
+// elements of the matrices b = j, j + vmod, j + 2 vmod, ... (j = blockIdx.y) that share
+// V[m] and Vi[m], RP_GROUP of them per pass: the V and Vi elements are read once per pass
+// instead of once per matrix.  The form reads only these lower triangles.
 constexpr int RP_ELEMS = 4096;
+constexpr int RP_GROUP = 8;
 __global__ __launch_bounds__(256) void resolvent_prep_kernel(const c128* V, const c128* Vi, long sV, int vmod,
                                                              int voff, const c128* P, c128* A, long sA, int n,
-                                                             c128* part) {
-  __shared__ double red[2][256];
-  const int b = blockIdx.y;
+                                                             c128* part, int batch) {
+  __shared__ double red[RP_GROUP][256];
+  const int j = blockIdx.y;
   const long nn = static_cast<long>(n) * n;
-  const long m = static_cast<long>((b + voff) % vmod) * sV;
-  const c128* Pb = P + static_cast<long>(b) * sA;
-  c128* Ab = A + static_cast<long>(b) * sA;
+  const long m = static_cast<long>((j + voff) % vmod) * sV;
   const long e0 = static_cast<long>(blockIdx.x) * RP_ELEMS;
   const long e1 = e0 + RP_ELEMS < nn ? e0 + RP_ELEMS : nn;
-  double ar = 0.0, ai = 0.0;
-  for (long e = e0 + threadIdx.x; e < e1; e += 256) {
-    const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
-    if (c > r) continue;
-    const c128 p = Pb[e];
-    if (part) {
-      const c128 v = V[m + e];
-      const double re = v.x * p.x + v.y * p.y;  // Re(conj(v) p)
-      ar += (c == r) ? re : 2.0 * re;
+  for (int bg = j; bg < batch; bg += RP_GROUP * vmod) {
+    int nb = 0;  // matrices in this pass: bg + k vmod for k < nb
+    while (nb < RP_GROUP && bg + nb * vmod < batch) ++nb;
+    double acc[RP_GROUP];
+#pragma unroll
+    for (int k = 0; k < RP_GROUP; ++k) acc[k] = 0.0;
+    const long first = e0 + threadIdx.x;
+    int r = static_cast<int>(first / n), c = static_cast<int>(first % n);
+    for (long e = first; e < e1; e += 256) {
+      if (c <= r) {
+        const c128 w = Vi[m + e];
+        const c128 v = part ? V[m + e] : c128{0.0, 0.0};
+        const double f = (c == r) ? 1.0 : 2.0;
+#pragma unroll
+        for (int k = 0; k < RP_GROUP; ++k) {
+          if (k < nb) {
+            const long off = static_cast<long>(bg + k * vmod) * sA + e;
+            const c128 pv = P[off];
+            A[off] = c128{w.x - pv.x, w.y - pv.y};
+            acc[k] += f * (v.x * pv.x + v.y * pv.y);  // Re(conj(v) p)
+          }
+        }
+      }
+      c += 256;  // next element of this thread: 256 further in row-major order
+      while (c >= n) {
+        c -= n;
+        ++r;
+      }
     }
-    const c128 w = Vi[m + e];
-    Ab[e] = c128{w.x - p.x, w.y - p.y};
-  }
-  if (!part) return;
-  red[0][threadIdx.x] = ar;
-  red[1][threadIdx.x] = ai;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + w];
-      red[1][threadIdx.x] += red[1][threadIdx.x + w];
-    }
+    if (!part) continue;
+#pragma unroll
+    for (int k = 0; k < RP_GROUP; ++k) red[k][threadIdx.x] = acc[k];
     __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+#pragma unroll
+        for (int k = 0; k < RP_GROUP; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x < nb)
+      part[static_cast<long>(bg + threadIdx.x * vmod) * gridDim.x + blockIdx.x] = c128{red[threadIdx.x][0], 0.0};
+    __syncthreads();  // red is reused by the next pass
   }
-  if (threadIdx.x == 0) part[static_cast<long>(b) * gridDim.x + blockIdx.x] = c128{red[0][0], red[1][0]};
 }
 
 // tr[b] = sum of the np partial sums part[b * np ..]; one workgroup per matrix.
@@ -542,8 +562,10 @@ void resolvent_prep(hipStream_t s, const c128* V, const c128* Vi, long sV, int v
   THCGW_REQUIRE(!tr || part, "resolvent_prep: the trace needs the partial-sum buffer");
   THCGW_REQUIRE(batch <= 65535, "resolvent_prep: at most 65535 matrices per launch");
   const int np = resolvent_prep_parts(n);
-  hipLaunchKernelGGL(resolvent_prep_kernel, dim3(np, batch), dim3(256), 0, s, V, Vi, sV, vmod, voff, P, A, sA, n,
-                     tr ? part : nullptr);
+  THCGW_REQUIRE(vmod >= 1 && voff >= 0, "resolvent_prep: bad V indexing");
+  const int slots = batch < vmod ? batch : vmod;  // one workgroup row per V member in use
+  hipLaunchKernelGGL(resolvent_prep_kernel, dim3(np, slots), dim3(256), 0, s, V, Vi, sV, vmod, voff, P, A, sA, n,
+                     tr ? part : nullptr, batch);
   THCGW_HIP(hipGetLastError());
   if (tr) {
     hipLaunchKernelGGL(sum_parts_kernel, dim3(batch), dim3(256), 0, s, part, np, tr);
