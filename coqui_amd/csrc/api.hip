@@ -42,6 +42,8 @@ struct thcgw_ctx {
   double* d_tau = nullptr;  // tau values (nt)
   c128* d_Tplus = nullptr;  // nw_half x nt_half (IAFT.icc:62-70)
   c128* d_Tminus = nullptr; // nt_half x nw_half (IAFT.icc:117-125)
+  bool real_Tplus = false;  // the folded tables are real up to rounding: only Re T is applied
+  bool real_Tminus = false;
   c128* d_Twt_ff = nullptr; // nw_f x nt
   c128* d_Ttw_ff = nullptr; // nt x nw_f
   c128* d_Tbeta = nullptr;  // 1 x nt
@@ -351,7 +353,8 @@ void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t 
 void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
   const long cols = ctx->M * ctx->nk;
   if (thcgw::ir_transform_fits(ctx->nw_half, ctx->nt_half)) {
-    thcgw::ir_transform(s, ctx->d_Tplus, ctx->nw_half, ctx->nt_half, Pit, cols, 0, Piw, cols, 0, cols, 1);
+    thcgw::ir_transform(s, ctx->d_Tplus, ctx->nw_half, ctx->nt_half, Pit, cols, 0, Piw, cols, 0, cols, 1,
+                        ctx->real_Tplus);
     return;
   }
   THCGW_REQUIRE(cols < (1L << 31), "tau_to_w: N_q N_mu^2 too large for one GEMM");
@@ -365,7 +368,7 @@ void run_w_to_tau(thcgw_ctx* ctx, const c128* Ww, c128* Wqt, hipStream_t s) {
   const long M = ctx->M, nk = ctx->nk;
   if (thcgw::ir_transform_fits(ctx->nt_half, ctx->nw_half)) {
     thcgw::ir_transform(s, ctx->d_Tminus, ctx->nt_half, ctx->nw_half, Ww, nk * M, M, Wqt, M, ctx->nt_half * M, M,
-                        ctx->nk);
+                        ctx->nk, ctx->real_Tminus);
     return;
   }
   THCGW_REQUIRE(M < (1L << 31), "w_to_tau: N_mu^2 too large for one GEMM");
@@ -1084,6 +1087,19 @@ int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ct
     THCGW_HIP(hipGetDevice(&ctx->device));
     std::vector<std::complex<double>> Tp, Tm;
     thcgw::host::fold_ph(ctx->tab, d, Tp, Tm);
+    // Folding G(tau) and G(beta - tau) pairs e^{i W tau} with e^{-i W tau}: the tables are real
+    // up to the rounding of the IR tables (|Im| ~ 1e-14 |Re|).  Below 1e-12 the transforms
+    // apply the real part only, with half the MFMA work.
+    auto is_real = [](const std::vector<std::complex<double>>& T) {
+      double re = 0.0, im = 0.0;
+      for (const auto& z : T) {
+        re = std::max(re, std::fabs(z.real()));
+        im = std::max(im, std::fabs(z.imag()));
+      }
+      return im <= 1e-12 * re;
+    };
+    ctx->real_Tplus = is_real(Tp);
+    ctx->real_Tminus = is_real(Tm);
     THCGW_HIP(hipMalloc(&ctx->d_tau, sizeof(double) * d.nt));
     THCGW_HIP(hipMalloc(&ctx->d_Tplus, sizeof(c128) * Tp.size()));
     THCGW_HIP(hipMalloc(&ctx->d_Tminus, sizeof(c128) * Tm.size()));

@@ -10,7 +10,9 @@
 // the column blocks with a grid stride.  Per column the useful work is 8 R I flops against
 // 16 (R + I) bytes of HBM traffic, about 9 flops per byte at R = I = 36: close to the FP64
 // ridge of MI355X (78.6 TFLOP/s over 8 TB/s).  Padding R to 16 RB rows costs 30% of the
-// MFMA work at R = 37 against 73% for 64-row tiles.
+// MFMA work at R = 37 against 73% for 64-row tiles.  The particle-hole folded tables are real
+// up to rounding (|Im T| ~ 1e-14 |Re T|): with real_t the imaginary part is dropped, which
+// halves the MFMAs (2 instead of 4 per complex product) and leaves the kernel HBM-bound.
 #include "thcgw_internal.hpp"
 
 #include <algorithm>
@@ -20,7 +22,7 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-template <int RB, int KB>
+template <int RB, int KB, bool CT>
 __global__ __launch_bounds__(256) void ir_transform_kernel(const c128* __restrict__ T, int R, int I,
                                                            const c128* __restrict__ in, long ld_in, long bs_in,
                                                            c128* __restrict__ out, long ld_out, long bs_out,
@@ -68,8 +70,10 @@ __global__ __launch_bounds__(256) void ir_transform_kernel(const c128* __restric
         const c128 t = Tf[(rb * KB + kb) * 64 + lane];
         cr[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(t.x, xr[kb], cr[rb], 0, 0, 0);
         ci[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(t.x, xi[kb], ci[rb], 0, 0, 0);
-        cr[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(t.y, xn[kb], cr[rb], 0, 0, 0);
-        ci[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(t.y, xr[kb], ci[rb], 0, 0, 0);
+        if constexpr (CT) {  // complex T: the two products of its imaginary part
+          cr[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(t.y, xn[kb], cr[rb], 0, 0, 0);
+          ci[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(t.y, xr[kb], ci[rb], 0, 0, 0);
+        }
       }
     // accumulator register v of lane l holds row l/16 + 4v, column l%16
     if (col < cols) {
@@ -86,13 +90,17 @@ __global__ __launch_bounds__(256) void ir_transform_kernel(const c128* __restric
 
 template <int RB, int KB>
 void launch_irt(hipStream_t s, const c128* T, int R, int I, const c128* in, long ld_in, long bs_in, c128* out,
-                long ld_out, long bs_out, long cols, int batch) {
+                long ld_out, long bs_out, long cols, int batch, bool real_t) {
   const long nblk = (cols + 15) / 16;
   const long want = (nblk + 3) / 4;                  // one 16-column block per wave
   const long cap = std::max(1L, 4096L / batch);      // about 16 workgroups per CU in all
   const dim3 g(static_cast<unsigned>(std::min(want, cap)), batch);
-  hipLaunchKernelGGL((ir_transform_kernel<RB, KB>), g, dim3(256), 0, s, T, R, I, in, ld_in, bs_in, out, ld_out,
-                     bs_out, cols);
+  if (real_t)
+    hipLaunchKernelGGL((ir_transform_kernel<RB, KB, false>), g, dim3(256), 0, s, T, R, I, in, ld_in, bs_in, out,
+                       ld_out, bs_out, cols);
+  else
+    hipLaunchKernelGGL((ir_transform_kernel<RB, KB, true>), g, dim3(256), 0, s, T, R, I, in, ld_in, bs_in, out,
+                       ld_out, bs_out, cols);
   THCGW_HIP(hipGetLastError());
 }
 
@@ -102,16 +110,16 @@ void launch_irt(hipStream_t s, const c128* T, int R, int I, const c128* in, long
 bool ir_transform_fits(int R, int I) { return R >= 1 && I >= 1 && R <= 48 && I <= 40; }
 
 void ir_transform(hipStream_t s, const c128* T, int R, int I, const c128* in, long ld_in, long bs_in, c128* out,
-                  long ld_out, long bs_out, long cols, int batch) {
+                  long ld_out, long bs_out, long cols, int batch, bool real_t) {
   THCGW_REQUIRE(ir_transform_fits(R, I), "ir_transform: at most 48 rows and 40 columns of T");
   THCGW_REQUIRE(batch >= 1 && batch <= 65535, "ir_transform: batch out of range");
   if (cols <= 0) return;
   if (R <= 32 && I <= 24)
-    launch_irt<2, 6>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch);
+    launch_irt<2, 6>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
   else if (I <= 36)
-    launch_irt<3, 9>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch);
+    launch_irt<3, 9>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
   else
-    launch_irt<3, 10>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch);
+    launch_irt<3, 10>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
 }
 
 }  // namespace thcgw

@@ -168,10 +168,10 @@ void resolvent_prep(hipStream_t s, const c128* V, const c128* Vi, long sV, int v
 void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch);
 
 // irt.hip: out[b][r][c] = sum_i T[r][i] in[b][i][c] (T R x I row-major, R <= 48, I <= 40; rows of in
-// and out ld_in / ld_out apart, matrices bs_in / bs_out apart)
+// and out ld_in / ld_out apart, matrices bs_in / bs_out apart); real_t: only Re T is used
 bool ir_transform_fits(int R, int I);
 void ir_transform(hipStream_t s, const c128* T, int R, int I, const c128* in, long ld_in, long bs_in, c128* out,
-                  long ld_out, long bs_out, long cols, int batch);
+                  long ld_out, long bs_out, long cols, int batch, bool real_t = false);
 
 // misc.hip
 void scale_mo(hipStream_t s, const c128* C, const double* E, const double* tau, double mu, double beta, int nt, int nk,
