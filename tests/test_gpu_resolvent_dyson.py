@@ -64,6 +64,26 @@ def test_resolvent_dyson_nk8_nmu200(nk8_nmu200, torch):
     assert relmax(W, np.conj(np.swapaxes(W, -1, -2))) < 1e-14
 
 
+def test_resolvent_chunking_bit_exact(nk8_nmu200, torch):
+    """Each matrix's arithmetic does not depend on how the matrices are chunked: the
+    automatic chunk (one chunk here), ragged chunks of 7 (fewer than the 8 q, so each V_q
+    serves one matrix per chunk) and chunks of 20 (several matrices per V_q) give
+    bit-identical W, Tr(V Pi) and ln det."""
+    c = nk8_nmu200
+    outs = []
+    for chunk in (0, 7, 20):
+        c["ctx"].set_flag(3, chunk)
+        try:
+            PiW = torch.from_numpy(c["ref"]["Pi_w"].copy()).cuda()
+            tr, ld = c["ctx"].dyson_w(PiW, c["s"].V, with_terms=True)
+        finally:
+            c["ctx"].set_flag(3, 0)
+        outs.append((to_np(PiW), np.asarray(tr), np.asarray(ld)))
+    for W, tr, ld in outs[1:]:
+        assert np.array_equal(W, outs[0][0])
+        assert np.array_equal(tr, outs[0][1]) and np.array_equal(ld, outs[0][2])
+
+
 def test_resolvent_terms_and_rpa_ec(nk8_nmu200, torch):
     c = nk8_nmu200
     c["ctx"].set_flag(3, 7)  # ragged chunks
