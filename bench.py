@@ -185,6 +185,7 @@ def main():
     ap.add_argument("--gemm-tile", type=int, default=0, help="GEMM tile shape: 0 auto, 1 64x64, 2 128x64, 3 128x128")
     ap.add_argument("--gemm-tile32", type=int, default=0, help="FP32 GEMM tile shape: 0 64x64, 1 128x64, 2 128x128")
     ap.add_argument("--dyson-chunk", type=int, default=0, help="matrices per Dyson batch (0 = sized from free memory)")
+    ap.add_argument("--pair-chunk", type=int, default=0, help="tau pairs per projection batch (0 = sized from free memory)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")
     ap.add_argument("--scgw-iters", type=int, default=10, help="scgw-k64-mu1200: self-consistent iterations")
@@ -238,6 +239,7 @@ def main():
     ctx.set_flag(5, args.gemm_tile)
     ctx.set_flag(8, args.gemm_tile32)
     ctx.set_flag(3, args.dyson_chunk)
+    ctx.set_flag(2, args.pair_chunk)
     if s.V is not None:
         Vd = torch.from_numpy(s.V).cuda()
     else:
@@ -280,6 +282,7 @@ def main():
             ctx.set_flag(5, args.gemm_tile)
             ctx.set_flag(8, args.gemm_tile32)
             ctx.set_flag(3, args.dyson_chunk)
+            ctx.set_flag(2, args.pair_chunk)
         first_form[0] = False
         set_form(form)
         ec = None
