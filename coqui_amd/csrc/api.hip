@@ -71,6 +71,7 @@ struct thcgw_ctx {
                         // 2 = mixed with split-BF16 products in place of FP32 MFMA
   int tile32 = -1;      // FP32 tile shape chosen for this context (-1: process default)
   bool deferred = false;  // enqueue without host synchronisation (stream capture): checks run later
+  int nf_arrays = 0;      // arrays whose non-finite counts a deferred enqueue left in "nf_cnt"
   std::map<std::string, DevBuf>* pool = nullptr;  // scratch shared with the other systems of a batch
   // cgemm flags of this context: precision and FP32 tile shape
   int pf() const {
@@ -78,7 +79,7 @@ struct thcgw_ctx {
   }
   // Arrays read back after the stream completes stay private to the context.
   static bool private_buf(const std::string& name) {
-    return name == "dy_info" || name == "dy_vinfo" || name == "ec_tr" || name == "ec_ld";
+    return name == "dy_info" || name == "dy_vinfo" || name == "ec_tr" || name == "ec_ld" || name == "nf_cnt";
   }
 
   template <class T>
@@ -753,6 +754,43 @@ void run_dyson_sym(thcgw_ctx* ctx, c128* PiW, const c128* V, c128* d_trace, doub
   thcgw::qsym_fill_terms(s, nq, nw, src, cj, d_trace, d_ldet, info_all);
 }
 
+// Input guard of the entry points that factorise: refuses arrays holding a NaN or an
+// infinity (THCGW_ERR_NONFINITE) before any factorisation runs, so that no pivot search
+// or Cholesky step ever sees a non-finite column.  One HBM-bound pass per array.  In
+// deferred mode (graph capture) the counts stay on the device and check_deferred()
+// reads them after the stream completes.
+struct FiniteArg {
+  const void* p;
+  long ncplx;  // complex elements
+  const char* name;
+};
+constexpr int kNonfiniteBlocks = 512;
+
+void check_finite(thcgw_ctx* ctx, const char* entry, std::initializer_list<FiniteArg> args, hipStream_t s) {
+  std::vector<FiniteArg> a;
+  for (const FiniteArg& x : args)
+    if (x.p && x.ncplx > 0) a.push_back(x);
+  if (a.empty()) return;
+  int* cnt = ctx->buf<int>("nf_cnt", a.size() * kNonfiniteBlocks);
+  for (size_t i = 0; i < a.size(); ++i)
+    thcgw::nonfinite_count(s, static_cast<const double*>(a[i].p), 2 * a[i].ncplx, kNonfiniteBlocks,
+                           cnt + i * kNonfiniteBlocks);
+  if (ctx->deferred) {
+    ctx->nf_arrays = static_cast<int>(a.size());
+    return;
+  }
+  std::vector<int> h(a.size() * kNonfiniteBlocks);
+  THCGW_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int) * h.size(), hipMemcpyDeviceToHost, s));
+  THCGW_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < a.size(); ++i) {
+    long bad = 0;
+    for (int b = 0; b < kNonfiniteBlocks; ++b) bad += h[i * kNonfiniteBlocks + b];
+    if (bad > 0)
+      throw thcgw::Error(THCGW_ERR_NONFINITE, std::string(entry) + ": " + a[i].name + " holds " + std::to_string(bad) +
+                                                  " NaN or infinite values");
+  }
+}
+
 // In place Pi(iW) -> W(iW) - V; optional Tr(V Pi) and ln|Re det(I - V Pi)| per matrix
 // (device arrays of length nw_half*nk).  pi_first selects A = I - Pi V (thc_rpa.icc:141)
 // instead of I - V Pi, and solve = false stops after the determinant.  The Hermitian
@@ -952,6 +990,10 @@ void enqueue_iteration(thcgw_ctx* ctx, const c128* G, const c128* X, const c128*
   c128* Piw = ctx->buf<c128>("it_piw", static_cast<size_t>(total) * M);
   c128* dtr = want_ec ? ctx->buf<c128>("ec_tr", total) : nullptr;
   double* dld = want_ec ? ctx->buf<double>("ec_ld", total) : nullptr;
+  const long nk = ctx->nk, no = ctx->d.no;
+  check_finite(ctx, "gw_iteration",
+               {{G, ctx->d.nt * nk * no * no, "G_tskij"}, {X, nk * ctx->d.nmu * no, "X_skPa"}, {V, ctx->vq() * M, "V_qPQ"}},
+               s);
   run_pi(ctx, G, X, Pit, s);
   run_tau_to_w(ctx, Pit, Piw, s);
   run_dyson(ctx, Piw, V, dtr, dld, true, false, s);
@@ -962,6 +1004,17 @@ void enqueue_iteration(thcgw_ctx* ctx, const c128* G, const c128* X, const c128*
 // After a deferred enqueue has completed: Cholesky / LU flags of the Dyson step.
 void check_deferred(thcgw_ctx* ctx, hipStream_t s) {
   const int total = ctx->nw_half * ctx->nk, nq = ctx->nk;
+  if (ctx->nf_arrays > 0) {
+    std::vector<int> h(static_cast<size_t>(ctx->nf_arrays) * kNonfiniteBlocks);
+    THCGW_HIP(hipMemcpyAsync(h.data(), ctx->buf<int>("nf_cnt", h.size()), sizeof(int) * h.size(),
+                             hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    long bad = 0;
+    for (int x : h) bad += x;
+    if (bad > 0)
+      throw thcgw::Error(THCGW_ERR_NONFINITE,
+                         "batch: the inputs G, X or V hold " + std::to_string(bad) + " NaN or infinite values");
+  }
   if (ctx->precision != 0 || ctx->dyson_algo != 0) {
     std::vector<int> vi(nq);
     THCGW_HIP(hipMemcpyAsync(vi.data(), ctx->buf<int>("dy_vinfo", static_cast<size_t>(nq)), sizeof(int) * nq,
@@ -991,6 +1044,14 @@ double finish_ec(thcgw_ctx* ctx, hipStream_t s) {
 }
 
 // ---- Dyson equation for G on the Matsubara axis ------------------------------------
+
+// The Dyson equation for G refuses non-finite H0, F, S or Sigma (see check_finite).
+void check_dyson_g_inputs(thcgw_ctx* ctx, const char* entry, const double* H0, const double* F, const double* Sov,
+                          const double* Sigma, hipStream_t s) {
+  const long per = static_cast<long>(ctx->nk) * ctx->d.no * ctx->d.no;
+  check_finite(ctx, entry, {{H0, per, "H0_skij"}, {F, per, "F_skij"}, {Sov, per, "S_skij"},
+                            {Sigma, ctx->d.nt * per, "Sigma_tskij"}}, s);
+}
 
 // Sw(iw_n) = tau_to_w(Sigma(tau), fermi) over all (k, i, j)   (simple_dyson.cpp:39-40)
 c128* sigma_to_w(thcgw_ctx* ctx, const c128* Sigma, hipStream_t s) {
@@ -1333,6 +1394,7 @@ int thcgw_dyson_w(thcgw_ctx* ctx, double* PiW, const double* V, double* trace_ou
   return guarded(ctx, [&] {
     THCGW_REQUIRE(PiW && V, "thcgw_dyson_w: null pointer");
     const int total = ctx->nw_half * ctx->nk;
+    check_finite(ctx, "thcgw_dyson_w", {{PiW, total * ctx->M, "Pi_wqPQ"}, {V, ctx->vq() * ctx->M, "V_qPQ"}}, S(stream));
     const bool terms = trace_out || logdet_out;
     c128* dtr = terms ? ctx->buf<c128>("ec_tr", total) : nullptr;
     double* dld = terms ? ctx->buf<double>("ec_ld", total) : nullptr;
@@ -1369,6 +1431,7 @@ int thcgw_rpa_ec(thcgw_ctx* ctx, const double* Piw, const double* V, double* Ec,
   return guarded(ctx, [&] {
     THCGW_REQUIRE(Piw && V && Ec, "thcgw_rpa_ec: null pointer");
     const int total = ctx->nw_half * ctx->nk;
+    check_finite(ctx, "thcgw_rpa_ec", {{Piw, total * ctx->M, "Pi_wqPQ"}, {V, ctx->vq() * ctx->M, "V_qPQ"}}, S(stream));
     c128* dtr = ctx->buf<c128>("ec_tr", total);
     double* dld = ctx->buf<double>("ec_ld", total);
     // Pi_wqPQ is read only: factor copies of (I - Pi V) in the Dyson workspace.
@@ -1449,6 +1512,7 @@ int thcgw_dyson_g(thcgw_ctx* ctx, const double* H0, const double* F, const doubl
   return guarded(ctx, [&] {
     THCGW_REQUIRE(F && Sigma && G, "thcgw_dyson_g: null pointer");
     hipStream_t s = S(stream);
+    check_dyson_g_inputs(ctx, "thcgw_dyson_g", H0, F, Sov, Sigma, s);
     const int nt = ctx->d.nt, nw = ctx->d.nw_f;
     const long dim = static_cast<long>(ctx->nk) * ctx->d.no * ctx->d.no;
     c128* Sw = sigma_to_w(ctx, C(Sigma), s);
@@ -1469,6 +1533,7 @@ int thcgw_nelec(thcgw_ctx* ctx, const double* H0, const double* F, const double*
   return guarded(ctx, [&] {
     THCGW_REQUIRE(F && Sigma && nel, "thcgw_nelec: null pointer");
     hipStream_t s = S(stream);
+    check_dyson_g_inputs(ctx, "thcgw_nelec", H0, F, Sov, Sigma, s);
     c128* Sw = sigma_to_w(ctx, C(Sigma), s);
     *nel = nelec_at(ctx, C(H0), C(F), C(Sov), Sw, mu, s);
   });
@@ -1481,6 +1546,7 @@ int thcgw_update_mu(thcgw_ctx* ctx, const double* H0, const double* F, const dou
     THCGW_REQUIRE(F && Sigma && mu_out, "thcgw_update_mu: null pointer");
     THCGW_REQUIRE(mu_tol > 0.0, "thcgw_update_mu: mu_tol must be positive");
     hipStream_t s = S(stream);
+    check_dyson_g_inputs(ctx, "thcgw_update_mu", H0, F, Sov, Sigma, s);
     c128* Sw = sigma_to_w(ctx, C(Sigma), s);
     auto N = [&](double mu) { return nelec_at(ctx, C(H0), C(F), C(Sov), Sw, mu, s); };
     // scf_common.cpp:147-202: bracket in steps of 0.2, then bisect to |nel - target| < mu_tol
@@ -2050,6 +2116,9 @@ int thcgw_lri_dyson(thcgw_lri* L, double* PiW, const double* V, int krylov, int 
   return guard_err(L->err, ctx->device, [&] {
     THCGW_REQUIRE(PiW && V, "thcgw_lri_dyson: null pointer");
     hipStream_t s = S(stream);
+    check_finite(ctx, "thcgw_lri_dyson",
+                 {{PiW, static_cast<long>(ctx->nw_half) * ctx->nk * ctx->M, "Pi_wqPQ"}, {V, ctx->vq() * ctx->M, "V_qPQ"}},
+                 s);
     const int n = ctx->d.nmu, nq = ctx->nk, r = L->r, ncol = n + r;
     const long M = ctx->M, R2 = static_cast<long>(r) * r, QS = static_cast<long>(n) * r,
                MB = static_cast<long>(n) * ncol, US = static_cast<long>(r) * ncol;

@@ -61,6 +61,7 @@ std::vector<hsize_t> dims_of(hid_t f, const char* name) {
   return dims;
 }
 
+// Reads the whole dataset into out, which holds exactly `count` values of mem_type.
 template <class T>
 void read_all(hid_t f, const char* name, hid_t mem_type, T* out, size_t count) {
   Hid d(H5Dopen2(f, name, H5P_DEFAULT), H5Dclose);
@@ -166,6 +167,25 @@ int thcgw_eri_read_info(const char* path, thcgw_eri_info* info, char* err, int e
     r.has_Y = has(f, "Y_collocation_matrix") ? 1 : 0;
     if ((r.x_first != r.y_first || r.x_last != r.y_last) && !r.has_Y)
       fail("X and Y orbital ranges differ but Y_collocation_matrix is missing");
+    if (r.has_Y) {
+      const std::vector<hsize_t> Y = complex_dims(f, "Y_collocation_matrix");
+      if (Y.size() != 4 || static_cast<int>(Y[0]) != r.ns_npol || static_cast<int>(Y[1]) != r.nkpts ||
+          static_cast<int>(Y[2]) != r.Np || static_cast<int>(Y[3]) != r.y_last - r.y_first)
+        fail("Y_collocation_matrix must be (ns*npol, nk, Np, y_last - y_first) like collocation_matrix");
+    }
+    r.nkpts_file = 0;
+    if (has(f, "kpts")) {
+      const std::vector<hsize_t> k = dims_of(f, "kpts");
+      if (k.size() != 2 || k[1] != 3 || static_cast<int>(k[0]) != r.nkpts)
+        fail("kpts must be (nk, 3) with the nk of collocation_matrix");
+      r.nkpts_file = r.nkpts;
+    }
+    r.nqpts = 0;
+    if (has(f, "qpts")) {
+      const std::vector<hsize_t> q = dims_of(f, "qpts");
+      if (q.size() != 2 || q[1] != 3) fail("qpts must be (nq, 3)");
+      r.nqpts = static_cast<int>(q[0]);
+    }
     const std::vector<hsize_t> V = complex_dims(f, "coulomb_matrix");
     if (V.size() != 3 || static_cast<int>(V[1]) != r.Np || static_cast<int>(V[2]) != r.Np)
       fail("coulomb_matrix must be (nq_ibz, Np, Np)");
@@ -182,41 +202,45 @@ int thcgw_eri_read_info(const char* path, thcgw_eri_info* info, char* err, int e
   });
 }
 
-int thcgw_eri_read_X(const char* path, int which, double* X, char* err, int errlen) {
+// Values (doubles) of the dataset `name`; fails unless it equals the caller's capacity.
+size_t sized_for(hid_t f, const char* name, long capacity, bool complex) {
+  const std::vector<hsize_t> d = complex ? complex_dims(f, name) : dims_of(f, name);
+  size_t n = complex ? 2 : 1;
+  for (hsize_t v : d) n *= static_cast<size_t>(v);
+  if (capacity < 0 || static_cast<size_t>(capacity) != n)
+    fail(std::string("dataset '") + name + "' holds " + std::to_string(n) + " doubles, the output buffer " +
+         std::to_string(capacity));
+  return n;
+}
+
+int thcgw_eri_read_X(const char* path, int which, double* X, long capacity, char* err, int errlen) {
   return guard(err, errlen, [&] {
     if (!X) fail("null output");
     Hid f(open_ro(path), H5Fclose);
     const char* name = which ? "Y_collocation_matrix" : "collocation_matrix";
-    const std::vector<hsize_t> d = complex_dims(f, name);
-    size_t n = 2;
-    for (hsize_t v : d) n *= static_cast<size_t>(v);
-    read_all(f, name, H5T_NATIVE_DOUBLE, X, n);
+    read_all(f, name, H5T_NATIVE_DOUBLE, X, sized_for(f, name, capacity, true));
   });
 }
 
-int thcgw_eri_read_V(const char* path, double* V, char* err, int errlen) {
+int thcgw_eri_read_V(const char* path, double* V, long capacity, char* err, int errlen) {
   return guard(err, errlen, [&] {
     if (!V) fail("null output");
     Hid f(open_ro(path), H5Fclose);
-    const std::vector<hsize_t> d = complex_dims(f, "coulomb_matrix");
-    size_t n = 2;
-    for (hsize_t v : d) n *= static_cast<size_t>(v);
-    read_all(f, "coulomb_matrix", H5T_NATIVE_DOUBLE, V, n);
+    read_all(f, "coulomb_matrix", H5T_NATIVE_DOUBLE, V, sized_for(f, "coulomb_matrix", capacity, true));
   });
 }
 
-int thcgw_eri_read_kpts(const char* path, double* kpts, double* qpts, char* err, int errlen) {
+int thcgw_eri_read_kpts(const char* path, double* kpts, long kpts_capacity, double* qpts, long qpts_capacity,
+                        char* err, int errlen) {
   return guard(err, errlen, [&] {
     Hid f(open_ro(path), H5Fclose);
-    if (kpts) {
-      const std::vector<hsize_t> d = dims_of(f, "kpts");
-      if (d.size() != 2 || d[1] != 3) fail("kpts must be (nk, 3)");
-      read_all(f, "kpts", H5T_NATIVE_DOUBLE, kpts, static_cast<size_t>(d[0]) * 3);
-    }
-    if (qpts) {
-      const std::vector<hsize_t> d = dims_of(f, "qpts");
-      if (d.size() != 2 || d[1] != 3) fail("qpts must be (nq, 3)");
-      read_all(f, "qpts", H5T_NATIVE_DOUBLE, qpts, static_cast<size_t>(d[0]) * 3);
+    for (int w = 0; w < 2; ++w) {
+      double* out = w ? qpts : kpts;
+      const char* name = w ? "qpts" : "kpts";
+      if (!out) continue;
+      const std::vector<hsize_t> d = dims_of(f, name);
+      if (d.size() != 2 || d[1] != 3) fail(std::string(name) + " must be (n, 3)");
+      read_all(f, name, H5T_NATIVE_DOUBLE, out, sized_for(f, name, w ? qpts_capacity : kpts_capacity, false));
     }
   });
 }
@@ -240,7 +264,8 @@ int thcgw_eri_write(const char* path, const thcgw_eri_info* info, const double* 
     write_ds(f, "X_orbital_range", H5T_STD_I32LE, H5T_NATIVE_INT, {2}, xr, false);
     write_ds(f, "Y_orbital_range", H5T_STD_I32LE, H5T_NATIVE_INT, {2}, yr, false);
     if (kpts) write_ds(f, "kpts", H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, {hsize_t(r.nkpts), 3}, kpts, false);
-    if (qpts) write_ds(f, "qpts", H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, {hsize_t(r.nkpts), 3}, qpts, false);
+    if (qpts)
+      write_ds(f, "qpts", H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, {hsize_t(r.nqpts > 0 ? r.nqpts : r.nkpts), 3}, qpts, false);
     const int nki = r.nkpts_ibz > 0 ? r.nkpts_ibz : r.nkpts;
     write_ds(f, "nkpts_ibz", H5T_STD_I32LE, H5T_NATIVE_INT, {}, &nki, false);
     write_ds(f, "nqpts_ibz", H5T_STD_I32LE, H5T_NATIVE_INT, {}, &r.nqpts_ibz, false);

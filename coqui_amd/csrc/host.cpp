@@ -1,9 +1,12 @@
 // Host-side parts of libthcgw: IR table handling, RPA energy assembly and the
 // quasiparticle equation (Thiele Pade continuation + bisection).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "thcgw_host.hpp"
 #include "thcgw_internal.hpp"
@@ -339,19 +342,30 @@ void solve_qp_diag(const Tables& tab, int nk, int no, const cd* sig_t, const cd*
   const std::vector<int> idx = pade_fit_indices(nw, nfit);
   std::vector<cd> zf(nfit);
   for (int i = 0; i < nfit; ++i) zf[i] = cd(0.0, double(tab.wn_f[idx[i]]) * M_PI / tab.beta);  // IAFT.hpp:55-57
-#pragma omp parallel for schedule(dynamic)
-  for (long I = 0; I < dim; ++I) {
+  // One state per task, handed out by an atomic counter to plain std::threads.  The
+  // library carries no OpenMP runtime of its own: a host process that also loads an
+  // OpenMP-parallel library (torch's libgomp, the test oracle) then runs a single one.
+  std::atomic<long> next{0};
+  auto work = [&] {
     std::vector<cd> Af(nfit);
-    for (int i = 0; i < nfit; ++i) {  // Sigma(iw_n) = Twt_ff Sigma(tau) at the fit frequencies
-      cd acc = 0.0;
-      const int w = idx[i];
-      for (int t = 0; t < nt; ++t) acc += tab.Twt_ff[size_t(w) * nt + t] * sig_t[size_t(t) * dim + I];
-      Af[i] = acc;
+    for (long I = next++; I < dim; I = next++) {
+      for (int i = 0; i < nfit; ++i) {  // Sigma(iw_n) = Twt_ff Sigma(tau) at the fit frequencies
+        cd acc = 0.0;
+        const int w = idx[i];
+        for (int t = 0; t < nt; ++t) acc += tab.Twt_ff[size_t(w) * nt + t] * sig_t[size_t(t) * dim + I];
+        Af[i] = acc;
+      }
+      Thiele p;
+      p.fit(zf, Af);
+      E_out[I] = bisection(vhf[I].real(), p, mu, E0[I], tol, eta);
     }
-    Thiele p;
-    p.fit(zf, Af);
-    E_out[I] = bisection(vhf[I].real(), p, mu, E0[I], tol, eta);
-  }
+  };
+  const long hw = std::max(1L, static_cast<long>(std::thread::hardware_concurrency()));
+  const int nth = static_cast<int>(std::min({hw, 16L, std::max(1L, dim / 64)}));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+  work();
+  for (std::thread& th : pool) th.join();
 }
 
 }  // namespace host

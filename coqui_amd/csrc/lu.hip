@@ -66,7 +66,10 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(c128* A, long sA, int l
   for (int c = 0; c < W; ++c) {
     if (c < w) {
       const int jc = j0 + c;
-      // 1. pivot: largest |Re|+|Im| in M-column jc over M-rows >= jc, first index on ties
+      // 1. pivot: largest |Re|+|Im| in M-column jc over M-rows >= jc, first index on ties.
+      //    A thread's candidate index starts at its first row >= jc, so a column whose
+      //    entries are all NaN (no comparison succeeds) still pivots on row jc, as LAPACK's
+      //    izamax does; threads with no row >= jc keep INT_MAX and lose every reduction.
       double best = -1.0;
       int bidx = INT_MAX;
 #pragma unroll
@@ -74,6 +77,7 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(c128* A, long sA, int l
         const int r = j0 + t + s * PT;
         if (r >= jc && r < n) {
           const double val = fabs(v[s][c].x) + fabs(v[s][c].y);
+          if (bidx == INT_MAX) bidx = r;
           if (val > best) {
             best = val;
             bidx = r;
@@ -131,8 +135,10 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(c128* A, long sA, int l
             for (int cc = 0; cc < W; ++cc) v[s][cc] = jcol[cc];
         }
       }
-      // 3. scale below the pivot and update the rest of the panel
+      // 3. scale below the pivot and update the rest of the panel.  A zero or non-finite
+      //    pivot is reported in info (first such column + 1), as a singular matrix.
       const c128 d = pcol[c];
+      if (!(isfinite(d.x) && isfinite(d.y)) && first_zero == 0) first_zero = jc + 1;
       if (d.x != 0.0 || d.y != 0.0) {
         const c128 inv = crecip(d);
 #pragma unroll
@@ -179,7 +185,7 @@ __global__ __launch_bounds__(256) void laswp_kernel(c128* A, long sA, int lda, i
   const int* piv = ipiv + static_cast<long>(b) * n;
   for (int k = k0; k < k1; ++k) {
     const int p = piv[k];
-    if (p != k) {
+    if (p != k && p >= 0 && p < n) {  // the panel writes pivots in [k, n); the range check guards the read
       const c128 x = Ar[k];
       Ar[k] = Ar[p];
       Ar[p] = x;

@@ -109,6 +109,7 @@ __global__ __launch_bounds__(T256) void dyson_finish_kernel(const c128* Z, long 
     for (int r = 0; r < n; ++r) idx[r] = r;
     for (int c = n - 1; c >= 0; --c) {
       const int p = piv[c];
+      if (p < 0 || p >= n) continue;  // the panel writes pivots in [c, n); out-of-range ones are skipped
       const int tmp = idx[c];
       idx[c] = idx[p];
       idx[p] = tmp;
@@ -376,6 +377,22 @@ __global__ __launch_bounds__(T256) void trev_diff_kernel(const c128* __restrict_
   }
 }
 
+
+// Input guard: out[block] = number of NaN / infinite doubles among x[0, n) in the block's
+// grid-stride share (blocks read 256 consecutive doubles per step).
+__global__ __launch_bounds__(T256) void nonfinite_kernel(const double* __restrict__ x, long n, int* __restrict__ out) {
+  __shared__ int r[T256];
+  int c = 0;
+  for (long e = static_cast<long>(blockIdx.x) * T256 + threadIdx.x; e < n; e += static_cast<long>(gridDim.x) * T256)
+    c += isfinite(x[e]) ? 0 : 1;
+  r[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = T256 / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) r[threadIdx.x] += r[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = r[0];
+}
 
 // ---- long-wavelength (Gygi) correction ----
 // out[t * nq + q] = sum_P c[q][P] sum_Q W[q][t][P][Q] conj(c[q][Q])   (g0_div_utils.hpp:199-209)
@@ -699,6 +716,11 @@ void qsym_diff(hipStream_t s, const c128* A, int n, int nq, const int* src, cons
 
 void herm_diff(hipStream_t s, const c128* A, int n, long nb, int nbx, double* out) {
   hipLaunchKernelGGL(herm_diff_kernel, dim3(nbx), dim3(T256), 0, s, A, n, nb, out);
+  THCGW_HIP(hipGetLastError());
+}
+
+void nonfinite_count(hipStream_t s, const double* x, long n, int nbx, int* out) {
+  hipLaunchKernelGGL(nonfinite_kernel, dim3(nbx), dim3(T256), 0, s, x, n, out);
   THCGW_HIP(hipGetLastError());
 }
 

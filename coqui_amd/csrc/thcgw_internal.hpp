@@ -208,6 +208,8 @@ void scatter_ir(hipStream_t s, const c128* S, long M, int n0, int nq, const int*
 void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const int* infos, int n0, int nq,
                       const int* qir, const int* mq, int nqi, int cnt, c128* tr, double* ld, int* info);
 void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out);
+// out[b] (b < nbx) = number of non-finite doubles among x[0, n) in block b's grid-stride share
+void nonfinite_count(hipStream_t s, const double* x, long n, int nbx, int* out);
 // per block b' < nbx: max |A_ij - conj(A_ji)| and max |A_ij| over nb matrices of order n
 void herm_diff(hipStream_t s, const c128* A, int n, long nb, int nbx, double* out);
 // q-point symmetry map: W[w][q] = ph_q conj(ph_q)^T o W^[w][src q](perm_q, perm_q) for the q with

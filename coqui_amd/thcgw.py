@@ -31,7 +31,8 @@ EXPORTS = (
     "thcgw_set_qsym",
 )
 
-ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation"}
+ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation",
+          6: "non-finite input"}
 
 
 class ThcgwError(RuntimeError):
@@ -145,6 +146,29 @@ def _np_ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _squeezed(shape):
+    return tuple(int(d) for d in shape if int(d) != 1)
+
+
+def check_array(a, shape, name, dtype="complex128", device=True):
+    """Raises ValueError unless `a` (torch tensor or numpy array) has exactly `shape` (axes
+    of length 1 may be added or dropped), the dtype and C-contiguity, and, for torch
+    tensors with device=True, lives on the GPU.  The C ABI takes raw pointers, so this is
+    the only place a mis-sized array is caught before the library reads or writes it."""
+    shp = tuple(int(d) for d in a.shape)
+    if _squeezed(shp) != _squeezed(shape):
+        raise ValueError("%s: expected shape %s, got %s" % (name, tuple(int(d) for d in shape), shp))
+    dt = str(a.dtype).replace("torch.", "")
+    if dt != dtype:
+        raise ValueError("%s: expected dtype %s, got %s" % (name, dtype, dt))
+    contiguous = a.is_contiguous() if hasattr(a, "is_contiguous") else a.flags["C_CONTIGUOUS"]
+    if not contiguous:
+        raise ValueError("%s: expected a C-contiguous array" % name)
+    if device and hasattr(a, "is_cuda") and not a.is_cuda:
+        raise ValueError("%s: expected a GPU tensor, got one on %s" % (name, a.device))
+    return a
+
+
 class ThcGW:
     """One system shape on the current GPU (thcgw_create / thcgw_destroy)."""
 
@@ -192,15 +216,50 @@ class ThcGW:
         s = stream if stream is not None else self._torch.cuda.current_stream()
         return ctypes.c_void_p(s.cuda_stream)
 
-    def _dev(self, a):
+    def _dev(self, a, shape, name, dtype="complex128"):
+        """`a` as a GPU tensor of `shape`: device tensors are checked (check_array) and used
+        in place, host arrays are checked and copied."""
         t = self._torch
         if isinstance(a, t.Tensor):
-            assert a.is_cuda and a.dtype == t.complex128 and a.is_contiguous(), "expected a contiguous complex128 GPU tensor"
-            return a
-        return t.from_numpy(np.ascontiguousarray(a, np.complex128)).cuda()
+            return check_array(a, shape, name, dtype)
+        h = np.ascontiguousarray(a, np.dtype(dtype))
+        check_array(h, shape, name, dtype)
+        return t.from_numpy(h).cuda()
+
+    def _out(self, a, shape, name):
+        """A caller-given output tensor, checked, or a new one of `shape`."""
+        return self._empty(*shape) if a is None else check_array(a, shape, name)
 
     def _empty(self, *shape):
         return self._torch.empty(shape, dtype=self._torch.complex128, device="cuda")
+
+    # array shapes of the C ABI (include/thcgw.h)
+    def _g(self):
+        return (self.nt, 1, self.nk, self.no, self.no)
+
+    def _x(self):
+        return (1, self.nk, self.nmu, self.no)
+
+    def _v(self):
+        return (self.nk, self.nmu, self.nmu)
+
+    def _kij(self):
+        return (1, self.nk, self.no, self.no)
+
+    def _pit(self):
+        return (self.nt_half, self.nk, self.nmu, self.nmu)
+
+    def _piw(self):
+        return (self.nw_half, self.nk, self.nmu, self.nmu)
+
+    def _wqt(self):
+        return (self.nk, self.nt_half, self.nmu, self.nmu)
+
+    def _nb(self, O, per, name):
+        n = int(np.prod(O.shape))
+        if n == 0 or n % per:
+            raise ValueError("%s: %d elements is not a multiple of %d" % (name, n, per))
+        return n // per
 
     @property
     def nt(self):
@@ -245,56 +304,56 @@ class ThcGW:
 
     # ------------------------------------------------------------------ stages
     def update_G(self, C, E, mu=0.0, out=None, stream=None):
-        C = self._dev(C)
-        E = self._torch.as_tensor(np.ascontiguousarray(E, np.float64)).cuda() if not isinstance(E, self._torch.Tensor) else E
-        G = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        C = self._dev(C, (self.nk, self.no, self.no), "C")
+        E = self._dev(E, (self.nk, self.no), "E", "float64")
+        G = self._out(out, self._g(), "out")
         self._check(self.lib.thcgw_update_G(self.h, _ptr(C), _ptr(E), ctypes.c_double(mu), _ptr(G), self._stream(stream)))
         return G
 
     def primary_to_aux(self, X, O, stream=None):
-        X, O = self._dev(X), self._dev(O)
-        nb = O.numel() // (self.nk * self.no * self.no)
+        nb = self._nb(O, self.nk * self.no * self.no, "O")
+        X, O = self._dev(X, self._x(), "X"), self._dev(O, (nb, self.nk, self.no, self.no), "O")
         out = self._empty(nb, self.nk, self.nmu, self.nmu)
         self._check(self.lib.thcgw_primary_to_aux(self.h, nb, _ptr(X), _ptr(O), _ptr(out), self._stream(stream)))
         return out
 
     def aux_to_primary(self, X, O, stream=None):
-        X, O = self._dev(X), self._dev(O)
-        nb = O.numel() // (self.nk * self.nmu * self.nmu)
+        nb = self._nb(O, self.nk * self.nmu * self.nmu, "O")
+        X, O = self._dev(X, self._x(), "X"), self._dev(O, (nb, self.nk, self.nmu, self.nmu), "O")
         out = self._empty(nb, self.nk, self.no, self.no)
         self._check(self.lib.thcgw_aux_to_primary(self.h, nb, _ptr(X), _ptr(O), _ptr(out), self._stream(stream)))
         return out
 
     def pi(self, G, X, out=None, stream=None):
-        G, X = self._dev(G), self._dev(X)
-        P = out if out is not None else self._empty(self.nt_half, self.nk, self.nmu, self.nmu)
+        G, X = self._dev(G, self._g(), "G"), self._dev(X, self._x(), "X")
+        P = self._out(out, self._pit(), "out")
         self._check(self.lib.thcgw_pi(self.h, _ptr(G), _ptr(X), _ptr(P), self._stream(stream)))
         return P
 
     def pi_kspace(self, G, X, out=None, stream=None):
         """k-space chi0 (rpa_pi.icc:182-336): Pi^k_PQ(q) = Pi^R_QP(-q) for Hermitian G."""
-        G, X = self._dev(G), self._dev(X)
-        P = out if out is not None else self._empty(self.nt_half, self.nk, self.nmu, self.nmu)
+        G, X = self._dev(G, self._g(), "G"), self._dev(X, self._x(), "X")
+        P = self._out(out, self._pit(), "out")
         self._check(self.lib.thcgw_pi_kspace(self.h, _ptr(G), _ptr(X), _ptr(P), self._stream(stream)))
         return P
 
     def sigma_kspace(self, G, W_qt, X, hermitize=True, out=None, stream=None):
         """k-space self-energy (thc_gw.icc:342-443)."""
-        G, W_qt, X = self._dev(G), self._dev(W_qt), self._dev(X)
-        S = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        G, W_qt, X = self._dev(G, self._g(), "G"), self._dev(W_qt, self._wqt(), "W_qt"), self._dev(X, self._x(), "X")
+        S = self._out(out, self._g(), "out")
         self._check(self.lib.thcgw_sigma_kspace(self.h, _ptr(G), _ptr(W_qt), _ptr(X), _ptr(S), int(bool(hermitize)),
                                                 self._stream(stream)))
         return S
 
     def tau_to_w(self, Pi_t, out=None, stream=None):
-        Pi_t = self._dev(Pi_t)
-        P = out if out is not None else self._empty(self.nw_half, self.nk, self.nmu, self.nmu)
+        Pi_t = self._dev(Pi_t, self._pit(), "Pi_t")
+        P = self._out(out, self._piw(), "out")
         self._check(self.lib.thcgw_tau_to_w(self.h, _ptr(Pi_t), _ptr(P), self._stream(stream)))
         return P
 
     def dyson_w(self, PiW, V, with_terms=False, stream=None):
         """In place: PiW <- W - V.  Returns (trace, logdet) host arrays when with_terms."""
-        PiW, V = self._dev(PiW), self._dev(V)
+        PiW, V = self._dev(PiW, self._piw(), "PiW"), self._dev(V, self._v(), "V")
         n = self.nw_half * self.nk
         tr = np.zeros(n, np.complex128) if with_terms else None
         ld = np.zeros(n, np.float64) if with_terms else None
@@ -303,35 +362,37 @@ class ThcGW:
         return (tr, ld) if with_terms else PiW
 
     def w_to_tau(self, W_w, out=None, stream=None):
-        W_w = self._dev(W_w)
-        W = out if out is not None else self._empty(self.nk, self.nt_half, self.nmu, self.nmu)
+        W_w = self._dev(W_w, self._piw(), "W_w")
+        W = self._out(out, self._wqt(), "out")
         self._check(self.lib.thcgw_w_to_tau(self.h, _ptr(W_w), _ptr(W), self._stream(stream)))
         return W
 
     def sigma(self, G, W_qt, X, hermitize=True, out=None, stream=None):
-        G, W_qt, X = self._dev(G), self._dev(W_qt), self._dev(X)
-        S = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        G, W_qt, X = self._dev(G, self._g(), "G"), self._dev(W_qt, self._wqt(), "W_qt"), self._dev(X, self._x(), "X")
+        S = self._out(out, self._g(), "out")
         self._check(self.lib.thcgw_sigma(self.h, _ptr(G), _ptr(W_qt), _ptr(X), _ptr(S), int(bool(hermitize)),
                                          self._stream(stream)))
         return S
 
     def hermitize(self, A, stream=None):
-        A = self._dev(A)
-        n = A.shape[-1]
+        if len(A.shape) < 2 or A.shape[-1] != A.shape[-2]:
+            raise ValueError("A: expected a stack of square matrices, got shape %s" % (tuple(A.shape),))
+        n = int(A.shape[-1])
+        A = self._dev(A, (self._nb(A, n * n, "A"), n, n), "A")
         self._check(self.lib.thcgw_hermitize(self.h, A.numel() // (n * n), n, _ptr(A), self._stream(stream)))
         return A
 
     def rpa_ec(self, Pi_w, V, stream=None):
-        Pi_w, V = self._dev(Pi_w), self._dev(V)
+        Pi_w, V = self._dev(Pi_w, self._piw(), "Pi_w"), self._dev(V, self._v(), "V")
         ec = ctypes.c_double()
         self._check(self.lib.thcgw_rpa_ec(self.h, _ptr(Pi_w), _ptr(V), ctypes.byref(ec), self._stream(stream)))
         return ec.value
 
     def gw_iteration(self, G, X, V, with_ec=True, hermitize=True, Sigma=None, W_qt=None, stream=None):
         """update_w + evaluate: returns (Sigma_tskij, W_qtPQ, Ec or None)."""
-        G, X, V = self._dev(G), self._dev(X), self._dev(V)
-        S = Sigma if Sigma is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
-        W = W_qt if W_qt is not None else self._empty(self.nk, self.nt_half, self.nmu, self.nmu)
+        G, X, V = self._dev(G, self._g(), "G"), self._dev(X, self._x(), "X"), self._dev(V, self._v(), "V")
+        S = self._out(Sigma, self._g(), "Sigma")
+        W = self._out(W_qt, self._wqt(), "W_qt")
         ec = ctypes.c_double()
         self._check(self.lib.thcgw_gw_iteration(self.h, _ptr(G), _ptr(X), _ptr(V), _ptr(S), _ptr(W),
                                                 ctypes.byref(ec) if with_ec else None, int(bool(hermitize)),
@@ -339,8 +400,9 @@ class ThcGW:
         return S, W, (ec.value if with_ec else None)
 
     def solve_qp(self, Sigma, C, Vhf, E0, mu=0.0, nfit=18, eta=1e-4, tol=1e-8, stream=None):
-        Sigma, C, Vhf = self._dev(Sigma), self._dev(C), self._dev(Vhf)
-        E0 = np.ascontiguousarray(E0, np.float64)
+        Sigma, C = self._dev(Sigma, self._g(), "Sigma"), self._dev(C, (self.nk, self.no, self.no), "C")
+        Vhf = self._dev(Vhf, (self.nk, self.no, self.no), "Vhf")
+        E0 = check_array(np.ascontiguousarray(E0, np.float64), (self.nk, self.no), "E0", "float64")
         E = np.zeros_like(E0)
         self._check(self.lib.thcgw_solve_qp(self.h, _ptr(Sigma), _ptr(C), _ptr(Vhf), _np_ptr(E0), ctypes.c_double(mu),
                                             int(nfit), ctypes.c_double(eta), ctypes.c_double(tol), _np_ptr(E),
@@ -349,26 +411,30 @@ class ThcGW:
 
     def hf(self, Dm, X, V, hartree=True, exchange=True, out=None, stream=None):
         """THC Hartree-Fock potential F_skij (1, nk, no, no) of the density matrix Dm_skij."""
-        Dm, X, V = self._dev(Dm), self._dev(X), self._dev(V)
-        F = out if out is not None else self._empty(1, self.nk, self.no, self.no)
+        Dm, X, V = self._dev(Dm, self._kij(), "Dm"), self._dev(X, self._x(), "X"), self._dev(V, self._v(), "V")
+        F = self._out(out, self._kij(), "out")
         self._check(self.lib.thcgw_hf(self.h, _ptr(Dm), _ptr(X), _ptr(V), int(bool(hartree)), int(bool(exchange)),
                                       _ptr(F), self._stream(stream)))
         return F
 
-    def _opt_dev(self, a):
-        return None if a is None else self._dev(a)
+    def _opt_dev(self, a, shape, name):
+        return None if a is None else self._dev(a, shape, name)
+
+    def _dyson_g_args(self, F, Sigma, H0, S):
+        return (self._dev(F, self._kij(), "F"), self._dev(Sigma, self._g(), "Sigma"),
+                self._opt_dev(H0, self._kij(), "H0"), self._opt_dev(S, self._kij(), "S"))
 
     def dyson_g(self, F, Sigma, mu, H0=None, S=None, want_dm=True, out=None, stream=None):
         """Matsubara Dyson equation for G: returns (G_tskij, Dm_skij or None)."""
-        F, Sigma, H0, S = self._dev(F), self._dev(Sigma), self._opt_dev(H0), self._opt_dev(S)
-        G = out if out is not None else self._empty(self.nt, 1, self.nk, self.no, self.no)
+        F, Sigma, H0, S = self._dyson_g_args(F, Sigma, H0, S)
+        G = self._out(out, self._g(), "out")
         Dm = self._empty(1, self.nk, self.no, self.no) if want_dm else None
         self._check(self.lib.thcgw_dyson_g(self.h, _ptr(H0), _ptr(F), _ptr(S), _ptr(Sigma), ctypes.c_double(mu),
                                            _ptr(G), _ptr(Dm), self._stream(stream)))
         return G, Dm
 
     def nelec(self, F, Sigma, mu, H0=None, S=None, stream=None):
-        F, Sigma, H0, S = self._dev(F), self._dev(Sigma), self._opt_dev(H0), self._opt_dev(S)
+        F, Sigma, H0, S = self._dyson_g_args(F, Sigma, H0, S)
         n = ctypes.c_double()
         self._check(self.lib.thcgw_nelec(self.h, _ptr(H0), _ptr(F), _ptr(S), _ptr(Sigma), ctypes.c_double(mu),
                                          ctypes.byref(n), self._stream(stream)))
@@ -376,7 +442,7 @@ class ThcGW:
 
     def update_mu(self, F, Sigma, nel_target, old_mu, mu_tol=1e-9, H0=None, S=None, stream=None):
         """Chemical potential for nel_target: returns (mu, nelec)."""
-        F, Sigma, H0, S = self._dev(F), self._dev(Sigma), self._opt_dev(H0), self._opt_dev(S)
+        F, Sigma, H0, S = self._dyson_g_args(F, Sigma, H0, S)
         mu, n = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.thcgw_update_mu(self.h, _ptr(H0), _ptr(F), _ptr(S), _ptr(Sigma),
                                              ctypes.c_double(nel_target), ctypes.c_double(old_mu),
@@ -388,8 +454,8 @@ class ThcGW:
 
     def eps_inv_head(self, W_qt, chi_bar, qpts_cart, volume, div_treatment="gygi_extrplt", stream=None):
         """Head of the inverse dielectric function: returns (eps_inv_tq (nt_half, nk), eps_inv_t (nt_half))."""
-        W_qt, chi_bar = self._dev(W_qt), self._dev(chi_bar)
-        q = np.ascontiguousarray(qpts_cart, np.float64)
+        W_qt, chi_bar = self._dev(W_qt, self._wqt(), "W_qt"), self._dev(chi_bar, (self.nk, self.nmu), "chi_bar")
+        q = check_array(np.ascontiguousarray(qpts_cart, np.float64), (self.nk, 3), "qpts_cart", "float64")
         tq = np.zeros((self.nt_half, self.nk), np.complex128)
         t0 = np.zeros(self.nt_half, np.complex128)
         mode = self.DIV[div_treatment] if isinstance(div_treatment, str) else int(div_treatment)
@@ -400,8 +466,9 @@ class ThcGW:
     def sigma_div_correction(self, Sigma, G, X, chi_head, madelung, eps_inv_t, div_treatment="gygi_extrplt",
                              stream=None):
         """In place: Sigma += the long-wavelength correction (thc_gw.icc:446-526)."""
-        Sigma, G, X, chi_head = self._dev(Sigma), self._dev(G), self._dev(X), self._dev(chi_head)
-        e = np.ascontiguousarray(eps_inv_t, np.complex128)
+        Sigma, G = self._dev(Sigma, self._g(), "Sigma"), self._dev(G, self._g(), "G")
+        X, chi_head = self._dev(X, self._x(), "X"), self._dev(chi_head, (self.nmu,), "chi_head")
+        e = check_array(np.ascontiguousarray(eps_inv_t, np.complex128), (self.nt_half,), "eps_inv_t")
         mode = self.DIV[div_treatment] if isinstance(div_treatment, str) else int(div_treatment)
         self._check(self.lib.thcgw_sigma_div_correction(self.h, _ptr(Sigma), _ptr(G), _ptr(X), _ptr(chi_head),
                                                         ctypes.c_double(madelung), _np_ptr(e), mode,
@@ -409,7 +476,7 @@ class ThcGW:
         return Sigma
 
     def gm_energy(self, G, Sigma, stream=None):
-        G, Sigma = self._dev(G), self._dev(Sigma)
+        G, Sigma = self._dev(G, self._g(), "G"), self._dev(Sigma, self._g(), "Sigma")
         e = ctypes.c_double()
         self._check(self.lib.thcgw_gm_energy(self.h, _ptr(G), _ptr(Sigma), ctypes.byref(e), self._stream(stream)))
         return e.value
@@ -454,10 +521,14 @@ class ThcBatch:
         Returns the list of Ec values (or None)."""
         n = len(self.ctxs)
         for lst in (G, X, V, Sigma, W_qt):
-            assert len(lst) == n, "one array per system"
-        for c, g, x, v, sg, w in zip(self.ctxs, G, X, V, Sigma, W_qt):
-            for t in (g, x, v, sg, w):
-                c._dev(t)  # type and layout checks (no copy for device tensors)
+            if len(lst) != n:
+                raise ValueError("one array per system: %d systems, %d arrays" % (n, len(lst)))
+        for i, (c, g, x, v, sg, w) in enumerate(zip(self.ctxs, G, X, V, Sigma, W_qt)):
+            for t, shape, name in ((g, c._g(), "G"), (x, c._x(), "X"), (v, c._v(), "V"), (sg, c._g(), "Sigma"),
+                                   (w, c._wqt(), "W_qt")):
+                if not isinstance(t, self._torch.Tensor):
+                    raise ValueError("system %d %s: expected a GPU tensor" % (i, name))
+                check_array(t, shape, "system %d %s" % (i, name))
         mk = lambda lst: (ctypes.c_void_p * n)(*[t.data_ptr() for t in lst])
         ec = (ctypes.c_double * n)() if with_ec else None
         s = stream if stream is not None else self._torch.cuda.current_stream()
@@ -509,7 +580,7 @@ class LearnedDyson:
     def dyson(self, PiW, V, krylov=True, record=False, keep_state=False, stream=None):
         """In place PiW <- W - V; returns the call's statistics (dict).  record stores
         training pairs; keep_state leaves the previous-iteration state untouched."""
-        PiW, V = self.ctx._dev(PiW), self.ctx._dev(V)
+        PiW, V = self.ctx._dev(PiW, self.ctx._piw(), "PiW"), self.ctx._dev(V, self.ctx._v(), "V")
         st = LriStats()
         flags = int(bool(record)) | (2 if keep_state else 0)
         self._check(self.lib.thcgw_lri_dyson(self.h, _ptr(PiW), _ptr(V), int(bool(krylov)), flags,
@@ -579,36 +650,42 @@ class Mlp:
         return h
 
     def set_params(self, flat):
-        flat = np.ascontiguousarray(flat, np.float32)
-        assert flat.size == self.n_params
+        flat = check_array(np.ascontiguousarray(flat, np.float32), (self.n_params,), "flat", "float32")
         self._check(self.lib.thcgw_mlp_set_params(self.h, _np_ptr(flat)))
 
-    def _f32(self, a):
+    def _f32(self, a, shape, name):
         t = self._torch
         if isinstance(a, t.Tensor):
-            assert a.is_cuda and a.dtype == t.float32 and a.is_contiguous()
-            return a
-        return t.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+            return check_array(a, shape, name, "float32")
+        h = check_array(np.ascontiguousarray(a, np.float32), shape, name, "float32")
+        return t.from_numpy(h).cuda()
+
+    def _rows(self, X):
+        if len(X.shape) != 2:
+            raise ValueError("X: expected (batch, %d), got shape %s" % (self.n_in, tuple(X.shape)))
+        return int(X.shape[0])
 
     def forward(self, X):
-        X = self._f32(X)
+        X = self._f32(X, (self._rows(X), self.n_in), "X")
         Y = self._torch.empty((X.shape[0], self.n_out), dtype=self._torch.float32, device="cuda")
         self._check(self.lib.thcgw_mlp_forward(self.h, int(X.shape[0]), _ptr(X), _ptr(Y), self._stream()))
         return Y
 
     def backward(self, X, dY):
         """Gradient of sum(Y * dY) w.r.t. the flat parameters, after forward(X)."""
-        X, dY = self._f32(X), self._f32(dY)
+        b = self._rows(X)
+        X, dY = self._f32(X, (b, self.n_in), "X"), self._f32(dY, (b, self.n_out), "dY")
         g = self._torch.empty(self.n_params, dtype=self._torch.float32, device="cuda")
         self._check(self.lib.thcgw_mlp_backward(self.h, int(X.shape[0]), _ptr(X), _ptr(dY), _ptr(g), self._stream()))
         return g
 
     def adam_step(self, grad, lr):
-        grad = self._f32(grad)
+        grad = self._f32(grad, (self.n_params,), "grad")
         self._check(self.lib.thcgw_mlp_adam_step(self.h, _ptr(grad), ctypes.c_float(lr), self._stream()))
 
     def train_mse(self, X, T, lr):
-        X, T = self._f32(X), self._f32(T)
+        b = self._rows(X)
+        X, T = self._f32(X, (b, self.n_in), "X"), self._f32(T, (b, self.n_out), "T")
         loss = ctypes.c_float()
         self._check(self.lib.thcgw_mlp_train_mse(self.h, int(X.shape[0]), _ptr(X), _ptr(T), ctypes.c_float(lr),
                                                  ctypes.byref(loss), self._stream()))

@@ -16,6 +16,10 @@
  *     the others return once the work is enqueued unless stated otherwise;
  *   - every function returns THCGW_OK (0) or an error code; the message is
  *     available from thcgw_last_error(ctx).  No C++ exception crosses the ABI;
+ *   - the entry points that factorise (thcgw_dyson_w, thcgw_rpa_ec, thcgw_gw_iteration,
+ *     thcgw_batch_gw_iteration, thcgw_dyson_g, thcgw_nelec, thcgw_update_mu,
+ *     thcgw_lri_dyson) first check their input arrays and return THCGW_ERR_NONFINITE,
+ *     before any factorisation, when one holds a NaN or an infinity (one read pass);
  *   - ns = 1 (no spin polarisation), npol = 1, no k-point symmetry: k-points form
  *     a Gamma-centred kgrid[0] x kgrid[1] x kgrid[2] mesh in C order with
  *     lattice vectors = identity, and q-points = k-points.
@@ -43,6 +47,7 @@ extern "C" {
 #define THCGW_ERR_SINGULAR 3
 #define THCGW_ERR_UNSUPPORTED 4
 #define THCGW_ERR_ALLOC 5
+#define THCGW_ERR_NONFINITE 6 /* an input array holds a NaN or an infinity */
 
 typedef struct thcgw_ctx thcgw_ctx;
 

@@ -141,3 +141,52 @@ def test_consistency_checks(tmp_path):
             read_thc_eri(str(bad))
     with pytest.raises(IOError, match="cannot open"):
         read_thc_eri(str(tmp_path / "missing.h5"))
+
+
+def test_qpts_rows_differ_from_kpts(tmp_path):
+    """qpts with nq != nk rows round-trip into a buffer sized from qpts, not from nk."""
+    s, kc = sample()
+    qc = np.vstack([kc, kc[:3] + 0.5])  # nq = nk + 3
+    p = tmp_path / "nq.eri.h5"
+    write_thc_eri(str(p), s.X, s.V, kpts=kc, qpts=qc, nbnd=s.no)
+    r = read_thc_eri(str(p))
+    assert r["info"]["nqpts"] == s.nk + 3 and np.array_equal(r["qpts"], qc) and np.array_equal(r["kpts"], kc)
+
+
+def test_read_refuses_wrong_capacity(tmp_path):
+    """Every read function refuses an output buffer whose size differs from the dataset."""
+    import ctypes
+
+    s, kc = sample()
+    p = tmp_path / "cap.eri.h5"
+    write_thc_eri(str(p), s.X, s.V, kpts=kc, qpts=kc, nbnd=s.no)
+    lib = load_eri_library()
+    b = os.fsencode(str(p))
+    big = np.zeros(2 * s.X.size + 64)
+    ptr = big.ctypes.data_as(ctypes.c_void_p)
+    err = ctypes.create_string_buffer(256)
+    assert lib.thcgw_eri_read_X(b, 0, ptr, 2 * s.X.size - 2, err, 256) != 0 and b"output buffer" in err.value
+    assert lib.thcgw_eri_read_X(b, 0, ptr, 2 * s.X.size, err, 256) == 0
+    vb = np.zeros(2 * s.V.size + 2)
+    assert lib.thcgw_eri_read_V(b, vb.ctypes.data_as(ctypes.c_void_p), 2 * s.V.size + 2, err, 256) != 0
+    kb = np.zeros(3 * s.nk)
+    assert lib.thcgw_eri_read_kpts(b, None, 0, kb.ctypes.data_as(ctypes.c_void_p), 3 * s.nk - 3, err, 256) != 0
+    assert lib.thcgw_eri_read_kpts(b, None, 0, kb.ctypes.data_as(ctypes.c_void_p), 3 * s.nk, err, 256) == 0
+    with pytest.raises(ValueError):
+        write_thc_eri(str(tmp_path / "bad.eri.h5"), s.X, s.V, kpts=kc[:-1])
+
+
+@need_h5py
+def test_mis_shaped_y_is_refused(tmp_path):
+    """read_info checks Y_collocation_matrix against (ns*npol, nk, Np, ny) before any read."""
+    s, kc = sample(no=6)
+    good = tmp_path / "y.eri.h5"
+    write_thc_eri(str(good), s.X, s.V, x_range=(0, 6), Y=s.X[..., 2:5], y_range=(2, 5), nbnd=6)
+    bad = tmp_path / "y_bad.eri.h5"
+    bad.write_bytes(good.read_bytes())
+    edit = ('y = f["Y_collocation_matrix"][()]; del f["Y_collocation_matrix"]; '
+            'f["Y_collocation_matrix"] = np.concatenate([y, y], axis=2); f["Y_collocation_matrix"].attrs["__complex__"] = "1"')
+    subprocess.run([H5PY_PYTHON, "-c", "import h5py, numpy as np, sys\nwith h5py.File(sys.argv[1], 'a') as f:\n    "
+                    + edit, str(bad)], check=True)
+    with pytest.raises(IOError, match="Y_collocation_matrix"):
+        read_thc_eri(str(bad))
