@@ -1,11 +1,13 @@
 """Benchmark: wall time of one THC G0W0 iteration on one MI355X.
 
-Workload (default): N_k = 64 (4x4x4), N_orb = 52, N_mu = 800, beta = 1000, complex128
-throughout, on the shipped IR table with 72 tau points (the nearest to 80; lambda = 1e3,
-eps = 1e-15).  One step = chi0 -> tau_to_w -> Dyson W (+ RPA Ec from the same factors)
--> w_to_tau -> Sigma (+ hermitize), with G(tau,k), X and V resident in HBM.
+Workload (default, --config k64-mu1600): N_k = 64 (4x4x4), N_orb = 104, N_mu = 1600,
+N_tau = 80 (IR table lambda = 2e3, eps = 2e-15, built by coqui_amd/data/ir/gen_ir_tables.py),
+beta = 1000, complex128 inputs and outputs.  One step = chi0 -> tau_to_w -> Dyson W (+ RPA
+Ec from the same factors) -> w_to_tau -> Sigma (+ hermitize), with G(tau,k), X and V
+resident in HBM.
 
-Forms (--forms, the first is the headline value):
+Forms (--forms; --headline names the one reported as `value`; errors are taken against
+the plain form, which runs first):
   plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm);
   hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
@@ -19,13 +21,13 @@ Forms (--forms, the first is the headline value):
              --trev, which draws the system time-reversal symmetric);
   *_c4       the form with the Dyson step on the q irreducible under a fourfold rotation
              (needs --c4, which draws the system symmetric under it).
-Every further form is reported with its speedup over the first and its errors against it
-(W at q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
+Every form is reported with its speedup over plain and its errors against plain (W at
+q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
 
-Prints one JSON line.  Also reported: the complex-FP64 GEMM kernel (the dominant kernel of
-the plain form) measured with HIP events on its launch stream during the timed steps, and
-the CPU oracle timed on a bounded sample of the same workload and extrapolated to one
-iteration.
+Prints one JSON line.  Also reported: the dominant GEMM class of the headline form measured
+with HIP events on its launch stream during the timed steps (roofline), and the CPU
+baseline: the reference algorithm on the host's OpenBLAS (oracle/blas_port.py) timed on a
+bounded sample of the same workload and extrapolated to one iteration.
 """
 import argparse
 import faulthandler
@@ -42,35 +44,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    # name: kgrid, N_orb, N_mu, IR table
-    "k64-mu800": ((4, 4, 4), 52, 800, "1e3.1e-15"),
-    "k8-mu200": ((2, 2, 2), 26, 200, "1e2.1e-15"),
-    "k64-mu1600": ((4, 4, 4), 104, 1600, "1e3.1e-15"),
+    # name: kgrid, N_orb, N_mu, IR table (N_tau)
+    "k8-mu200": ((2, 2, 2), 26, 200, "1e2.1e-15"),  # 40 tau points
+    "k64-mu800": ((4, 4, 4), 52, 800, "2e3.2e-15"),  # 80 tau points
+    "k64-mu1600": ((4, 4, 4), 104, 1600, "2e3.2e-15"),  # 80 tau points
     "batch24": None,  # batched: see batch_systems()
-    "scgw-k64-mu1200": ((4, 4, 4), 78, 1200, "1e3.1e-15"),  # self-consistent GW, learned Dyson form: see run_scgw()
+    "scgw-k64-mu1200": ((4, 4, 4), 78, 1200, "2e3.2e-15"),  # self-consistent GW, learned Dyson form: see run_scgw()
 }
-# The batched workload asks for N_tau = 60; the nearest shipped table (1e5.1e-06, 61 points)
-# makes the Pade continuation of the QP equation move by tens of meV under 1e-7 relative
-# noise in Sigma, so the batch uses the 72-point lambda = 1e3, eps = 1e-15 table of the
-# single-system workloads.
-BATCH_TABLE = "1e3.1e-15"
+# The batched workload: 60 tau points (lambda = 1e3, eps = 1e-12).
+BATCH_TABLE = "1e3.1e-12"
 
 
 def batch_systems(n=24):
-    """The batched workload: n systems with N_orb ~ U{16..96}, N_mu = 8 N_orb and a k-grid of
-    2^3, 3^3 or 4^3 points, drawn from the generator's base seed; system i uses seed id i."""
-    from coqui_amd.system import BASE_SEED
+    """The batched workload (coqui_amd.system.batch_specs)."""
+    from coqui_amd.system import batch_specs
 
-    rng = np.random.default_rng(BASE_SEED)
-    out = []
-    for i in range(n):
-        no = int(rng.integers(16, 97))
-        n1 = int(rng.choice([2, 3, 4]))
-        out.append(((n1, n1, n1), no, 8 * no, i))
-    return out
+    return batch_specs(n)
 
 
 # form: (flag 6 Dyson algorithm, flag 7 precision, symmetry: 0 none, 1 time reversal (flag 9),
@@ -82,6 +75,16 @@ FORMS = {"plain": (0, 0, 0), "hermitian": (1, 0, 0), "mixed": (1, 1, 0), "mixed_
          "plain_c4": (0, 0, 2), "hermitian_c4": (1, 0, 2), "mixed_c4": (1, 1, 2),
          "resolvent_c4": (2, 0, 2), "mixed_resolvent_c4": (2, 1, 2)}
 HA_PER_MEV = 1.0 / 27211.386245988
+# GEMM profile class -> kernel template as rocprofv3 names it (64x64 tiles)
+OPS = {"N": 0, "T": 1, "C": 2}
+
+
+def gemm_kernel_name(cls):
+    if cls.endswith("32"):
+        return "gemm_kernel<float, c64, c64, c64, %d, %d, 64, 64, false>" % (OPS[cls[0]], OPS[cls[1]])
+    if cls.endswith("bf"):
+        return "gemm_bf16x6_kernel<c64, c64, c64, %d, %d, false>" % (OPS[cls[0]], OPS[cls[1]])
+    return "gemm_kernel<double, c128, c128, c128, %d, %d, 64, 64, false>" % (OPS[cls[0]], OPS[cls[1]])
 
 
 def log(*a):
@@ -94,25 +97,19 @@ def cpu_baseline_child(path):
 
     z = np.load(path)
     kgrid = tuple(int(v) for v in z["kgrid"])
-
-    class _S:
-        pass
-
-    s = _S()
-    s.kgrid, s.nk, s.nmu, s.X, s.V_host2 = kgrid, int(np.prod(kgrid)), int(z["nmu"]), z["X"], z["V2"]
     ia = IAFT.from_table(float(z["beta"]), str(z["table"]))
-    print(json.dumps(cpu_baseline(s, ia, z["G_pair"], "FP64, all host threads")), flush=True)
+    print(json.dumps(cpu_baseline(kgrid, z["X"], z["V2"], ia, z["G_pair"])), flush=True)
 
 
 def run_cpu_baseline(s, ia, G_pair):
-    """The oracle sample in a child process: a host-side crash cannot take the GPU run down."""
+    """The CPU sample in a child process that never initialises the GPU, so that host BLAS
+    threads do not share the process with the HIP runtime's."""
     import subprocess
     import tempfile
 
     with tempfile.TemporaryDirectory() as d:
         path = os.path.join(d, "cpu_inputs.npz")
-        np.savez(path, kgrid=np.array(s.kgrid), nmu=s.nmu, X=s.X, V2=s.V_host2, G_pair=G_pair, beta=ia.beta,
-                 table=ia.name)
+        np.savez(path, kgrid=np.array(s.kgrid), X=s.X, V2=s.V_host2, G_pair=G_pair, beta=ia.beta, table=ia.name)
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", path], capture_output=True,
                            text=True, timeout=900)
         if r.returncode != 0:
@@ -121,16 +118,20 @@ def run_cpu_baseline(s, ia, G_pair):
         return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def cpu_baseline(s, ia, G_host_pair, budget_note, reps=2):
-    """Oracle (C++/OpenMP FP64 restatement of the reference) on a bounded sample, each part
-    timed `reps` times (mean): one tau slice of chi0, one tau pair of Sigma, 4 reps Dyson and
-    4 reps Ec matrices, 2 reps q blocks of each imaginary-axis transform; extrapolated to a
-    full iteration."""
-    from oracle import oracle
+def cpu_baseline(kgrid, X, V2, ia, G_pair, reps=2):
+    """The reference algorithm on the host's BLAS/LAPACK (oracle/blas_port.py: ZGEMM
+    projections and dense k <-> R transforms, getrf + getri Dyson, getrf determinants,
+    ZGEMM tau <-> iW) on a bounded sample, each part timed `reps` times (mean): one tau slice
+    of chi0, one tau pair of Sigma, 2 Dyson and 2 Ec matrices, one q block of the
+    imaginary-axis transforms; extrapolated to a full iteration."""
+    import threadpoolctl
+
+    from oracle import blas_port, oracle
 
     tab = oracle.load_ir(ia.beta, None, table=ia.name)
-    nk, nmu = s.nk, s.nmu
-    M = nmu * nmu
+    Tp, Tm = blas_port.fold_tables(tab)
+    nk = int(np.prod(kgrid))
+    nmu = X.shape[1]
     nth, nwh = ia.nt_half, ia.nw_half
     rng = np.random.default_rng(0)
     t = {}
@@ -147,28 +148,28 @@ def cpu_baseline(s, ia, G_host_pair, budget_note, reps=2):
         t[key] = dt / count
         return out
 
-    Pi1 = timed("pi_slice", lambda: oracle.pi_rspace(G_host_pair, s.X, s.kgrid), reps)  # nt = 2 -> one tau slice
-    Wqt = np.ascontiguousarray(np.swapaxes(Pi1, 0, 1))  # (nk, 1, nmu, nmu): any W of the right shape
-    timed("sigma_pair", lambda: oracle.sigma_rspace(G_host_pair, s.X, Wqt, s.kgrid), reps)  # both passes of one pair
+    Pi1 = timed("pi_slice", lambda: blas_port.pi_pair(G_pair, X, kgrid), reps)
+    timed("sigma_pair", lambda: blas_port.sigma_pair(G_pair, X, Pi1, kgrid), reps)  # any W of the right shape
+    del Pi1
     B = rng.standard_normal((1, 2, nmu, nmu)) + 1j * rng.standard_normal((1, 2, nmu, nmu))
     Pi = -(B @ np.conj(np.swapaxes(B, -1, -2))) / nmu * 0.05
-    V2 = s.V_host2
-    timed("dyson_2", lambda: oracle.dyson(Pi, V2), 2 * reps)
-    tab2 = dict(tab, nw_b=1, Ttw_bb=np.ascontiguousarray(tab["Ttw_bb"][:, :1]))  # one iW_n: two (q, iW) matrices
-    timed("ec_2", lambda: oracle.rpa_ec(tab2, Pi, V2), 2 * reps)
-    Xt = rng.standard_normal((nth, 1, M)) + 0j
-    timed("ft_q", lambda: oracle.w_to_tau_phsym(tab, oracle.tau_to_w_phsym(tab, Xt)), 2 * reps)
+    timed("dyson_2", lambda: blas_port.dyson(Pi, V2), reps)
+    timed("ec_2", lambda: blas_port.rpa_terms(Pi, V2), reps)
+    Xt = rng.standard_normal((nth, nmu * nmu)) + 0j
+    timed("ft_q", lambda: blas_port.w_to_tau(Tm, blas_port.tau_to_w(Tp, Xt)), reps)
     total = (t["pi_slice"] * nth + (t["dyson_2"] + t["ec_2"]) / 2.0 * nwh * nk + t["ft_q"] * nk
              + t["sigma_pair"] * nth)
+    threads = max([p.get("num_threads", 1) for p in threadpoolctl.threadpool_info()
+                   if p.get("internal_api") == "openblas"] or [1])
     return {
         "value": total,
         "unit": "s",
-        "cores": oracle.num_threads(),
+        "cores": int(threads),
         "kind": "port",
-        "sample": ("oracle on one tau slice of chi0 and one tau pair of Sigma (%d times each), 2 Dyson + 2 Ec "
-                   "(q, iW) matrices and one q block of tau<->iW (%d times each), timed %.1f s in all and "
-                   "extrapolated from the mean times to %d tau slices, %d (q, iW) matrices, %d q blocks; %s"
-                   % (reps, 2 * reps, sample_s, nth, nwh * nk, nk, budget_note)),
+        "sample": ("reference algorithm on the host's OpenBLAS (numpy/scipy, %d threads): one tau slice of chi0, "
+                   "one tau pair of Sigma, 2 Dyson + 2 Ec (q, iW) matrices and one q block of tau<->iW, each "
+                   "timed %d times (%.1f s in all), extrapolated from the mean times to %d tau slices, %d (q, iW) "
+                   "matrices and %d q blocks" % (threads, reps, sample_s, nth, nwh * nk, nk)),
         "sample_times_s": {k: round(v, 3) for k, v in t.items()},
     }
 
@@ -178,9 +179,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="k64-mu800", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="k64-mu1600", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--breakdown", action="store_true", help="also time each stage separately")
+    ap.add_argument("--breakdown", action="store_true", help="also time each stage of the headline form separately")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--gemm-tile", type=int, default=0, help="GEMM tile shape: 0 auto, 1 64x64, 2 128x64, 3 128x128")
     ap.add_argument("--gemm-tile32", type=int, default=0, help="FP32 GEMM tile shape: 0 64x64, 1 128x64, 2 128x128")
@@ -196,9 +197,10 @@ def main():
                     help="scgw-k64-mu1200: also solve the final state's Dyson step with every preconditioner (0 to skip)")
     ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
                     help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
-    ap.add_argument("--forms", default="plain,mixed_resolvent,mixed",
-                    help="comma-separated forms to time; the first is the headline (plain, hermitian, mixed, "
-                         "mixed_bf16, resolvent, mixed_resolvent, and their _trev / _c4 variants)")
+    ap.add_argument("--forms", default="plain,mixed_resolvent",
+                    help="comma-separated forms to time (plain, hermitian, mixed, mixed_bf16, resolvent, "
+                         "mixed_resolvent, and their _trev / _c4 variants); plain runs first")
+    ap.add_argument("--headline", default="mixed_resolvent", help="the form reported as value")
     args = ap.parse_args()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     for f in forms:
@@ -208,6 +210,10 @@ def main():
             ap.error("form %r needs --trev" % f)
         if FORMS[f][2] == 2 and not args.c4:
             ap.error("form %r needs --c4" % f)
+    if args.headline not in forms:
+        forms.append(args.headline) if args.headline in FORMS else ap.error("unknown headline %r" % args.headline)
+    if "plain" in forms:  # the reference of every error runs first
+        forms = ["plain"] + [f for f in forms if f != "plain"]
     if args.cpu_child:
         cpu_baseline_child(args.cpu_child)
         return
@@ -226,7 +232,8 @@ def main():
         return run_scgw(args)
     kgrid, no, nmu, table = CONFIGS[args.config]
     beta = 1000.0
-    log("bench: config %s kgrid=%s no=%d nmu=%d table=%s" % (args.config, kgrid, no, nmu, table))
+    log("bench: config %s kgrid=%s no=%d nmu=%d table=%s forms=%s headline=%s" % (
+        args.config, kgrid, no, nmu, table, forms, args.headline))
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
     if args.c4:
@@ -235,11 +242,16 @@ def main():
     else:
         s = make_system(kgrid, no, nmu, system_id=0, with_v=False, trev=args.trev)
     ia = IAFT.from_table(beta, table)
-    ctx = ThcGW(kgrid, no, nmu, ia)
-    ctx.set_flag(5, args.gemm_tile)
-    ctx.set_flag(8, args.gemm_tile32)
-    ctx.set_flag(3, args.dyson_chunk)
-    ctx.set_flag(2, args.pair_chunk)
+
+    def new_ctx():
+        c = ThcGW(kgrid, no, nmu, ia)
+        c.set_flag(5, args.gemm_tile)
+        c.set_flag(8, args.gemm_tile32)
+        c.set_flag(3, args.dyson_chunk)
+        c.set_flag(2, args.pair_chunk)
+        return c
+
+    ctx = new_ctx()
     if s.V is not None:
         Vd = torch.from_numpy(s.V).cuda()
     else:
@@ -253,6 +265,8 @@ def main():
     nth, nwh = ia.nt_half, ia.nw_half
     Sig = torch.empty((ia.nt, 1, s.nk, no, no), dtype=torch.complex128, device="cuda")
     Wqt = torch.empty((s.nk, nth, nmu, nmu), dtype=torch.complex128, device="cuda")
+    H = torch.from_numpy(s.H).cuda()
+    Cd = torch.from_numpy(s.C).cuda()
     torch.cuda.synchronize()
     log("bench: inputs ready in %.1f s" % (time.perf_counter() - t0))
 
@@ -268,22 +282,15 @@ def main():
         else:
             ctx.set_qsym()
 
-    first_form = [True]
-
-    def run_form(form, steps, warmup):
+    def run_form(form, steps, warmup, first):
         # every form after the first gets a fresh context, so that no form runs with the
         # workspace of the forms before it held on the device (the Dyson chunk size follows
         # the free memory)
         nonlocal ctx
-        if not first_form[0]:
+        if not first:
             ctx.close()
             torch.cuda.synchronize()
-            ctx = ThcGW(kgrid, no, nmu, ia)
-            ctx.set_flag(5, args.gemm_tile)
-            ctx.set_flag(8, args.gemm_tile32)
-            ctx.set_flag(3, args.dyson_chunk)
-            ctx.set_flag(2, args.pair_chunk)
-        first_form[0] = False
+            ctx = new_ctx()
         set_form(form)
         ec = None
         for i in range(warmup):
@@ -305,46 +312,49 @@ def main():
         ctx.set_flag(4, 0)
         return ev0.elapsed_time(ev1) / steps, wall, ec, ctx.profile_read()
 
-    ms_step, wall, ec, prof = run_form(forms[0], args.steps, args.warmup)
-    log("bench: %s %.1f ms per step" % (forms[0], ms_step))
-    g_ms, g_fl, g_n = prof.get("NN", (0.0, 0.0, 0))  # Dyson, LU and triangular-solve GEMMs
-    a_ms = sum(v[0] for k, v in prof.items() if len(k) == 2)
-    a_fl = sum(v[1] for k, v in prof.items() if len(k) == 2)
-    achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
-    gemm_classes = {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
-                        "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
-                    for k, v in prof.items()}
+    def classes(prof):
+        return {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
+                    "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
+                for k, v in prof.items()}
 
-    other_forms = {}
-    if len(forms) > 1:
-        # reference results of the headline form: Sigma, W at q = 0, Ec, QP energies
-        H = torch.from_numpy(s.H).cuda()
-        Cd = torch.from_numpy(s.C).cuda()
-        Sig_ref = Sig.clone()
-        W0_ref = Wqt[0].clone()
-        ec_ref = ec
-        E_ref = ctx.solve_qp(Sig_ref, Cd, H, s.E, mu=s.mu)
-        for form in forms[1:]:
-            f_ms, f_wall, f_ec, f_prof = run_form(form, args.steps, 1)
-            E_f = ctx.solve_qp(Sig, Cd, H, s.E, mu=s.mu)
+    results, ref = {}, None
+    for i, form in enumerate(forms):
+        f_ms, f_wall, f_ec, f_prof = run_form(form, args.steps, args.warmup, i == 0)
+        E_f = ctx.solve_qp(Sig, Cd, H, s.E, mu=s.mu)
+        r = {"ms_per_step": round(f_ms, 3), "wall_s_timed": round(f_wall, 3), "Ec_Ha": f_ec, "prof": f_prof,
+             "gemm_classes": classes(f_prof)}
+        if ref is None:
+            ref = {"form": form, "ms": f_ms, "Sigma": Sig.clone(), "W0": Wqt[0].clone(), "ec": f_ec, "E": E_f}
+        else:
             rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
-            other_forms[form] = {
-                "ms_per_step": round(f_ms, 3),
-                "speedup_vs_%s" % forms[0]: round(ms_step / f_ms, 3),
-                "err_W_q0_rel": rel(Wqt[0], W0_ref),
-                "err_Sigma_rel": rel(Sig, Sig_ref),
-                "err_Ec_Ha": abs(f_ec - ec_ref),
-                "err_qp_meV": float(np.max(np.abs(E_f - E_ref)) / HA_PER_MEV),
-                "err_qp_meV_median": float(np.median(np.abs(E_f - E_ref)) / HA_PER_MEV),
-                "qp_states_above_1meV": int(np.sum(np.abs(E_f - E_ref) > HA_PER_MEV)),
-                "gemm_classes": {k: {"ms_per_step": round(v[0] / args.steps, 2), "launches": v[2],
-                                     "tflops": round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[0] > 0 else 0.0}
-                                 for k, v in f_prof.items()},
-            }
-            log("bench: %s %.1f ms per step, %s" % (form, f_ms, json.dumps(
-                {k: v for k, v in other_forms[form].items() if k != "gemm_classes"})))
-        del Sig_ref, W0_ref, H, Cd
-        set_form(forms[0])
+            r.update({
+                "speedup_vs_%s" % ref["form"]: round(ref["ms"] / f_ms, 3),
+                "err_W_q0_rel": rel(Wqt[0], ref["W0"]),
+                "err_Sigma_rel": rel(Sig, ref["Sigma"]),
+                "err_Ec_Ha": abs(f_ec - ref["ec"]),
+                "err_qp_meV": float(np.max(np.abs(E_f - ref["E"])) / HA_PER_MEV),
+                "err_qp_meV_median": float(np.median(np.abs(E_f - ref["E"])) / HA_PER_MEV),
+                "qp_states_above_1meV": int(np.sum(np.abs(E_f - ref["E"]) > HA_PER_MEV)),
+            })
+        results[form] = r
+        log("bench: %s %.1f ms per step %s" % (form, f_ms, json.dumps(
+            {k: v for k, v in r.items() if k not in ("gemm_classes", "prof")})))
+    del ref["Sigma"], ref["W0"]
+
+    hd = results[args.headline]
+    ms_step = hd["ms_per_step"]
+    # roofline: the headline form's GEMM class with the most kernel time
+    prof = hd["prof"]
+    cls = max(prof, key=lambda k: prof[k][0])
+    g_ms, g_fl, g_n, g_by = prof[cls]
+    peak = FP32_PEAK_TFLOPS if cls.endswith("32") else FP64_PEAK_TFLOPS
+    achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    a_ms = sum(v[0] for v in prof.values())
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "gemm_%s_%s_%s_traffic.json" % (cls, args.config, args.headline))
+    if os.path.exists(tpath):  # HBM bytes per dispatch from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+        traffic = json.load(open(tpath))["traffic_bytes_per_dispatch"]
+        traffic_src = os.path.relpath(tpath, ROOT)
 
     if args.copy_calib:
         xa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -357,6 +367,7 @@ def main():
 
     stages = None
     if args.breakdown:
+        set_form(args.headline)
         stages = {}
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
         Pit = Wqt
@@ -380,17 +391,12 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("bench: CPU oracle sample ...")
+        log("bench: CPU baseline sample ...")
         cpu = run_cpu_baseline(s, ia, G_pair)
         if cpu:
             log("bench: CPU baseline %.1f s per iteration (extrapolated)" % cpu["value"])
 
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "zgemm_nn_%s_traffic.json" % args.config)
-    if os.path.exists(tpath):  # HBM bytes per dispatch from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes
-        traffic = json.load(open(tpath))["traffic_bytes_per_dispatch"]
-        traffic_src = os.path.relpath(tpath, ROOT)
-
+    plain = results.get("plain")
     out = {
         "metric": "THC-GW iteration wall-time (s) + QP error (meV) vs FP64; speedup vs plain GPU",
         "value": round(ms_step / 1000.0, 6),
@@ -402,29 +408,31 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": ("c128" if FORMS[args.headline][1] == 0 else
+                  "c128 in/out; FP32 projections and k-grid transforms, FP64 tau<->iW and Dyson"),
         "data": "synthetic (seeded generator, random H(k)/X/V; no checkpoint)",
-        "config": {"workload": "%s %s THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % (args.config, forms[0]),
+        "config": {"workload": "%s %s THC G0W0 iteration (chi0, W, Sigma, RPA Ec)" % (args.config, args.headline),
                    "kgrid": list(kgrid), "nk": s.nk, "n_orb": no, "n_mu": nmu, "n_tau": ia.nt,
                    "n_iw_b": ia.nw_b, "ir_table": table, "beta": beta, "parallelism": "single GPU",
                    "time_reversal_symmetric_system": bool(args.trev), "c4_symmetric_system": bool(args.c4)},
-        "Ec_Ha": ec,
-        "form": forms[0],
-        "speedup_vs_plain": 1.0 if forms[0] == "plain" else None,
-        "qp_error_meV": 0.0 if forms[0] == "plain" else None,
-        "forms": other_forms,
-        "roofline": {"kernel": "zgemm_kernel<0,0> (complex FP64 GEMM, N,N)", "bound": "mfma",
-                     "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "traffic_source": traffic_src,
+        "form": args.headline,
+        "Ec_Ha": hd["Ec_Ha"],
+        "plain_s_per_iteration": round(plain["ms_per_step"] / 1000.0, 6) if plain else None,
+        "speedup_vs_plain": (1.0 if args.headline == "plain" else hd.get("speedup_vs_plain")) if plain else None,
+        "qp_error_meV": (0.0 if args.headline == "plain" else hd.get("err_qp_meV")) if plain else None,
+        "ec_error_Ha": (0.0 if args.headline == "plain" else hd.get("err_Ec_Ha")) if plain else None,
+        "forms": {k: {kk: vv for kk, vv in v.items() if kk != "prof"} for k, v in results.items()},
+        "roofline": {"kernel": "%s (GEMM class %s of the %s form)" % (gemm_kernel_name(cls), cls, args.headline),
+                     "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": round(g_by / max(g_n, 1)),
                      "flops_per_launch": round(g_fl / max(g_n, 1)),
                      "launches": g_n, "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
                      "share_of_step_time": round(g_ms / max(ms_step * args.steps, 1e-9), 4),
-                     "all_gemm_tflops": round(a_fl / (a_ms * 1e-3) / 1e12, 3) if a_ms > 0 else 0.0,
-                     "gemm_classes": gemm_classes},
+                     "all_gemm_share_of_step_time": round(a_ms / max(ms_step * args.steps, 1e-9), 4)},
         "cpu_baseline": cpu,
         "stages_ms": stages,
-        "wall_s_timed": round(wall, 3),
+        "wall_s_timed": hd["wall_s_timed"],
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -1306,11 +1306,11 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
   });
 }
 
-int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, long* launches) {
+int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, double* bytes, long* launches) {
   if (!ctx) return THCGW_ERR_ARG;
   return guarded(ctx, [&] {
     THCGW_REQUIRE(ms && flops && launches, "thcgw_profile_read: null pointer");
-    thcgw::gemm_profile_read(ms, flops, launches);
+    thcgw::gemm_profile_read(ms, flops, bytes, launches);
   });
 }
 

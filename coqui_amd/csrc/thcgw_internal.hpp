@@ -114,7 +114,7 @@ void gemm_profile_enable(bool on);
 void gemm_set_tile_mode(int mode);    // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
 void gemm_set_tile_mode32(int mode);  // FP32 products without FP64 folding: 0 = 64x64, 1 = 128x64, 2 = 128x128
 constexpr int GEMM_PROFILE_CLASSES = 17;
-void gemm_profile_read(double* ms, double* flops, long* launches);
+void gemm_profile_read(double* ms, double* flops, double* bytes, long* launches);
 
 // kfft.hip
 bool kfft_supported(const int* kgrid);

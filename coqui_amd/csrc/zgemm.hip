@@ -599,7 +599,7 @@ std::atomic<long long> g_flops_milli{0};
 // Optional per-launch timing with HIP events recorded on the launch stream.
 struct ProfRec {
   hipEvent_t a, b;
-  double flops;
+  double flops, bytes;
   int cls;
 };
 std::mutex g_prof_mu;
@@ -719,12 +719,24 @@ double model_flops(int M, int N, int K, int batch, int tri) {
   return flops;
 }
 
+// Modelled algorithmic HBM bytes: op(A), op(B) read once, C written once (and read once
+// when beta != 0), with element sizes ea, eb, ec.  A triangular output (GEMM_TRI_LOWER)
+// counts half of C; a triangular operand (the K range of a tile bounded by its row or
+// column) counts half of that operand.
+double model_bytes(int M, int N, int K, int batch, int tri, bool beta_nonzero, int ea, int eb, int ec) {
+  double a = double(M) * K * ea, b = double(K) * N * eb, c = double(M) * N * ec * (beta_nonzero ? 2 : 1);
+  if (tri & GEMM_TRI_LOWER) c *= 0.5;
+  if (tri & (GEMM_TRI_KFROM_ROW | GEMM_TRI_KTO_ROW)) a *= 0.5;
+  if (tri & (GEMM_TRI_KFROM_COL | GEMM_TRI_KTO_COL)) b *= 0.5;
+  return (a + b + c) * batch;
+}
+
 // Launch with optional event timing; cls indexes the profile classes (opA*3 + opB for
 // FP64, 9 + ... for FP32).
 template <class F>
-void timed(hipStream_t s, double flops, int cls, F&& launch_fn) {
+void timed(hipStream_t s, double flops, double bytes, int cls, F&& launch_fn) {
   std::unique_lock<std::mutex> lk(g_prof_mu, std::defer_lock);
-  ProfRec rec{nullptr, nullptr, flops, cls};
+  ProfRec rec{nullptr, nullptr, flops, bytes, cls};
   if (g_prof_on) {
     lk.lock();
     rec.a = pool_event();
@@ -766,7 +778,8 @@ void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
   if (M == 0 || N == 0 || batch == 0) return;
   const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri, hsub);
   const int nb = tile_shape(M, N, K);
-  timed(s, model_flops(M, N, K, batch, tri), opA * 3 + opB, [&] {
+  const bool bnz = beta.re != 0.0 || beta.im != 0.0;
+  timed(s, model_flops(M, N, K, batch, tri), model_bytes(M, N, K, batch, tri, bnz, 16, 16, 16), opA * 3 + opB, [&] {
     switch (opA * 3 + opB) {
       case 0: launch<OP_N, OP_N>(s, a, nb); break;
       case 1: launch<OP_N, OP_T>(s, a, nb); break;
@@ -790,7 +803,9 @@ void cgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, Bat
                 "cgemm: GEMM_HERM_SUB with a subtracted matrix writes complex<double>");
   const Args a = make_args(M, N, K, alpha, A, lda, B, ldb, beta, C, strideC, ldc, batch, gamma, tri, hsub);
   const int cls = ((tri & GEMM_SPLIT_BF16) ? 13 : 9) + (opA == OP_C ? 2 : 0) + (opB == OP_C ? 1 : 0);
-  timed(s, model_flops(M, N, K, batch, tri), cls, [&] {
+  const bool bnz = beta.re != 0.0 || beta.im != 0.0;
+  const int ea = ta == DT_C128 ? 16 : 8, eb = tb == DT_C128 ? 16 : 8, ec = tc == DT_C128 ? 16 : 8;
+  timed(s, model_flops(M, N, K, batch, tri), model_bytes(M, N, K, batch, tri, bnz, ea, eb, ec), cls, [&] {
     if (ta == DT_C128 && tb == DT_C128) dispatch_f32_c<c128, c128>(s, opA, opB, tc, a);
     else if (ta == DT_C128) dispatch_f32_c<c128, c64>(s, opA, opB, tc, a);
     else if (tb == DT_C128) dispatch_f32_c<c64, c128>(s, opA, opB, tc, a);
@@ -814,13 +829,14 @@ void gemm_profile_enable(bool on) {
 
 // Per class (FP64: opA*3 + opB, 0..8; FP32: 9 + 2*(opA == C) + (opB == C), 9..12; split
 // BF16: 13 + 2*(opA == C) + (opB == C), 13..16): summed
-// launch durations (ms), modelled flops and launch counts of the recorded launches; the
-// records are then dropped.
-void gemm_profile_read(double* ms, double* flops, long* launches) {
+// launch durations (ms), modelled flops, modelled algorithmic bytes (bytes may be null)
+// and launch counts of the recorded launches; the records are then dropped.
+void gemm_profile_read(double* ms, double* flops, double* bytes, long* launches) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   for (int c = 0; c < GEMM_PROFILE_CLASSES; ++c) {
     ms[c] = 0.0;
     flops[c] = 0.0;
+    if (bytes) bytes[c] = 0.0;
     launches[c] = 0;
   }
   for (const ProfRec& r : g_prof) {
@@ -829,6 +845,7 @@ void gemm_profile_read(double* ms, double* flops, long* launches) {
     THCGW_HIP(hipEventElapsedTime(&e, r.a, r.b));
     ms[r.cls] += e;
     flops[r.cls] += r.flops;
+    if (bytes) bytes[r.cls] += r.bytes;
     launches[r.cls] += 1;
     g_pool.push_back(r.a);
     g_pool.push_back(r.b);

@@ -171,3 +171,16 @@ def make_system(kgrid, no, nmu, system_id=0, with_v=True, trev=False, c4=False):
             V = np.matmul(Z, np.conj(np.swapaxes(Z, 1, 2))) / nmu
     return System(kgrid=kgrid, no=no, nmu=nmu, seed=seed, C=C, E=np.ascontiguousarray(E), X=np.ascontiguousarray(X),
                   V=V, Z=Z, qsym=qsym)
+
+
+def batch_specs(n=24):
+    """The batched workload: n systems with N_orb ~ U{16..96}, N_mu = 8 N_orb and a k-grid of
+    2^3, 3^3 or 4^3 points, drawn from the base seed; system i uses seed id i.  Returns
+    [(kgrid, N_orb, N_mu, system_id)]."""
+    rng = np.random.default_rng(BASE_SEED)
+    out = []
+    for i in range(n):
+        no = int(rng.integers(16, 97))
+        n1 = int(rng.choice([2, 3, 4]))
+        out.append(((n1, n1, n1), no, 8 * no, i))
+    return out

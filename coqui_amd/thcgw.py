@@ -105,7 +105,7 @@ def load_library(path=LIB_PATH):
                                          ctypes.POINTER(p), ctypes.POINTER(p), ctypes.POINTER(d), i, p]),
         "thcgw_workspace_bytes": (l, [p]),
         "thcgw_set_flag": (i, [p, i, l]),
-        "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
+        "thcgw_profile_read": (i, [p, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(l)]),
         "thcgw_eps_inv_head": (i, [p, p, p, p, d, i, p, p, p]),
         "thcgw_set_qsym": (i, [p, p, p, p, p]),
         "thcgw_sigma_div_correction": (i, [p, p, p, p, p, d, p, i, p]),
@@ -287,17 +287,18 @@ class ThcGW:
         self._check(self.lib.thcgw_set_qsym(self.h, *[None if a is None else _np_ptr(a) for a in self._qsym]))
 
     def profile_read(self):
-        """{class: (ms, flops, launches)} of the GEMM launches recorded since the last read
-        (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B) for FP64 products,
+        """{class: (ms, flops, launches, bytes)} of the GEMM launches recorded since the last
+        read (flag 4); class = "NN", "NT", ..., "CC" (op of A then op of B) for FP64 products,
         "NN32", "NC32", "CN32", "CC32" for FP32 products and "NNbf" ... "CCbf" for split-BF16
-        products."""
+        products; flops and bytes are the library's algorithmic models."""
         ms = (ctypes.c_double * 17)()
         fl = (ctypes.c_double * 17)()
+        by = (ctypes.c_double * 17)()
         n = (ctypes.c_long * 17)()
-        self._check(self.lib.thcgw_profile_read(self.h, ms, fl, n))
+        self._check(self.lib.thcgw_profile_read(self.h, ms, fl, by, n))
         names = ([a + b for a in "NTC" for b in "NTC"] + ["NN32", "NC32", "CN32", "CC32"]
                  + ["NNbf", "NCbf", "CNbf", "CCbf"])
-        return {names[c]: (ms[c], fl[c], n[c]) for c in range(17) if n[c]}
+        return {names[c]: (ms[c], fl[c], n[c], by[c]) for c in range(17) if n[c]}
 
     def workspace_bytes(self):
         return int(self.lib.thcgw_workspace_bytes(self.h))

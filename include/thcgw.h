@@ -369,13 +369,15 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           Inputs and outputs keep their complex<double> layouts.
  * thcgw_profile_read fills, for each of the 17 GEMM classes, the summed kernel time ms[c]
  * of the launches recorded since the last read, their modelled flops[c] (8 M N K per
- * complex product, reduced for triangular products) and launches[c], then drops the
+ * complex product, reduced for triangular products), their modelled algorithmic HBM
+ * bytes[c] (each operand read once, C written once and read once when beta != 0, halved
+ * for a triangular operand or output; bytes may be NULL) and launches[c], then drops the
  * records.  Classes 0..8 are the FP64 products c = opA*3 + opB (op 0 = N, 1 = T, 2 = C);
  * 9..12 the FP32 products NN, NC, CN, CC; 13..16 the split-BF16 products NN, NC, CN, CC.
  * It waits for the recorded launches to finish. */
 long thcgw_workspace_bytes(const thcgw_ctx* ctx);
 int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value);
-int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, long* launches);
+int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, double* bytes, long* launches);
 
 #ifdef __cplusplus
 }
