@@ -1472,11 +1472,21 @@ int thcgw_solve_qp(thcgw_ctx* ctx, const double* Sigma, const double* Cmo, const
     thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, no, kOne, thcgw::bp(C(Vhf), sm), no, thcgw::bp(C(Cmo), sm, nk),
                  no, kZero, T1, sm, no, nk);
     thcgw::mo_diag(s, T1, C(Cmo), dg + static_cast<long>(nt) * nk * no, nk, no, nk);
-    std::vector<std::complex<double>> hd(static_cast<size_t>(nt + 1) * nk * no);
-    THCGW_HIP(hipMemcpyAsync(hd.data(), dg, sizeof(c128) * hd.size(), hipMemcpyDeviceToHost, s));
+    // Pade fit frequencies (pade_driver.hpp:44-62) and the QP equation per state, on the device
+    const std::vector<int> idx = thcgw::host::pade_fit_indices(ctx->d.nw_f, nfit);
+    THCGW_REQUIRE(nfit <= thcgw::QP_MAX_FIT, "thcgw_solve_qp: nfit above 64 is not supported");
+    thcgw::QpFit fit{};
+    fit.nfit = nfit;
+    for (int i = 0; i < nfit; ++i) {
+      fit.w[i] = idx[i];
+      fit.zim[i] = double(ctx->tab.wn_f[idx[i]]) * M_PI / ctx->d.beta;  // IAFT.hpp:55-57
+    }
+    const long dim = static_cast<long>(nk) * no;
+    double* dE = ctx->buf<double>("qp_e", static_cast<size_t>(2 * dim));
+    THCGW_HIP(hipMemcpyAsync(dE, E0, sizeof(double) * dim, hipMemcpyHostToDevice, s));
+    thcgw::qp_solve(s, dg, dg + static_cast<long>(nt) * dim, dE, ctx->d_Twt_ff, fit, nt, dim, mu, eta, tol, dE + dim);
+    THCGW_HIP(hipMemcpyAsync(E_out, dE + dim, sizeof(double) * dim, hipMemcpyDeviceToHost, s));
     THCGW_HIP(hipStreamSynchronize(s));
-    thcgw::host::solve_qp_diag(ctx->tab, nk, no, hd.data(), hd.data() + static_cast<size_t>(nt) * nk * no, E0, mu,
-                               nfit, eta, tol, E_out);
   });
 }
 

@@ -1,11 +1,9 @@
-// Host-side parts of libthcgw: IR table handling, RPA energy assembly and the
-// quasiparticle equation (Thiele Pade continuation + bisection).
+// Host-side parts of libthcgw: IR table handling, RPA energy assembly and the Pade
+// fit-frequency selection of the quasiparticle equation (solved on the device, qp.hip).
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "thcgw_host.hpp"
@@ -198,174 +196,6 @@ std::vector<int> pade_fit_indices(int niw, int nfit) {
   }
   THCGW_REQUIRE(static_cast<int>(idx.size()) == nfit, "Pade: fit-frequency selection size mismatch");
   return idx;
-}
-
-namespace {
-
-// Double-double arithmetic (Hida, Li & Bailey, "Algorithms for quad-double precision
-// floating point arithmetic", ARITH-15, 2001): a value is hi + lo with |lo| <= ulp(hi)/2.
-struct dd {
-  double hi, lo;
-};
-inline dd two_sum(double a, double b) {
-  const double s = a + b, bb = s - a;
-  return {s, (a - (s - bb)) + (b - bb)};
-}
-inline dd quick_two_sum(double a, double b) {
-  const double s = a + b;
-  return {s, b - (s - a)};
-}
-inline dd two_prod(double a, double b) {
-  const double p = a * b;
-  return {p, std::fma(a, b, -p)};
-}
-inline dd operator+(dd x, dd y) {
-  dd s = two_sum(x.hi, y.hi), t = two_sum(x.lo, y.lo);
-  s.lo += t.hi;
-  s = quick_two_sum(s.hi, s.lo);
-  s.lo += t.lo;
-  return quick_two_sum(s.hi, s.lo);
-}
-inline dd operator-(dd x) { return {-x.hi, -x.lo}; }
-inline dd operator-(dd x, dd y) { return x + (-y); }
-inline dd operator*(dd x, dd y) {
-  dd p = two_prod(x.hi, y.hi);
-  p.lo += x.hi * y.lo + x.lo * y.hi;
-  return quick_two_sum(p.hi, p.lo);
-}
-inline dd operator*(dd x, double y) {
-  dd p = two_prod(x.hi, y);
-  p.lo += x.lo * y;
-  return quick_two_sum(p.hi, p.lo);
-}
-inline dd operator/(dd x, dd y) {
-  const double q1 = x.hi / y.hi;
-  dd r = x - y * q1;
-  const double q2 = r.hi / y.hi;
-  r = r - y * q2;
-  const double q3 = r.hi / y.hi;
-  dd q = quick_two_sum(q1, q2);
-  return q + dd{q3, 0.0};
-}
-
-struct zdd {
-  dd re, im;
-};
-inline zdd operator+(zdd a, zdd b) { return {a.re + b.re, a.im + b.im}; }
-inline zdd operator-(zdd a, zdd b) { return {a.re - b.re, a.im - b.im}; }
-inline zdd operator*(zdd a, zdd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-inline zdd operator/(zdd a, zdd b) {
-  const dd den = b.re * b.re + b.im * b.im;
-  return {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
-}
-inline zdd zd(cd v) { return {dd{v.real(), 0.0}, dd{v.imag(), 0.0}}; }
-
-// Thiele continued-fraction Pade through (z_i, A_i): coefficients a_i = g_i(z_i) with
-// g_0(z_j) = A_j, g_i(z_j) = (g_{i-1}(z_{i-1}) - g_{i-1}(z_j)) / ((z_j - z_{i-1}) g_{i-1}(z_j))
-// (pade_t.hpp:176-192; J. Low Temp. Phys. 29, 179 (1977)).
-struct Thiele {
-  std::vector<zdd> z, a;
-  void fit(const std::vector<cd>& zf, const std::vector<cd>& Af) {
-    const int n = static_cast<int>(zf.size());
-    z.resize(n);
-    a.resize(n);
-    std::vector<zdd> g(Af.size());
-    for (int j = 0; j < n; ++j) {
-      z[j] = zd(zf[j]);
-      g[j] = zd(Af[j]);
-    }
-    a[0] = g[0];
-    for (int i = 1; i < n; ++i) {
-      // g holds g_{i-1}(z_j) for j >= i-1; update in place for j >= i
-      const zdd gd = g[i - 1];
-      for (int j = n - 1; j >= i; --j) g[j] = (gd - g[j]) / ((z[j] - z[i - 1]) * g[j]);
-      a[i] = g[i];
-    }
-  }
-  // A(w) = a_0 / (1 + a_1 (w - z_0) / (1 + a_2 (w - z_1) / (1 + ...)))   (pade_t.hpp:109-128)
-  cd eval(cd w) const {
-    const int n = static_cast<int>(a.size());
-    const zdd W = zd(w), one{dd{1.0, 0.0}, dd{0.0, 0.0}};
-    zdd X = a[n - 1] * (W - z[n - 2]);
-    for (int f = 1; f < n - 1; ++f) {
-      const int idx = n - f - 1;
-      X = (a[idx] * (W - z[idx - 1])) / (one + X);
-    }
-    X = a[0] / (one + X);
-    return cd(X.re.hi + X.re.lo, X.im.hi + X.im.lo);
-  }
-};
-
-// qp_eqn_bisection, qp_scf_common.cpp:679-728, on res(e) = Re[e + i eta - Vhf - Sigma(e - mu + i eta)].
-double bisection(double vhf, const Thiele& S, double mu, double eps0, double tol, double eta) {
-  auto res_f = [&](double e) { return (cd(e, eta) - vhf - S.eval(cd(e - mu, eta))).real(); };
-  const double delta = 0.01;
-  double res = res_f(eps0);
-  if (std::fabs(res) < tol) return eps0;
-  double e1, e2;
-  int guard = 0;
-  if (res >= 0) {
-    e2 = eps0;
-    e1 = eps0 - delta;
-    double r1 = res_f(e1);
-    while (r1 > 0 && guard++ < 100000) {
-      e1 -= delta;
-      r1 = res_f(e1);
-    }
-  } else {
-    e1 = eps0;
-    e2 = eps0 + delta;
-    double r2 = res_f(e2);
-    while (r2 < 0 && guard++ < 100000) {
-      e2 += delta;
-      r2 = res_f(e2);
-    }
-  }
-  double mid = 0.5 * (e1 + e2);
-  res = res_f(mid);
-  guard = 0;
-  while (std::fabs(res) > tol && guard++ < 200) {
-    if (res >= 0) e2 = mid;
-    else e1 = mid;
-    mid = 0.5 * (e1 + e2);
-    res = res_f(mid);
-  }
-  return mid;
-}
-
-}  // namespace
-
-void solve_qp_diag(const Tables& tab, int nk, int no, const cd* sig_t, const cd* vhf, const double* E0, double mu,
-                   int nfit, double eta, double tol, double* E_out) {
-  const int nt = tab.nt, nw = tab.nw_f;
-  const long dim = long(nk) * no;
-  const std::vector<int> idx = pade_fit_indices(nw, nfit);
-  std::vector<cd> zf(nfit);
-  for (int i = 0; i < nfit; ++i) zf[i] = cd(0.0, double(tab.wn_f[idx[i]]) * M_PI / tab.beta);  // IAFT.hpp:55-57
-  // One state per task, handed out by an atomic counter to plain std::threads.  The
-  // library carries no OpenMP runtime of its own: a host process that also loads an
-  // OpenMP-parallel library (torch's libgomp, the test oracle) then runs a single one.
-  std::atomic<long> next{0};
-  auto work = [&] {
-    std::vector<cd> Af(nfit);
-    for (long I = next++; I < dim; I = next++) {
-      for (int i = 0; i < nfit; ++i) {  // Sigma(iw_n) = Twt_ff Sigma(tau) at the fit frequencies
-        cd acc = 0.0;
-        const int w = idx[i];
-        for (int t = 0; t < nt; ++t) acc += tab.Twt_ff[size_t(w) * nt + t] * sig_t[size_t(t) * dim + I];
-        Af[i] = acc;
-      }
-      Thiele p;
-      p.fit(zf, Af);
-      E_out[I] = bisection(vhf[I].real(), p, mu, E0[I], tol, eta);
-    }
-  };
-  const long hw = std::max(1L, static_cast<long>(std::thread::hardware_concurrency()));
-  const int nth = static_cast<int>(std::min({hw, 16L, std::max(1L, dim / 64)}));
-  std::vector<std::thread> pool;
-  for (int t = 1; t < nth; ++t) pool.emplace_back(work);
-  work();
-  for (std::thread& th : pool) th.join();
 }
 
 }  // namespace host

@@ -32,11 +32,6 @@ void fold_ph(const Tables& tab, const thcgw_dims& d, std::vector<cd>& Tp, std::v
 // Ec = Re[T_beta_t_ff . (Ttw_bb e_w)] / (2 nq)   (thc_rpa.icc:171-173)
 double ec_from_ew(const Tables& tab, const std::vector<cd>& ew, int nq);
 
-// Quasiparticle energies from the MO-diagonal self-energy Sigma_nn(tau) (nt x nk*no),
-// the MO-diagonal Vhf (nk*no) and starting energies E0 (nk*no).
-void solve_qp_diag(const Tables& tab, int nk, int no, const cd* sigma_diag_t, const cd* vhf_diag, const double* E0,
-                   double mu, int nfit, double eta, double tol, double* E_out);
-
 // GM energy from SigmaG(iw_n) = sum_k w_k Tr[Sigma(iw_n) G(iw_n)] (nw_f values), scf_common.cpp:119-128.
 double gm_from_sigmag(const Tables& tab, const std::vector<cd>& sg);
 

@@ -210,6 +210,19 @@ void scatter_ir_terms(hipStream_t s, const c128* trs, const double* lds, const i
 void trev_diff(hipStream_t s, const c128* A, long M, const int* mq, int nq, int nbx, double* out);
 // out[b] (b < nbx) = number of non-finite doubles among x[0, n) in block b's grid-stride share
 void nonfinite_count(hipStream_t s, const double* x, long n, int nbx, int* out);
+
+// Quasiparticle equation per (k, n) state on the device (qp.hip): Pade fit frequencies as
+// rows of Twt_ff (w) with z_i = i * zim[i].
+constexpr int QP_MAX_FIT = 64;
+struct QpFit {
+  int nfit;
+  int w[QP_MAX_FIT];
+  double zim[QP_MAX_FIT];
+};
+// E_out[I] for I < dim from sig_t (nt, dim) diagonal Sigma_nn(tau), vhf (dim) diagonal Vhf_nn,
+// E0 (dim) starting energies (all device arrays); Twt (nw_f, nt) the fermionic table.
+void qp_solve(hipStream_t s, const c128* sig_t, const c128* vhf, const double* E0, const c128* Twt, const QpFit& fit,
+              int nt, long dim, double mu, double eta, double tol, double* E_out);
 // per block b' < nbx: max |A_ij - conj(A_ji)| and max |A_ij| over nb matrices of order n
 void herm_diff(hipStream_t s, const c128* A, int n, long nb, int nbx, double* out);
 // q-point symmetry map: W[w][q] = ph_q conj(ph_q)^T o W^[w][src q](perm_q, perm_q) for the q with

@@ -24,7 +24,7 @@ tables:
 Regenerating a shipped table reproduces its meshes (Matsubara indices exactly, tau points to
 ~1e-16) and transforms (to ~1e-12 relative); tests/test_ir_generator.py checks this.
 
-    python coqui_amd/data/ir/gen_ir_tables.py 2e3 2e-15      # writes 2e3.2e-15.npz (80 points)
+    python coqui_amd/data/ir/gen_ir_tables.py 1e3 5e-18      # writes 1e3.5e-18.npz (80 points)
     python coqui_amd/data/ir/gen_ir_tables.py 1e3 1e-12      # writes 1e3.1e-12.npz (60 points)
     python coqui_amd/data/ir/gen_ir_tables.py --compare 1e3.1e-15   # against the shipped table
 """

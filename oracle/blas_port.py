@@ -91,8 +91,11 @@ def rpa_terms(Piw, V):
             A = Piw[w, q] @ V[q]
             tr[w, q] = np.trace(A)
             lu, piv = sla.lu_factor(eye - A, overwrite_a=True, check_finite=False)
-            d = np.prod(np.diag(lu))
-            ld[w, q] = np.log(abs(d.real))
+            # ln|Re det| = sum ln|u_ii| + ln|Re(phase)|, phase = sign(perm) prod u_ii/|u_ii|
+            # (the product of the diagonal itself overflows at N_mu in the thousands)
+            d = np.diag(lu)
+            sign = -1.0 if np.count_nonzero(piv != np.arange(n)) % 2 else 1.0
+            ld[w, q] = np.sum(np.log(np.abs(d))) + np.log(abs((sign * np.prod(d / np.abs(d))).real))
     return tr, ld
 
 

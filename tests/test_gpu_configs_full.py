@@ -96,7 +96,7 @@ def test_batch24_full(orc, torch):
 
 
 def test_scgw_k64_mu1200_learned(torch):
-    kgrid, no, nmu, table = (4, 4, 4), 78, 1200, "2e3.2e-15"
+    kgrid, no, nmu, table = (4, 4, 4), 78, 1200, "1e3.5e-18"
     print("case: scGW kgrid=%s no=%d nmu=%d table=%s, 3 iterations" % (kgrid, no, nmu, table), flush=True)
     s = make_system(kgrid, no, nmu, system_id=0)
     ia = IAFT.from_table(1000.0, table)

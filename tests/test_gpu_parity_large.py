@@ -31,7 +31,7 @@ from oracle import blas_port
 
 pytestmark = pytest.mark.gpu
 
-TABLE = "2e3.2e-15"  # 80 tau points
+TABLE = "1e3.5e-18"  # 80 tau points
 HA_PER_MEV = 1.0 / 27211.386245988
 SHAPES = {"k64-mu800": ((4, 4, 4), 52, 800), "k64-mu1600": ((4, 4, 4), 104, 1600)}
 FORMS = {"plain": {6: 0, 7: 0, 10: 0}, "mixed_resolvent": {6: 2, 7: 1, 10: 1}}
@@ -80,6 +80,12 @@ def g_pair(d, i):
     return d["G"][[i, nt - 1 - i]].cpu().numpy()
 
 
+def herm_lower(A):
+    """The Hermitian matrix whose lower triangle is A's: what the resolvent Dyson form reads."""
+    L = np.tril(A)
+    return L + np.conj(np.tril(A, -1).T)
+
+
 def stage_checks(d, orc, torch, form):
     s, tab, nth, nwh = d["s"], d["tab"], d["nth"], d["nwh"]
     nk, nmu = s.nk, s.nmu
@@ -88,57 +94,65 @@ def stage_checks(d, orc, torch, form):
     for f, v in FORMS[form].items():
         ctx.set_flag(f, v)
     errs = {}
-    # chi0
-    Pit = ctx.pi(d["G"], d["X"])
-    torch.cuda.synchronize()
-    for i in (0, nth - 1):
-        if ("pi", i) not in d["results"]:  # the oracle's chi0 depends on G and X only
-            d["results"][("pi", i)] = orc.pi_rspace(g_pair(d, i), s.X, s.kgrid)[0]
-        ref = d["results"][("pi", i)]
-        got = Pit[i].cpu().numpy()
-        errs["pi_%d" % i] = relmax(got, ref)
-        herm = np.max(np.abs(got - np.conj(np.swapaxes(got, 1, 2)))) / np.max(np.abs(got))
-        assert herm < tol["herm"], ("Pi not Hermitian", i, herm)
-    # tau -> iW
-    Piw = ctx.tau_to_w(Pit)
-    torch.cuda.synchronize()
-    qs = (0, nk - 5)
-    for q in qs:
-        ref = orc.tau_to_w_phsym(tab, Pit[:, q].cpu().numpy())
-        errs["tau_to_w_q%d" % q] = relmax(Piw[:, q].cpu().numpy(), ref)
-    # Dyson on every (q, iW_n); four sampled matrices against the oracle
-    samples = [(0, 0), (1, 5), (nwh // 2, 37 % nk), (nwh - 1, nk - 1)]
-    Pis = {(n, q): Piw[n, q].cpu().numpy() for n, q in samples}
-    Vs = {q: d["V"][q].cpu().numpy() for _, q in samples}
-    tr, ld = ctx.dyson_w(Piw, d["V"], with_terms=True)
-    torch.cuda.synchronize()
-    for n, q in samples:
-        ref = orc.dyson(Pis[(n, q)][None, None], Vs[q][None])[0, 0]
-        errs["dyson_n%d_q%d" % (n, q)] = relmax(Piw[n, q].cpu().numpy(), ref)
-        rtr, rld = blas_port.rpa_terms(Pis[(n, q)][None, None], Vs[q][None])
-        assert abs(tr[n * nk + q] - rtr[0, 0]) <= 1e-9 * abs(rtr[0, 0]), ("trace", n, q)
-        assert abs(ld[n * nk + q] - rld[0, 0]) <= 1e-9 * max(1.0, abs(rld[0, 0])), ("logdet", n, q)
-    del Pis
-    # iW -> tau, into the W_qt layout (the Pi_t buffer is reused for it)
-    Wqt = Pit.view(nk, nth, nmu, nmu)
-    ctx.w_to_tau(Piw, out=Wqt)
-    torch.cuda.synchronize()
-    for q in qs:
-        ref = orc.w_to_tau_phsym(tab, Piw[:, q].cpu().numpy())
-        errs["w_to_tau_q%d" % q] = relmax(Wqt[q].cpu().numpy(), ref)
-    del Piw
-    # Sigma, both passes, on the GPU's W
-    S = ctx.sigma(d["G"], Wqt, d["X"], hermitize=False)
-    torch.cuda.synchronize()
-    nt = d["ia"].nt
-    for i in (0, nth - 1):
-        ref = orc.sigma_rspace(g_pair(d, i), s.X, np.ascontiguousarray(Wqt[:, i:i + 1].cpu().numpy()), s.kgrid)
-        got = S[[i, nt - 1 - i]].cpu().numpy()
-        errs["sigma_%d" % i] = relmax(got, ref)
-    ctx.close()
-    del Wqt, Pit, S
-    gc.collect()
-    torch.cuda.empty_cache()
+    Pit = Piw = Wqt = S = None
+    try:
+        # chi0
+        Pit = ctx.pi(d["G"], d["X"])
+        torch.cuda.synchronize()
+        for i in (0, nth - 1):
+            if ("pi", i) not in d["results"]:  # the oracle's chi0 depends on G and X only
+                d["results"][("pi", i)] = orc.pi_rspace(g_pair(d, i), s.X, s.kgrid)[0]
+            ref = d["results"][("pi", i)]
+            got = Pit[i].cpu().numpy()
+            errs["pi_%d" % i] = relmax(got, ref)
+            herm = np.max(np.abs(got - np.conj(np.swapaxes(got, 1, 2)))) / np.max(np.abs(got))
+            assert herm < tol["herm"], ("Pi not Hermitian", i, herm)
+        # tau -> iW
+        Piw = ctx.tau_to_w(Pit)
+        torch.cuda.synchronize()
+        qs = (0, nk - 5)
+        for q in qs:
+            ref = orc.tau_to_w_phsym(tab, Pit[:, q].cpu().numpy())
+            errs["tau_to_w_q%d" % q] = relmax(Piw[:, q].cpu().numpy(), ref)
+        # Dyson on every (q, iW_n); four sampled matrices against the oracle.  The resolvent
+        # form reads the lower triangle of the Hermitian Pi, so its references take that too.
+        samples = [(0, 0), (1, 5), (nwh // 2, 37 % nk), (nwh - 1, nk - 1)]
+        lower = FORMS[form][6] == 2
+        Pis = {(n, q): Piw[n, q].cpu().numpy() for n, q in samples}
+        if lower:
+            Pis = {k: herm_lower(v) for k, v in Pis.items()}
+        Vs = {q: d["V"][q].cpu().numpy() for _, q in samples}
+        tr, ld = ctx.dyson_w(Piw, d["V"], with_terms=True)
+        torch.cuda.synchronize()
+        for n, q in samples:
+            ref = orc.dyson(Pis[(n, q)][None, None], Vs[q][None])[0, 0]
+            errs["dyson_n%d_q%d" % (n, q)] = relmax(Piw[n, q].cpu().numpy(), ref)
+            rtr, rld = blas_port.rpa_terms(Pis[(n, q)][None, None], Vs[q][None])
+            assert abs(tr[n * nk + q] - rtr[0, 0]) <= 1e-9 * abs(rtr[0, 0]), ("trace", n, q, tr[n * nk + q], rtr[0, 0])
+            assert abs(ld[n * nk + q] - rld[0, 0]) <= 1e-9 * max(1.0, abs(rld[0, 0])), ("logdet", n, q, ld[n * nk + q],
+                                                                                          rld[0, 0])
+        del Pis
+        # iW -> tau, into the W_qt layout (the Pi_t buffer is reused for it)
+        Wqt = Pit.view(nk, nth, nmu, nmu)
+        ctx.w_to_tau(Piw, out=Wqt)
+        torch.cuda.synchronize()
+        for q in qs:
+            ref = orc.w_to_tau_phsym(tab, Piw[:, q].cpu().numpy())
+            errs["w_to_tau_q%d" % q] = relmax(Wqt[q].cpu().numpy(), ref)
+        Piw = None
+        # Sigma, both passes, on the GPU's W
+        S = ctx.sigma(d["G"], Wqt, d["X"], hermitize=False)
+        torch.cuda.synchronize()
+        nt = d["ia"].nt
+        for i in (0, nth - 1):
+            ref = orc.sigma_rspace(g_pair(d, i), s.X, np.ascontiguousarray(Wqt[:, i:i + 1].cpu().numpy()), s.kgrid)
+            got = S[[i, nt - 1 - i]].cpu().numpy()
+            errs["sigma_%d" % i] = relmax(got, ref)
+    finally:
+        ctx.close()
+        del Pit, Piw, Wqt, S
+        gc.collect()
+        torch.cuda.empty_cache()
     print("%s %s errors vs oracle: %s" % (d["name"], form, {k: "%.2e" % v for k, v in errs.items()}), flush=True)
     for k, v in errs.items():
         lim = tol["pi"] if k.startswith(("pi", "tau_to_w")) else tol["s"] if k.startswith("sigma") else tol["w"]
@@ -148,18 +162,20 @@ def stage_checks(d, orc, torch, form):
 def full_iteration(d, torch, form):
     s = d["s"]
     ctx = ThcGW(s.kgrid, s.no, s.nmu, d["ia"])
-    for f, v in FORMS[form].items():
-        ctx.set_flag(f, v)
-    S, W, ec = ctx.gw_iteration(d["G"], d["X"], d["V"], with_ec=True, hermitize=True)
-    H = torch.from_numpy(s.H).cuda()
-    C = torch.from_numpy(s.C).cuda()
-    E = ctx.solve_qp(S, C, H, s.E, mu=s.mu)
-    out = dict(E=E, ec=ec, S=S.cpu().numpy(), W0=W[0].cpu().numpy())
-    ctx.close()
-    del S, W
-    gc.collect()
-    torch.cuda.empty_cache()
-    return out
+    S = W = None
+    try:
+        for f, v in FORMS[form].items():
+            ctx.set_flag(f, v)
+        S, W, ec = ctx.gw_iteration(d["G"], d["X"], d["V"], with_ec=True, hermitize=True)
+        H = torch.from_numpy(s.H).cuda()
+        C = torch.from_numpy(s.C).cuda()
+        E = ctx.solve_qp(S, C, H, s.E, mu=s.mu)
+        return dict(E=E, ec=ec, S=S.cpu().numpy(), W0=W[0].cpu().numpy())
+    finally:
+        ctx.close()
+        del S, W
+        gc.collect()
+        torch.cuda.empty_cache()
 
 
 def test_plain_vs_oracle(big, orc, torch):

@@ -32,7 +32,7 @@ def fold(tab):
     return Tp, Tm
 
 
-@pytest.mark.parametrize("table", ["1e2.1e-15", "1e3.1e-15", "1e4.1e-15", "2e3.2e-15", "1e3.1e-12"])
+@pytest.mark.parametrize("table", ["1e2.1e-15", "1e3.1e-15", "1e4.1e-15", "1e3.5e-18", "1e3.1e-12"])
 def test_folded_tables_real(table):
     Tp, Tm = fold(oracle.load_ir(1000.0, None, table=table))
     for T in (Tp, Tm):
