@@ -374,7 +374,7 @@ def main():
         set_form(args.headline)
         stages = {}
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
-        Pit = Wqt
+        Pit = Wqt.view(nth, s.nk, nmu, nmu)  # the W_qt buffer doubles as Pi(tau), as in gw_iteration
         Piw = torch.empty((nwh, s.nk, nmu, nmu), dtype=torch.complex128, device="cuda")
         ev[0].record(stream)
         ctx.pi(Gd, Xd, out=Pit)

@@ -81,9 +81,10 @@ def g_pair(d, i):
 
 
 def herm_lower(A):
-    """The Hermitian matrix whose lower triangle is A's: what the resolvent Dyson form reads."""
-    L = np.tril(A)
-    return L + np.conj(np.tril(A, -1).T)
+    """The Hermitian matrix whose strict lower triangle is A's, with the real part of A's
+    diagonal: what the resolvent Dyson form reads."""
+    L = np.tril(A, -1)
+    return L + np.conj(L.T) + np.diag(np.diag(A).real)
 
 
 def stage_checks(d, orc, torch, form):
@@ -122,6 +123,8 @@ def stage_checks(d, orc, torch, form):
         if lower:
             Pis = {k: herm_lower(v) for k, v in Pis.items()}
         Vs = {q: d["V"][q].cpu().numpy() for _, q in samples}
+        if lower:  # V too is read as Hermitian from its lower triangle (Cholesky of V)
+            Vs = {k: herm_lower(v) for k, v in Vs.items()}
         tr, ld = ctx.dyson_w(Piw, d["V"], with_terms=True)
         torch.cuda.synchronize()
         for n, q in samples:
