@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -30,6 +31,30 @@ struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
 };
+
+// Staging buffer of the entry point that allocates nothing (thcgw_primary_to_aux_noalloc):
+// a file-scope device array, so each GPU that loads the library holds its own copy until
+// the process ends.  Host code never passes the array's name as an address: its device
+// address is resolved once per device by hipGetSymbolAddress (noalloc_scratch, called by
+// thcgw_create with that device current) and kept per device.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;  // 64 MiB
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+std::mutex g_noalloc_mu;
+std::map<int, void*> g_noalloc_ptr;  // device -> address of that device's copy of scratch
+
+void* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  auto it = g_noalloc_ptr.find(device);
+  if (it != g_noalloc_ptr.end()) return it->second;
+  int cur = -1;
+  THCGW_HIP(hipGetDevice(&cur));
+  THCGW_REQUIRE(cur == device, "noalloc scratch: the context's device is not current");
+  void* p = nullptr;
+  THCGW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  THCGW_REQUIRE(p != nullptr, "noalloc scratch: hipGetSymbolAddress returned no address");
+  g_noalloc_ptr[device] = p;
+  return p;
+}
 
 }  // namespace
 
@@ -1150,6 +1175,7 @@ int thcgw_create(const thcgw_dims* dims, const thcgw_ir_tables* tables, thcgw_ct
     ctx->M = static_cast<long>(d.nmu) * d.nmu;
     thcgw::host::load_tables(d, *tables, ctx->tab);  // checks PH symmetry of the tau mesh
     THCGW_HIP(hipGetDevice(&ctx->device));
+    noalloc_scratch(ctx->device);  // once per device: address of the no-allocation staging buffer
     std::vector<std::complex<double>> Tp, Tm;
     thcgw::host::fold_ph(ctx->tab, d, Tp, Tm);
     // Folding G(tau) and G(beta - tau) pairs e^{i W tau} with e^{-i W tau}: the tables are real
@@ -1335,6 +1361,43 @@ int thcgw_primary_to_aux(thcgw_ctx* ctx, long nb, const double* X, const double*
     const long nbk = nb * ctx->nk;
     c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(nbk) * ctx->d.nmu * ctx->d.no);
     project_p2a(ctx, S(stream), C(X), C(Oin), Cm(Oout), nbk, Ab);
+  });
+}
+
+// thcgw_primary_to_aux without device allocation: the intermediate X_k O_{b,k} of each (b, k)
+// (N_mu x N_orb) is staged in the static scratch; the batch runs in chunks of whole (b, k)
+// entries that fit it (whole b blocks when a block of nk fits, else k ranges within a block,
+// so that the X_k of a chunk never wraps).  The GEMM overwrites the staged values (beta = 0),
+// so nothing needs clearing between calls; calls that overlap on one GPU race on scratch.
+int thcgw_primary_to_aux_noalloc(thcgw_ctx* ctx, long nb, const double* X, const double* Oin, double* Oout,
+                                 void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    THCGW_REQUIRE(X && Oin && Oout && nb >= 0, "thcgw_primary_to_aux_noalloc: bad argument");
+    const int nk = ctx->nk, no = ctx->d.no;
+    const long per = static_cast<long>(ctx->d.nmu) * no;  // c128 elements staged per (b, k)
+    const long cap = static_cast<long>(kNoallocScratchBytes / (sizeof(c128) * per));
+    THCGW_REQUIRE(cap >= 1, "thcgw_primary_to_aux_noalloc: one staged N_mu x N_orb block (" +
+                                std::to_string(per * 16) + " bytes) exceeds the " +
+                                std::to_string(kNoallocScratchBytes >> 20) + " MiB scratch");
+    c128* Ab = static_cast<c128*>(noalloc_scratch(ctx->device));
+    const long so = static_cast<long>(no) * no;
+    hipStream_t s = S(stream);
+    if (cap >= nk) {
+      const long bchunk = cap / nk;
+      for (long b0 = 0; b0 < nb; b0 += bchunk) {
+        const long cb = std::min(bchunk, nb - b0);
+        project_p2a(ctx, s, C(X), C(Oin) + b0 * nk * so, Cm(Oout) + b0 * nk * ctx->M, cb * nk, Ab);
+      }
+    } else {
+      const long sX = per;
+      for (long b = 0; b < nb; ++b)
+        for (long k0 = 0; k0 < nk; k0 += cap) {
+          const long ck = std::min(cap, nk - k0);
+          const long e0 = b * nk + k0;
+          project_p2a(ctx, s, C(X) + k0 * sX, C(Oin) + e0 * so, Cm(Oout) + e0 * ctx->M, ck, Ab);
+        }
+    }
   });
 }
 

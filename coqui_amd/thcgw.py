@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libthcgw.so")
 #: every symbol declared in include/thcgw.h
 EXPORTS = (
     "thcgw_create", "thcgw_destroy", "thcgw_last_error", "thcgw_update_G", "thcgw_primary_to_aux",
+    "thcgw_primary_to_aux_noalloc",
     "thcgw_aux_to_primary", "thcgw_pi", "thcgw_tau_to_w", "thcgw_dyson_w", "thcgw_w_to_tau", "thcgw_sigma",
     "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
     "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf", "thcgw_dyson_g",
@@ -80,6 +81,7 @@ def load_library(path=LIB_PATH):
         "thcgw_last_error": (ctypes.c_char_p, [p]),
         "thcgw_update_G": (i, [p, p, p, d, p, p]),
         "thcgw_primary_to_aux": (i, [p, l, p, p, p, p]),
+        "thcgw_primary_to_aux_noalloc": (i, [p, l, p, p, p, p]),
         "thcgw_aux_to_primary": (i, [p, l, p, p, p, p]),
         "thcgw_pi": (i, [p, p, p, p, p]),
         "thcgw_tau_to_w": (i, [p, p, p, p]),

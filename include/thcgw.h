@@ -95,6 +95,17 @@ int thcgw_update_G(thcgw_ctx* ctx, const double* C, const double* E, double mu, 
 int thcgw_primary_to_aux(thcgw_ctx* ctx, long nb, const double* X_skPa, const double* O_in, double* O_out,
                          void* stream);
 
+/* No counterpart in the reference: thcgw_primary_to_aux for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results.  The intermediate X_k O_{b,k} is staged in a
+ * static device buffer of 64 MiB per GPU (held until the process ends) instead of the
+ * context's workspace, and the batch runs in chunks that fit it; no hipMalloc/hipFree.
+ * THCGW_ERR_ARG if one N_mu x N_orb complex<double> block exceeds 64 MiB.  Every call on a
+ * GPU shares that GPU's buffer: calls that overlap on one GPU (other streams, threads or
+ * contexts) race, so serialise them. */
+int thcgw_primary_to_aux_noalloc(thcgw_ctx* ctx, long nb, const double* X_skPa, const double* O_in, double* O_out,
+                                 void* stream);
+
 /* a10: O_ab = X^H O_PQ X: O_in (nb, nk, nmu, nmu) -> O_out (nb, nk, no, no).
  * Replaces thc_solver_comm::aux_to_primary, thc_solver_comm.hpp:240-304,516-569. */
 int thcgw_aux_to_primary(thcgw_ctx* ctx, long nb, const double* X_skPa, const double* O_in, double* O_out,
