@@ -294,12 +294,15 @@ void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c64* O
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "aux_to_primary: batch too large");
+  // Both products sum over N_mu: each K chunk is folded into FP64 accumulators, so the
+  // rounding grows with the chunk length instead of with N_mu.
   thcgw::cgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
                thcgw::bp(Oin, ctx->M), DT_C64, nmu, kZero, Bbuf, DT_C64, sX, nmu,
-               static_cast<int>(nbk), kZero, ctx->pf() | (herm ? thcgw::GEMM_TRI_KFROM_COL : 0));
+               static_cast<int>(nbk), kZero,
+               ctx->pf() | thcgw::GEMM_FOLD64 | (herm ? thcgw::GEMM_TRI_KFROM_COL : 0));
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Bbuf, sX), DT_C64, nmu,
                thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C128, static_cast<long>(no) * no, no,
-               static_cast<int>(nbk), kZero, ctx->pf());
+               static_cast<int>(nbk), kZero, ctx->pf() | thcgw::GEMM_FOLD64);
   if (herm) thcgw::hermitize(s, Oout, no, nbk, 2.0);
 }
 

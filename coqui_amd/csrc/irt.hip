@@ -1,5 +1,5 @@
 // Imaginary-axis transforms between tau and iW_n (IAFT.icc:49-73, 104-128): a short,
-// wide product out[r][c] = sum_i T[r][i] in[i][c] with R <= 48, I <= 40 and up to
+// wide product out[r][c] = sum_i T[r][i] in[i][c] with R <= 48, I <= 44 and up to
 // N_q N_mu^2 columns.  As a 64 x 64 GEMM tile the R = 36..37 rows of the 72-point table
 // fill little more than half the tile and every tile restages T through LDS.
 //
@@ -106,20 +106,24 @@ void launch_irt(hipStream_t s, const c128* T, int R, int I, const c128* in, long
 
 }  // namespace
 
-// At most 3 row blocks (R <= 48) and 10 K blocks (I <= 40) of the MFMA; T takes 30 KB of LDS.
-bool ir_transform_fits(int R, int I) { return R >= 1 && I >= 1 && R <= 48 && I <= 40; }
+// At most 3 row blocks (R <= 48) and 11 K blocks (I <= 44) of the MFMA; T takes up to 33 KB
+// of LDS.  The 80-point tables (nt_half = 40, nw_half = 41) need the eleventh K block for
+// the iW -> tau direction.
+bool ir_transform_fits(int R, int I) { return R >= 1 && I >= 1 && R <= 48 && I <= 44; }
 
 void ir_transform(hipStream_t s, const c128* T, int R, int I, const c128* in, long ld_in, long bs_in, c128* out,
                   long ld_out, long bs_out, long cols, int batch, bool real_t) {
-  THCGW_REQUIRE(ir_transform_fits(R, I), "ir_transform: at most 48 rows and 40 columns of T");
+  THCGW_REQUIRE(ir_transform_fits(R, I), "ir_transform: at most 48 rows and 44 columns of T");
   THCGW_REQUIRE(batch >= 1 && batch <= 65535, "ir_transform: batch out of range");
   if (cols <= 0) return;
   if (R <= 32 && I <= 24)
     launch_irt<2, 6>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
   else if (I <= 36)
     launch_irt<3, 9>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
-  else
+  else if (I <= 40)
     launch_irt<3, 10>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
+  else
+    launch_irt<3, 11>(s, T, R, I, in, ld_in, bs_in, out, ld_out, bs_out, cols, batch, real_t);
 }
 
 }  // namespace thcgw

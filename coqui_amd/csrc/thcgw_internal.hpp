@@ -167,7 +167,7 @@ void resolvent_prep(hipStream_t s, const c128* V, const c128* Vi, long sV, int v
                     long sA, int n, c128* tr, c128* part, int batch);
 void herm_finish(hipStream_t s, c128* W, long sW, const c128* V, long sV, int vmod, int voff, int n, int batch);
 
-// irt.hip: out[b][r][c] = sum_i T[r][i] in[b][i][c] (T R x I row-major, R <= 48, I <= 40; rows of in
+// irt.hip: out[b][r][c] = sum_i T[r][i] in[b][i][c] (T R x I row-major, R <= 48, I <= 44; rows of in
 // and out ld_in / ld_out apart, matrices bs_in / bs_out apart); real_t: only Re T is used
 bool ir_transform_fits(int R, int I);
 void ir_transform(hipStream_t s, const c128* T, int R, int I, const c128* in, long ld_in, long bs_in, c128* out,

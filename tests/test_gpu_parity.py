@@ -154,6 +154,7 @@ def test_qp_and_gm_nk8_nmu200(nk8_nmu200, orc, torch):
     ((1, 1, 1), "1e2.1e-15", 5, 70),      # Gamma only
     ((2, 3, 1), "1e2.1e-15", 6, 20),      # non-cubic grid: generic transform kernel
     ((2, 2, 2), "1e3.1e-15", 6, 24),      # 72 tau points: tau <-> iW MFMA kernel with 3 row blocks
+    ((2, 2, 2), "1e3.5e-18", 6, 24),      # 80 tau points: iW -> tau with 41 frequencies (11 K blocks)
     ((1, 1, 1), "1e4.1e-15", 5, 20),      # 104 tau points: tau <-> iW by the GEMM (T has 53 rows)
 ])
 def test_iteration_shapes(orc, torch, kgrid, table, no, nmu):
