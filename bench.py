@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--krylov-max-iter", type=int, default=60, help="scgw-k64-mu1200: CG iterations before the direct fallback")
     ap.add_argument("--compare-modes", type=int, default=1,
                     help="scgw-k64-mu1200: also solve the final state's Dyson step with every preconditioner (0 to skip)")
+    ap.add_argument("--batch-lanes", type=int, default=4,
+                    help="batch24: streams the systems are spread over (parallel branches of the graph)")
     ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
                     help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
     ap.add_argument("--forms", default="plain,mixed_resolvent",
@@ -446,7 +448,8 @@ def run_batch(args):
     """Batched form: the 24 systems of batch_systems() in mixed precision (FP32 projections
     and k-grid transforms; Dyson step by the FP64 resolvent form, or with --batch-dyson
     hermitian by the FP32 Hermitian form), one iteration of all of them per step, replayed
-    from one captured HIP graph; also timed one system after another without the graph, and
+    from one captured HIP graph whose --batch-lanes streams (4) run as parallel branches;
+    also timed one system after another without the graph (same chunk sizes), and
     against a plain FP64 pass of the same batch (LU Dyson, one system after another), which
     is also the reference of the errors."""
     import torch
@@ -500,6 +503,10 @@ def run_batch(args):
     torch.cuda.synchronize()
     ms_plain = ev0.elapsed_time(ev1)
     log("bench: plain FP64 batch pass %.1f ms" % ms_plain)
+    # the lanes (one scratch pool each, automatic chunks at 1/lanes of the memory) only for the
+    # batched form: the plain pass above keeps the one-pool chunk sizes
+    if args.batch_lanes != 1:
+        b.set_lanes(args.batch_lanes)
     for d in sysd:
         d["ctx"].set_flag(6, 2 if args.batch_dyson == "resolvent" else 1)
         d["ctx"].set_flag(10, int(args.batch_dyson == "resolvent"))  # Hermitian G projections
@@ -560,6 +567,7 @@ def run_batch(args):
                        len(specs), args.batch_dyson),
                    "systems": [{"kgrid": list(k), "n_orb": no, "n_mu": nmu} for k, no, nmu, _ in specs],
                    "ir_table": BATCH_TABLE, "n_tau": ia.nt, "beta": beta, "parallelism": "single GPU"},
+        "batch_lanes": args.batch_lanes,
         "ms_per_step_one_by_one": round(ms_eager, 3),
         "graph_speedup": round(ms_eager / ms_graph, 3),
         "ms_plain_fp64_one_by_one": round(ms_plain, 3),

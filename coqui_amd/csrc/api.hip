@@ -89,6 +89,8 @@ struct thcgw_ctx {
   bool force_fallback = false;
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
+  int mem_share = 1;    // scratch pools side by side (batch lanes): automatic chunks take 1/mem_share
+  unsigned long* scratch_gen = nullptr;  // bumped on every scratch (re)allocation (a batch's graph)
   int herm_g = 0;       // flag 10: G(tau, k) Hermitian, its projections formed on the lower triangle
   int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky),
                         // 2 = resolvent (V^-1 - Pi)^-1
@@ -122,6 +124,7 @@ struct thcgw_ctx {
                                                         name + "' failed: " + hipGetErrorString(e));
       }
       b.bytes = need;
+      if (scratch_gen) ++*scratch_gen;
     }
     return static_cast<T*>(b.p);
   }
@@ -157,13 +160,19 @@ struct thcgw_ctx {
 };
 
 // A batch of independent systems (uneven sizes), each with its own context.  The systems
-// run one after another on one stream and share one scratch pool; after a first eager
-// iteration (which allocates the scratch, tunes each mixed-precision system's FP32 tile
-// shape and keeps the chunk sizes) the whole batch iteration is captured into one HIP
-// graph and replayed.
+// are spread over `lanes` streams (one by default); the systems of a lane run one after
+// another and share that lane's scratch pool.  After a first eager iteration (which
+// allocates the scratch, tunes each mixed-precision system's FP32 tile shape and keeps the
+// chunk sizes) the whole batch iteration is captured into one HIP graph, the lanes as
+// parallel branches between a fork and a join, and replayed.
 struct thcgw_batch {
   std::vector<thcgw_ctx*> ctxs;
-  std::map<std::string, DevBuf> pool;
+  std::vector<std::map<std::string, DevBuf>> pools{1};  // scratch per lane
+  std::vector<int> lane_of;                              // lane of each system
+  std::vector<hipStream_t> lane_s;                       // lane streams (lanes > 1)
+  std::vector<hipEvent_t> lane_ev;                       // join events (lanes > 1)
+  hipEvent_t ev_fork = nullptr;
+  unsigned long gen = 0;  // scratch allocations of the contexts: a change invalidates the graph
   bool warmed = false;
   bool use_graph = true;
   bool tune = true;
@@ -175,12 +184,21 @@ struct thcgw_batch {
   hipStream_t cap = nullptr;  // the batch's own stream: capture and replay (the legacy stream cannot be captured)
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::string err;
+  int lanes() const { return static_cast<int>(pools.size()); }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
     sig.clear();
+  }
+  void free_lanes() {
+    for (hipStream_t x : lane_s) (void)hipStreamDestroy(x);
+    for (hipEvent_t x : lane_ev) (void)hipEventDestroy(x);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    lane_s.clear();
+    lane_ev.clear();
+    ev_fork = nullptr;
   }
 };
 
@@ -309,7 +327,7 @@ void project_a2p_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c64* O
 int choose_pair_chunk(thcgw_ctx* ctx, int per_pair_arrays) {
   if (ctx->pair_chunk > 0) return std::min(ctx->pair_chunk, ctx->nt_half);
   const double per = double(per_pair_arrays) * ctx->nk * double(ctx->M) * 16.0;
-  const double budget = std::min(0.25 * double(free_device_bytes()), 24.0 * double(1L << 30));
+  const double budget = std::min(0.25 * double(free_device_bytes()), 24.0 * double(1L << 30)) / ctx->mem_share;
   int c = static_cast<int>(budget / per);
   return std::max(1, std::min(c, ctx->nt_half));
 }
@@ -417,7 +435,8 @@ int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
   // in chunks of 1677 + 691 matrices and 0.635 s in one chunk of 2368.
   double held = 0.0;
   for (const char* name : {"dy_lu", "dy_z", "dy_z64", "dy_m64"}) held += double(ctx->held_bytes(name));
-  const double budget = std::min(0.45 * (double(free_device_bytes()) + held), 64.0 * double(1L << 30));
+  const double budget =
+      std::min(0.45 * (double(free_device_bytes()) + held), 64.0 * double(1L << 30)) / ctx->mem_share;
   const int c = std::max(1, std::min(static_cast<int>(budget / per), total));
   // equal chunks rather than full ones and a short last one
   const int nchunks = (total + c - 1) / c;
@@ -1696,8 +1715,10 @@ int thcgw_batch_create(int nsys, thcgw_ctx* const* ctxs, thcgw_batch** out) {
       if (!thcgw_ctx::private_buf(kv.first) && kv.second.p) (void)hipFree(kv.second.p);
     for (auto it = c->ws.begin(); it != c->ws.end();)
       it = thcgw_ctx::private_buf(it->first) ? std::next(it) : c->ws.erase(it);
-    c->pool = &b->pool;
+    c->pool = &b->pools[0];
+    c->scratch_gen = &b->gen;
   }
+  b->lane_of.assign(nsys, 0);
   *out = b;
   return THCGW_OK;
 }
@@ -1710,11 +1731,18 @@ void thcgw_batch_destroy(thcgw_batch* b) {
     (void)hipStreamSynchronize(b->cap);
     (void)hipStreamDestroy(b->cap);
   }
+  for (hipStream_t x : b->lane_s) (void)hipStreamSynchronize(x);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_out) (void)hipEventDestroy(b->ev_out);
-  for (thcgw_ctx* c : b->ctxs) c->pool = nullptr;
-  for (auto& kv : b->pool)
-    if (kv.second.p) (void)hipFree(kv.second.p);
+  b->free_lanes();
+  for (thcgw_ctx* c : b->ctxs) {
+    c->pool = nullptr;
+    c->scratch_gen = nullptr;
+    c->mem_share = 1;
+  }
+  for (auto& pool : b->pools)
+    for (auto& kv : pool)
+      if (kv.second.p) (void)hipFree(kv.second.p);
   delete b;
 }
 
@@ -1726,6 +1754,55 @@ int thcgw_batch_set(thcgw_batch* b, int use_graph, int tune) {
   b->tune = tune != 0;
   if (!b->use_graph) b->drop_graph();
   return THCGW_OK;
+}
+
+// Spread the systems over `lanes` streams, largest work first onto the least loaded lane
+// (work ~ N_k nw_half N_mu^3, the Dyson step).  Only before the first iteration.
+int thcgw_batch_set_lanes(thcgw_batch* b, int lanes) {
+  if (!b) return THCGW_ERR_ARG;
+  try {
+    THCGW_REQUIRE(lanes >= 1 && lanes <= 8, "thcgw_batch_set_lanes: lanes must be in 1..8");
+    THCGW_REQUIRE(!b->warmed, "thcgw_batch_set_lanes: the batch has already run; set the lanes before the first iteration");
+    THCGW_HIP(hipSetDevice(b->device));
+    const int n = static_cast<int>(b->ctxs.size());
+    lanes = std::min(lanes, n);
+    b->drop_graph();
+    b->free_lanes();
+    THCGW_HIP(hipDeviceSynchronize());  // contexts run alone may still use the scratch
+    for (auto& pool : b->pools)
+      for (auto& kv : pool)
+        if (kv.second.p) THCGW_HIP(hipFree(kv.second.p));
+    b->pools.clear();
+    b->pools.resize(lanes);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto work = [&](int i) {
+      const thcgw_ctx* c = b->ctxs[i];
+      return double(c->nk) * c->nw_half * double(c->d.nmu) * c->d.nmu * c->d.nmu;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    std::vector<double> load(lanes, 0.0);
+    for (int i : order) {
+      const int l = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+      b->lane_of[i] = l;
+      load[l] += work(i);
+      b->ctxs[i]->pool = &b->pools[l];
+      b->ctxs[i]->mem_share = lanes;
+    }
+    if (lanes > 1) {
+      b->lane_s.resize(lanes, nullptr);
+      b->lane_ev.resize(lanes, nullptr);
+      for (int l = 0; l < lanes; ++l) {
+        THCGW_HIP(hipStreamCreateWithFlags(&b->lane_s[l], hipStreamNonBlocking));
+        THCGW_HIP(hipEventCreateWithFlags(&b->lane_ev[l], hipEventDisableTiming));
+      }
+      THCGW_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    }
+    return THCGW_OK;
+  } catch (const thcgw::Error& e) {
+    b->err = e.what();
+    return e.code;
+  }
 }
 
 int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const double* const* X, const double* const* V,
@@ -1741,6 +1818,17 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
     const bool want_ec = Ec != nullptr;
     if (!b->warmed) {
       // eager pass: allocates the shared scratch at its largest size, fixes the chunk sizes
+      // (with several lanes the pools live side by side, and the automatic chunk sizes of
+      // each context take 1/lanes of the free memory).
+      // Calls of the contexts alone before this may have grown every lane's pool to a chunk
+      // sized from the free memory of that moment; start the lanes from empty pools so that
+      // the batch's own chunk sizes are what they hold.
+      THCGW_HIP(hipDeviceSynchronize());
+      for (auto& pool : b->pools) {
+        for (auto& kv : pool)
+          if (kv.second.p) THCGW_HIP(hipFree(kv.second.p));
+        pool.clear();
+      }
       for (int i = 0; i < n; ++i) {
         thcgw_ctx* c = b->ctxs[i];
         c->pair_chunk = c->pair_chunk > 0 ? c->pair_chunk : choose_pair_chunk(c, 4);
@@ -1788,13 +1876,20 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
     }
     sig.push_back(stream);
     sig.push_back(reinterpret_cast<const void*>(static_cast<long>(want_ec)));
+    // A scratch buffer grown since the capture (a context run alone with other settings)
+    // leaves the graph pointing at freed memory: the allocation count is part of the key.
+    sig.push_back(reinterpret_cast<const void*>(b->gen));
     if (b->use_graph) {
       // the caller's earlier work first, then capture / replay on the batch's own stream
       THCGW_HIP(hipEventRecord(b->ev_in, s));
       THCGW_HIP(hipStreamWaitEvent(b->cap, b->ev_in, 0));
       hipStream_t cs = b->cap;
-      if (!b->exec || sig != b->sig || b->herm != herm) {
+      // A buffer that grows while the graph is captured frees memory an earlier node of the
+      // same graph uses: such a capture is dropped and taken again over the grown buffers.
+      for (int attempt = 0; !b->exec || sig != b->sig || b->herm != herm; ++attempt) {
+        THCGW_REQUIRE(attempt < 2, "thcgw_batch_gw_iteration: scratch kept growing during graph capture");
         b->drop_graph();
+        const unsigned long gen0 = b->gen;
         for (thcgw_ctx* c : b->ctxs) c->deferred = true;
         hipError_t ce = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
         if (ce != hipSuccess) {
@@ -1802,8 +1897,23 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
           THCGW_HIP(ce);
         }
         try {
-          for (int i = 0; i < n; ++i)
-            enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0, cs);
+          if (b->lanes() == 1) {
+            for (int i = 0; i < n; ++i)
+              enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0,
+                                cs);
+          } else {
+            // fork: every lane stream joins the capture through the fork event; join: the
+            // capture stream waits for each lane's last work
+            THCGW_HIP(hipEventRecord(b->ev_fork, cs));
+            for (int l = 0; l < b->lanes(); ++l) THCGW_HIP(hipStreamWaitEvent(b->lane_s[l], b->ev_fork, 0));
+            for (int i = 0; i < n; ++i)
+              enqueue_iteration(b->ctxs[i], C(G[i]), C(X[i]), C(V[i]), Cm(Sig[i]), Cm(Wqt[i]), want_ec, herm != 0,
+                                b->lane_s[b->lane_of[i]]);
+            for (int l = 0; l < b->lanes(); ++l) {
+              THCGW_HIP(hipEventRecord(b->lane_ev[l], b->lane_s[l]));
+              THCGW_HIP(hipStreamWaitEvent(cs, b->lane_ev[l], 0));
+            }
+          }
         } catch (...) {
           hipGraph_t g = nullptr;
           (void)hipStreamEndCapture(cs, &g);
@@ -1814,6 +1924,11 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
         }
         for (thcgw_ctx* c : b->ctxs) c->deferred = false;
         THCGW_HIP(hipStreamEndCapture(cs, &b->graph));
+        if (b->gen != gen0) {
+          sig.back() = reinterpret_cast<const void*>(b->gen);
+          b->drop_graph();
+          continue;
+        }
         THCGW_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->sig = sig;
         b->herm = herm;

@@ -23,7 +23,7 @@ EXPORTS = (
     "thcgw_hermitize", "thcgw_rpa_ec", "thcgw_gw_iteration", "thcgw_solve_qp", "thcgw_gm_energy",
     "thcgw_workspace_bytes", "thcgw_set_flag", "thcgw_profile_read", "thcgw_hf", "thcgw_dyson_g",
     "thcgw_nelec", "thcgw_update_mu", "thcgw_batch_create", "thcgw_batch_destroy", "thcgw_batch_last_error",
-    "thcgw_batch_set", "thcgw_batch_gw_iteration", "thcgw_pi_kspace", "thcgw_sigma_kspace",
+    "thcgw_batch_set", "thcgw_batch_set_lanes", "thcgw_batch_gw_iteration", "thcgw_pi_kspace", "thcgw_sigma_kspace",
     "thcgw_lri_create", "thcgw_lri_destroy", "thcgw_lri_last_error", "thcgw_lri_set", "thcgw_lri_dyson",
     "thcgw_lri_train", "thcgw_lri_num_pairs", "thcgw_lri_clear_pairs", "thcgw_lri_reset_state",
     "thcgw_mlp_create", "thcgw_mlp_destroy", "thcgw_mlp_last_error", "thcgw_mlp_num_params",
@@ -103,6 +103,7 @@ def load_library(path=LIB_PATH):
         "thcgw_batch_destroy": (None, [p]),
         "thcgw_batch_last_error": (ctypes.c_char_p, [p]),
         "thcgw_batch_set": (i, [p, i, i]),
+        "thcgw_batch_set_lanes": (i, [p, i]),
         "thcgw_batch_gw_iteration": (i, [p, ctypes.POINTER(p), ctypes.POINTER(p), ctypes.POINTER(p),
                                          ctypes.POINTER(p), ctypes.POINTER(p), ctypes.POINTER(d), i, p]),
         "thcgw_workspace_bytes": (l, [p]),
@@ -488,11 +489,12 @@ class ThcGW:
 class ThcBatch:
     """Independent systems (one ThcGW each, uneven sizes) iterated together: thcgw_batch_*.
 
-    The contexts share one scratch pool while the batch lives; close the batch before
-    its contexts.  The first iteration runs eagerly (and tunes the FP32 tile shape of the
-    mixed-precision systems); later ones replay one captured HIP graph."""
+    The contexts share one scratch pool while the batch lives (one per lane with lanes > 1);
+    close the batch before its contexts.  The first iteration runs eagerly (and tunes the
+    FP32 tile shape of the mixed-precision systems); later ones replay one captured HIP
+    graph, whose lanes run as parallel branches."""
 
-    def __init__(self, contexts, use_graph=True, tune=True):
+    def __init__(self, contexts, use_graph=True, tune=True, lanes=1):
         self.ctxs = list(contexts)
         self.lib = load_library()
         self._torch = self.ctxs[0]._torch
@@ -503,6 +505,13 @@ class ThcBatch:
             raise ThcgwError(rc, self.lib.thcgw_batch_last_error(None).decode())
         self.h = h
         self._check(self.lib.thcgw_batch_set(self.h, int(bool(use_graph)), int(bool(tune))))
+        if lanes != 1:
+            self.set_lanes(lanes)
+
+    def set_lanes(self, lanes):
+        """Spread the systems over `lanes` streams (parallel branches of the graph), before
+        the first iteration: thcgw_batch_set_lanes."""
+        self._check(self.lib.thcgw_batch_set_lanes(self.h, int(lanes)))
 
     def _check(self, rc):
         if rc != 0:

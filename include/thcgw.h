@@ -239,6 +239,12 @@ int thcgw_batch_create(int nsys, thcgw_ctx* const* ctxs, thcgw_batch** out);
 void thcgw_batch_destroy(thcgw_batch* b);
 const char* thcgw_batch_last_error(const thcgw_batch* b);
 int thcgw_batch_set(thcgw_batch* b, int use_graph, int tune);
+/* Spreads the systems over `lanes` (1..8) streams, largest Dyson work first onto the least
+ * loaded lane; the systems of a lane run one after another and share that lane's scratch
+ * pool, and the captured graph holds the lanes as parallel branches (the eager, uncaptured
+ * path stays one system after another).  Each lane's automatic chunk sizes take 1/lanes of
+ * the memory a single pool would.  Only before the first thcgw_batch_gw_iteration. */
+int thcgw_batch_set_lanes(thcgw_batch* b, int lanes);
 int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G_tskij, const double* const* X_skPa,
                              const double* const* V_qPQ, double* const* Sigma_tskij, double* const* W_qtPQ,
                              double* Ec, int hermitize, void* stream);

@@ -55,14 +55,18 @@ def eager(sys_):
     return S.cpu().numpy(), W.cpu().numpy(), ec
 
 
-@pytest.mark.parametrize("precision,trev,dyson", [(0, False, 0), (1, False, 0), (0, True, 0), (1, False, 2),
-                                                 (0, False, 2)])
-def test_batch_graph_replays_match_eager(orc, torch, precision, trev, dyson):
+@pytest.mark.parametrize("precision,trev,dyson,lanes", [(0, False, 0, 1), (1, False, 0, 1), (0, True, 0, 1),
+                                                       (1, False, 2, 1), (0, False, 2, 1), (1, False, 2, 2),
+                                                       (0, False, 0, 3)])
+def test_batch_graph_replays_match_eager(orc, torch, precision, trev, dyson, lanes):
     """With trev, every system is time-reversal symmetric and solves the irreducible q only
-    (flag 9), inside the captured graph too; dyson = 2 runs the resolvent Dyson form."""
-    print("case: batch precision=%d trev=%d dyson=%d systems=%s" % (precision, trev, dyson, SYSTEMS), flush=True)
+    (flag 9), inside the captured graph too; dyson = 2 runs the resolvent Dyson form; with
+    lanes > 1 the systems are spread over that many streams, parallel branches of the graph
+    (each lane with its own scratch)."""
+    print("case: batch precision=%d trev=%d dyson=%d lanes=%d systems=%s" % (precision, trev, dyson, lanes, SYSTEMS),
+          flush=True)
     systems = build(orc, torch, precision, trev, dyson)
-    b = ThcBatch([x["ctx"] for x in systems], use_graph=True, tune=(precision != 0))
+    b = ThcBatch([x["ctx"] for x in systems], use_graph=True, tune=(precision != 0), lanes=lanes)
     args = [[x[k] for x in systems] for k in ("G", "X", "V", "Sig", "W")]
     try:
         ec0 = b.gw_iteration(*args)  # eager pass (and tile tuning)

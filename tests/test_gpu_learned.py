@@ -145,8 +145,9 @@ def test_learned_core_gain(orc, torch):
     """Self-consistent loop: iterations 1-2 direct (recording pairs), training, then at
     iteration 3 the core of every mode against the exact core, before and after training,
     and the CG iterations of each mode.  What training gains is asserted against the
-    untrained network and the unpreconditioned solve; the comparison with the Woodbury
-    and carried-over cores is printed."""
+    untrained network, the unpreconditioned solve and the Woodbury core (measured: core
+    error 0.021 against 0.035, CG iterations 20 against 21); the carried-over core is
+    printed."""
     print("case: learned gain scGW kgrid=(2,2,1) no=8 nmu=64", flush=True)
     kgrid, no, nmu = (2, 2, 1), 8, 64
     s = make_system(kgrid, no, nmu, system_id=63)
@@ -181,3 +182,5 @@ def test_learned_core_gain(orc, torch):
     assert ll < 0.5 * lf
     assert errs["learned"] < untrained
     assert iters["learned"] <= iters["none"]
+    assert errs["learned"] < 0.8 * errs["woodbury"]
+    assert iters["learned"] <= iters["woodbury"]
