@@ -20,7 +20,11 @@ the plain form, which runs first):
   *_trev     the form with the Dyson step on the time-reversal irreducible q only (needs
              --trev, which draws the system time-reversal symmetric);
   *_c4       the form with the Dyson step on the q irreducible under a fourfold rotation
-             (needs --c4, which draws the system symmetric under it).
+             (needs --c4, which draws the system symmetric under it);
+  *_ibz      (plain_ibz, resolvent_ibz; needs --c4) the whole iteration on the irreducible
+             wedge of the fourfold rotation: G and Sigma on the irreducible k, chi0, W and
+             the Dyson step on the irreducible q (thcgw_gw_iteration_ibz); its errors are
+             taken at the irreducible points (no QP energies).
 Every form is reported with its speedup over plain and its errors against plain (W at
 q = 0, Sigma, Ec, and the G0W0 quasiparticle energies in meV).
 
@@ -67,13 +71,14 @@ def batch_systems(n=24):
 
 
 # form: (flag 6 Dyson algorithm, flag 7 precision, symmetry: 0 none, 1 time reversal (flag 9),
-# 2 the C4 q-point map (thcgw_set_qsym))
+# 2 the C4 q-point map (thcgw_set_qsym), 3 the C4 irreducible wedge (thcgw_set_bz))
 FORMS = {"plain": (0, 0, 0), "hermitian": (1, 0, 0), "mixed": (1, 1, 0), "mixed_bf16": (1, 2, 0),
          "resolvent": (2, 0, 0), "mixed_resolvent": (2, 1, 0),
          "plain_trev": (0, 0, 1), "hermitian_trev": (1, 0, 1), "mixed_trev": (1, 1, 1),
          "resolvent_trev": (2, 0, 1), "mixed_resolvent_trev": (2, 1, 1),
          "plain_c4": (0, 0, 2), "hermitian_c4": (1, 0, 2), "mixed_c4": (1, 1, 2),
-         "resolvent_c4": (2, 0, 2), "mixed_resolvent_c4": (2, 1, 2)}
+         "resolvent_c4": (2, 0, 2), "mixed_resolvent_c4": (2, 1, 2),
+         "plain_ibz": (0, 0, 3), "resolvent_ibz": (2, 0, 3)}
 HA_PER_MEV = 1.0 / 27211.386245988
 # GEMM profile class -> kernel template as rocprofv3 names it (64x64 tiles)
 OPS = {"N": 0, "T": 1, "C": 2}
@@ -210,7 +215,7 @@ def main():
             ap.error("unknown form %r (choose from %s)" % (f, ", ".join(FORMS)))
         if FORMS[f][2] == 1 and not args.trev:
             ap.error("form %r needs --trev" % f)
-        if FORMS[f][2] == 2 and not args.c4:
+        if FORMS[f][2] in (2, 3) and not args.c4:
             ap.error("form %r needs --c4" % f)
     if args.headline not in forms:
         forms.append(args.headline) if args.headline in FORMS else ap.error("unknown headline %r" % args.headline)
@@ -227,6 +232,7 @@ def main():
     import torch
 
     from coqui_amd import IAFT, ThcGW, make_system
+    from coqui_amd.system import make_system_bz
 
     if args.config == "batch24":
         return run_batch(args)
@@ -238,8 +244,9 @@ def main():
         args.config, kgrid, no, nmu, table, forms, args.headline))
     torch.cuda.set_device(0)
     t0 = time.perf_counter()
+    bz = None
     if args.c4:
-        s = make_system(kgrid, no, nmu, system_id=0, c4=True)
+        s, bz = make_system_bz(kgrid, no, nmu, system_id=0, kind="c4")
         s.Z = None
     else:
         s = make_system(kgrid, no, nmu, system_id=0, with_v=False, trev=args.trev)
@@ -269,6 +276,14 @@ def main():
     Wqt = torch.empty((s.nk, nth, nmu, nmu), dtype=torch.complex128, device="cuda")
     H = torch.from_numpy(s.H).cuda()
     Cd = torch.from_numpy(s.C).cuda()
+    if bz is not None:  # the irreducible wedge's inputs and outputs
+        k_ibz = torch.from_numpy(bz["order"][:bz["nk_ibz"]]).cuda()
+        q_ibz = torch.from_numpy(bz["order"][:bz["nq_ibz"]]).cuda()
+        Gi = Gd[:, :, k_ibz].contiguous()
+        Xl = Xd[torch.from_numpy(bz["order"]).cuda()].contiguous()
+        Vi = Vd[q_ibz].contiguous()
+        Sig_i = torch.empty((ia.nt, 1, bz["nk_ibz"], no, no), dtype=torch.complex128, device="cuda")
+        W_i = torch.empty((bz["nq_ibz"], nth, nmu, nmu), dtype=torch.complex128, device="cuda")
     torch.cuda.synchronize()
     log("bench: inputs ready in %.1f s" % (time.perf_counter() - t0))
 
@@ -283,6 +298,14 @@ def main():
             ctx.set_qsym(*s.qsym)
         else:
             ctx.set_qsym()
+        if FORMS[form][2] == 3:
+            ctx.set_flag(10, 0)
+            ctx.set_bz(bz)
+
+    def iterate(form):
+        if FORMS[form][2] == 3:
+            return ctx.gw_iteration_ibz(Gi, Xl, Vi, with_ec=True, hermitize=True, Sigma=Sig_i, W_qt=W_i)[2]
+        return ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)[2]
 
     def run_form(form, steps, warmup, first):
         # every form after the first gets a fresh context, so that no form runs with the
@@ -296,7 +319,7 @@ def main():
         set_form(form)
         ec = None
         for i in range(warmup):
-            _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
+            ec = iterate(form)
             log("bench: %s warmup %d done, Ec = %.12f" % (form, i, ec))
         ctx.profile_read()
         ctx.set_flag(4, 1)  # GEMM launch events during the timed steps
@@ -306,7 +329,7 @@ def main():
         ev1 = torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for i in range(steps):
-            _, _, ec = ctx.gw_iteration(Gd, Xd, Vd, with_ec=True, hermitize=True, Sigma=Sig, W_qt=Wqt)
+            ec = iterate(form)
             log("bench: %s step %d done" % (form, i))
         ev1.record(stream)
         torch.cuda.synchronize()
@@ -322,11 +345,20 @@ def main():
     results, ref = {}, None
     for i, form in enumerate(forms):
         f_ms, f_wall, f_ec, f_prof = run_form(form, args.steps, args.warmup, i == 0)
-        E_f = ctx.solve_qp(Sig, Cd, H, s.E, mu=s.mu)
+        wedge = FORMS[form][2] == 3
+        E_f = None if wedge else ctx.solve_qp(Sig, Cd, H, s.E, mu=s.mu)
         r = {"ms_per_step": round(f_ms, 3), "wall_s_timed": round(f_wall, 3), "Ec_Ha": f_ec, "prof": f_prof,
              "gemm_classes": classes(f_prof)}
         if ref is None:
             ref = {"form": form, "ms": f_ms, "Sigma": Sig.clone(), "W0": Wqt[0].clone(), "ec": f_ec, "E": E_f}
+        elif wedge:  # at the irreducible points (q = 0 is the first irreducible q)
+            rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
+            r.update({
+                "speedup_vs_%s" % ref["form"]: round(ref["ms"] / f_ms, 3),
+                "err_W_q0_rel": rel(W_i[0], ref["W0"]),
+                "err_Sigma_rel": rel(Sig_i, ref["Sigma"][:, :, k_ibz]),
+                "err_Ec_Ha": abs(f_ec - ref["ec"]),
+            })
         else:
             rel = lambda a, b: float((a - b).abs().max() / b.abs().max())
             r.update({

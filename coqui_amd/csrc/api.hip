@@ -83,6 +83,12 @@ struct thcgw_ctx {
   int sym_nir = 0;          // irreducible q (src q == q)
   int* d_sym = nullptr;     // src (nk), conj (nk), irreducible list (sym_nir), identity (nk), perm (nk * nmu)
   c128* d_symph = nullptr;  // phases (nk * nmu)
+  // Brillouin-zone symmetry (thcgw_set_bz): the wedge path's device tables
+  int bz_on = 0, bz_nk_ibz = 0, bz_nq_ibz = 0, bz_nk_nt = 0, bz_nsym = 0, bz_nterms = 0;
+  std::vector<double> bz_qw;  // multiplicity of each irreducible q
+  int* d_bz = nullptr;        // int tables at the offsets bz_o[BZ_*]
+  long bz_o[9] = {};
+  c128* d_bzD = nullptr;      // orbital rotations (nsym x nk_ibz x no x no)
   int vq() const { return dy_nq > 0 ? dy_nq : nk; }
   std::map<std::string, DevBuf> ws;
   std::string err;
@@ -153,6 +159,10 @@ struct thcgw_ctx {
     d_sym = nullptr;
     if (d_symph) (void)hipFree(d_symph);
     d_symph = nullptr;
+    if (d_bz) (void)hipFree(d_bz);
+    d_bz = nullptr;
+    if (d_bzD) (void)hipFree(d_bzD);
+    d_bzD = nullptr;
     d_tau = nullptr;
     d_Tplus = d_Tminus = d_Twt_ff = d_Ttw_ff = d_Tbeta = nullptr;
     d_wn_f = nullptr;
@@ -397,8 +407,8 @@ void run_pi(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t 
 }
 
 // Pi(iW_n, q, PQ) = sum_i T+[n][i] Pi(tau_i, q, PQ): one GEMM (nw_half x nt_half) x (nt_half x N_q N_mu^2).
-void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
-  const long cols = ctx->M * ctx->nk;
+void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s, int nq = 0) {
+  const long cols = ctx->M * (nq > 0 ? nq : ctx->nk);
   if (thcgw::ir_transform_fits(ctx->nw_half, ctx->nt_half)) {
     thcgw::ir_transform(s, ctx->d_Tplus, ctx->nw_half, ctx->nt_half, Pit, cols, 0, Piw, cols, 0, cols, 1,
                         ctx->real_Tplus);
@@ -411,17 +421,17 @@ void run_tau_to_w(thcgw_ctx* ctx, const c128* Pit, c128* Piw, hipStream_t s) {
 
 // W(q, tau_i, PQ) = sum_n T-[i][n] W(iW_n, q, PQ): per q a GEMM (nt_half x nw_half) x (nw_half x N_mu^2)
 // whose output rows land at stride N_mu^2 inside the q block of W_qtPQ.
-void run_w_to_tau(thcgw_ctx* ctx, const c128* Ww, c128* Wqt, hipStream_t s) {
-  const long M = ctx->M, nk = ctx->nk;
+void run_w_to_tau(thcgw_ctx* ctx, const c128* Ww, c128* Wqt, hipStream_t s, int nq = 0) {
+  const long M = ctx->M, nk = nq > 0 ? nq : ctx->nk;
   if (thcgw::ir_transform_fits(ctx->nt_half, ctx->nw_half)) {
     thcgw::ir_transform(s, ctx->d_Tminus, ctx->nt_half, ctx->nw_half, Ww, nk * M, M, Wqt, M, ctx->nt_half * M, M,
-                        ctx->nk, ctx->real_Tminus);
+                        static_cast<int>(nk), ctx->real_Tminus);
     return;
   }
   THCGW_REQUIRE(M < (1L << 31), "w_to_tau: N_mu^2 too large for one GEMM");
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, ctx->nt_half, static_cast<int>(M), ctx->nw_half, kOne,
                thcgw::bp(ctx->d_Tminus, 0), ctx->nw_half, thcgw::bp(Ww, M), nk * M, kZero, Wqt, ctx->nt_half * M, M,
-               ctx->nk);
+               static_cast<int>(nk));
 }
 
 int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
@@ -1027,6 +1037,152 @@ void run_sigma_kspace(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128
   if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nk);
 }
 
+// ---- Brillouin-zone symmetry: chi0 and Sigma on the irreducible wedge -------------------
+
+enum BzTab { BZ_G_SRC, BZ_G_TRANS, BZ_P_SRC, BZ_P_CONJ, BZ_P_DST, BZ_Q_GRID, BZ_X_SRC, BZ_T_OFF, BZ_T_TERMS };
+
+const int* bz_tab(const thcgw_ctx* ctx, BzTab t) { return ctx->d_bz + ctx->bz_o[t]; }
+
+void require_bz(const thcgw_ctx* ctx, const std::string& entry) {
+  THCGW_REQUIRE(ctx->bz_on, entry + ": no Brillouin-zone symmetry set (thcgw_set_bz)");
+  THCGW_REQUIRE(ctx->precision == 0, entry + ": the symmetry-reduced path runs in FP64 (flag 7 = 0)");
+  THCGW_REQUIRE(!ctx->trev && !ctx->sym, entry + ": flag 9 and the q map do not combine with thcgw_set_bz");
+}
+
+// G_PQ(tau_it) on the listed points without time reversal, from the irreducible G
+// (thc_solver_comm.hpp:395-426): B (nk_nt x M).
+void bz_project(thcgw_ctx* ctx, const c128* G, const c128* X, int it, c128* B, hipStream_t s) {
+  const int no = ctx->d.no, nmu = ctx->d.nmu, nkn = ctx->bz_nk_nt;
+  const long gs = static_cast<long>(ctx->bz_nk_ibz) * no * no;
+  c128* Gg = ctx->buf<c128>("bz_gg", static_cast<size_t>(nkn) * no * no);
+  c128* Ab = ctx->buf<c128>("bz_ab", static_cast<size_t>(std::max(nkn, ctx->bz_nsym * ctx->bz_nk_ibz)) * nmu * no);
+  thcgw::gather_blocks(s, G + it * gs, bz_tab(ctx, BZ_G_SRC), bz_tab(ctx, BZ_G_TRANS), no, nkn, Gg);
+  project_p2a(ctx, s, X, Gg, B, nkn, Ab);
+}
+
+// Pi(tau_i, q) on the irreducible q by the R-space algorithm (rpa_pi.icc:22-180 with
+// nqpts_ibz < nqpts): per tau pair, G_PQ on the points without time reversal, placed on the
+// C-order grid with the partner's conjugate on the others (rpa_pi.icc:106-113), the fused
+// k-grid kernel of the full grid, and the irreducible q gathered from its result
+// (rpa_pi.icc:160-169).
+void run_pi_ibz(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
+  const int nk = ctx->nk, nt = ctx->d.nt, nth = ctx->nt_half, nqi = ctx->bz_nq_ibz, nkn = ctx->bz_nk_nt;
+  const long M = ctx->M;
+  c128* B = ctx->buf<c128>("bz_buf", static_cast<size_t>(nkn) * M);
+  c128* GP = ctx->buf<c128>("pi_gp", static_cast<size_t>(nk) * M);
+  c128* GN = ctx->buf<c128>("pi_gn", static_cast<size_t>(nk) * M);
+  c128* PF = ctx->buf<c128>("bz_pf", static_cast<size_t>(nk) * M);
+  const double scale = -2.0 / (double(nk) * nk);  // sp = -2 for ns = 1 (rpa_pi.icc:46)
+  for (int i = 0; i < nth; ++i) {
+    bz_project(ctx, G, X, i, B, s);
+    thcgw::gather_rows(s, B, M, bz_tab(ctx, BZ_P_SRC), bz_tab(ctx, BZ_P_CONJ), nk, M, GP, M, bz_tab(ctx, BZ_P_DST));
+    bz_project(ctx, G, X, nt - 1 - i, B, s);
+    thcgw::gather_rows(s, B, M, bz_tab(ctx, BZ_P_SRC), bz_tab(ctx, BZ_P_CONJ), nk, M, GN, M, bz_tab(ctx, BZ_P_DST));
+    thcgw::pi_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP, GN, PF, M, scale);
+    thcgw::gather_rows(s, PF, M, bz_tab(ctx, BZ_Q_GRID), nullptr, nqi, M, Pi + static_cast<long>(i) * nqi * M, M,
+                       nullptr);
+  }
+}
+
+// X_{k S} D(S, k) for every operation S and irreducible k: the back-projection operands of
+// Sigma (no rotation for the identity, thc_gw.icc:316-326).  (nsym nk_ibz) x N_mu x N_orb.
+const c128* bz_xd(thcgw_ctx* ctx, const c128* X, hipStream_t s) {
+  const int no = ctx->d.no, nmu = ctx->d.nmu, nki = ctx->bz_nk_ibz, nout = ctx->bz_nsym * nki;
+  const long sX = static_cast<long>(nmu) * no, sD = static_cast<long>(no) * no;
+  c128* XD = ctx->buf<c128>("bz_xd", static_cast<size_t>(nout) * sX);
+  thcgw::gather_rows(s, X, sX, bz_tab(ctx, BZ_X_SRC), nullptr, nki, sX, XD, sX, nullptr);
+  if (nout > nki) {
+    c128* Xg = ctx->buf<c128>("bz_xg", static_cast<size_t>(nout - nki) * sX);
+    thcgw::gather_rows(s, X, sX, bz_tab(ctx, BZ_X_SRC) + nki, nullptr, nout - nki, sX, Xg, sX, nullptr);
+    thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(Xg, sX), no,
+                 thcgw::bp(ctx->d_bzD + nki * sD, sD), no, kZero, XD + nki * sX, sX, no, nout - nki);
+  }
+  return XD;
+}
+
+// Sigma(tau, k) on the irreducible k by the k-space algorithm (thc_gw.icc:152-178, 216-443):
+// per tau of both particle-hole passes, G_PQ on the points without time reversal, the
+// Hadamard sums of every operation S over the q of Qs[S] at k S (time-reversal conjugates
+// in the term flags, thc_gw.icc:366-395), Sigma_ab(k S) = X_{kS}^H Sigma_PQ X_{kS} rotated
+// by D (one product with X_{kS} D), and the sum over S.
+void run_sigma_ibz(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm,
+                   hipStream_t s) {
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const int nki = ctx->bz_nk_ibz, nqi = ctx->bz_nq_ibz, nkn = ctx->bz_nk_nt, nsym = ctx->bz_nsym;
+  const int nout = nsym * nki;
+  const long M = ctx->M, gs = static_cast<long>(nki) * no * no, sX = static_cast<long>(nmu) * no;
+  const c128* XD = bz_xd(ctx, X, s);
+  c128* B = ctx->buf<c128>("bz_buf", static_cast<size_t>(nkn) * M);
+  c128* SP = ctx->buf<c128>("bz_sp", static_cast<size_t>(nout) * M);
+  c128* SA = ctx->buf<c128>("bz_sa", static_cast<size_t>(nout) * no * no);
+  c128* Ab = ctx->buf<c128>("bz_ab", static_cast<size_t>(std::max(nkn, nout)) * nmu * no);
+  for (int pass = 0; pass < 2; ++pass) {
+    const int ntp = pass == 0 ? nth : nt / 2;  // the middle tau of an odd grid is the first pass's
+    for (int it = 0; it < ntp; ++it) {
+      const int itg = pass == 0 ? it : nt - it - 1;
+      bz_project(ctx, G, X, itg, B, s);
+      thcgw::kconv_terms(s, B, nkn, Wqt + static_cast<long>(it) * M, static_cast<long>(nth) * M, nqi,
+                         bz_tab(ctx, BZ_T_OFF), bz_tab(ctx, BZ_T_TERMS), nout, SP, M, -1.0 / nk);
+      thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(XD, sX), no, thcgw::bp(SP, M), nmu,
+                   kZero, Ab, sX, nmu, nout);
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Ab, sX), nmu, thcgw::bp(XD, sX), no,
+                   kZero, SA, static_cast<long>(no) * no, no, nout);
+      thcgw::sum_groups(s, SA, nsym, gs, Sig + itg * gs);
+    }
+  }
+  if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nki);
+}
+
+// Pi -> W on the wedge and Ec with each irreducible q weighted by its multiplicity
+// (thc_rpa.icc:130-160); Sigma on the wedge.
+void run_iteration_ibz(thcgw_ctx* ctx, const c128* G, const c128* X, const c128* V, c128* Sig, c128* Wqt, double* Ec,
+                       bool herm, hipStream_t s) {
+  const int nqi = ctx->bz_nq_ibz, nwh = ctx->nw_half;
+  const long M = ctx->M;
+  const int total = nwh * nqi;
+  c128* Piw = ctx->buf<c128>("it_piw", static_cast<size_t>(total) * M);
+  c128* dtr = Ec ? ctx->buf<c128>("ec_tr", total) : nullptr;
+  double* dld = Ec ? ctx->buf<double>("ec_ld", total) : nullptr;
+  run_pi_ibz(ctx, G, X, Wqt, s);  // W_qtPQ doubles as Pi_tqPQ scratch (same size)
+  run_tau_to_w(ctx, Wqt, Piw, s, nqi);
+  {
+    struct Restore {
+      thcgw_ctx* c;
+      ~Restore() { c->dy_nq = 0; }
+    } restore{ctx};
+    ctx->dy_nq = nqi;
+    int* info = ctx->buf<int>("dy_info", static_cast<size_t>(total));
+    const VFactors f = factor_dyson_v(ctx, V, s);
+    dyson_matrices(ctx, Piw, V, f, total, dtr, dld, true, false, info, s);
+    std::vector<int> h(total);
+    THCGW_HIP(hipMemcpyAsync(h.data(), info, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    for (int b = 0; b < total; ++b)
+      if (h[b] != 0)
+        throw thcgw::Error(THCGW_ERR_SINGULAR, "Dyson: (I - V Pi) is singular at (n, q_ibz) = (" +
+                                                   std::to_string(b / nqi) + ", " + std::to_string(b % nqi) + ")");
+  }
+  run_w_to_tau(ctx, Piw, Wqt, s, nqi);
+  run_sigma_ibz(ctx, G, Wqt, X, Sig, herm, s);
+  if (Ec) {
+    std::vector<std::complex<double>> tr(total);
+    std::vector<double> ld(total);
+    THCGW_HIP(hipMemcpyAsync(tr.data(), dtr, sizeof(c128) * total, hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipMemcpyAsync(ld.data(), dld, sizeof(double) * total, hipMemcpyDeviceToHost, s));
+    THCGW_HIP(hipStreamSynchronize(s));
+    const int nwb = ctx->d.nw_b;
+    std::vector<std::complex<double>> ew(nwb, 0.0);
+    for (int n = 0; n < nwh; ++n)
+      for (int q = 0; q < nqi; ++q) {
+        const std::complex<double> v = ctx->bz_qw[q] * (tr[n * nqi + q] + ld[n * nqi + q]);
+        const int wn = nwb / 2 + n, mwn = nwb / 2 - n;
+        ew[wn] += v;
+        if (mwn != wn) ew[mwn] += v;
+      }
+    *Ec = thcgw::host::ec_from_ew(ctx->tab, ew, ctx->nk);
+  }
+}
+
 // ---- one iteration as enqueue + finish (the batch captures the enqueue part) -----------
 
 void enqueue_iteration(thcgw_ctx* ctx, const c128* G, const c128* X, const c128* V, c128* Sig, c128* Wqt, bool want_ec,
@@ -1354,6 +1510,127 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
   });
 }
 
+// Brillouin-zone symmetry tables (bz_symmetry.hpp:497-576): every index is checked on the
+// host, then the wedge path's gather lists and Sigma term lists go to the device.
+int thcgw_set_bz(thcgw_ctx* ctx, const thcgw_bz* bz) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    if (ctx->d_bz) THCGW_HIP(hipFree(ctx->d_bz));
+    if (ctx->d_bzD) THCGW_HIP(hipFree(ctx->d_bzD));
+    ctx->d_bz = nullptr;
+    ctx->d_bzD = nullptr;
+    ctx->bz_on = 0;
+    if (!bz) return;
+    const int nk = ctx->nk, no = ctx->d.no;
+    const int nki = bz->nk_ibz, nqi = bz->nq_ibz, ntr = bz->nk_trev_pairs, nsym = bz->nsym;
+    THCGW_REQUIRE(bz->k_grid_index && bz->kp_to_ibz && bz->kp_trev && bz->kp_trev_pair && bz->qk_to_k2 &&
+                      bz->qminus && bz->qp_to_ibz && bz->qp_trev && bz->nq_per_s && bz->Qs && bz->ks_to_k && bz->D,
+                  "thcgw_set_bz: null table");
+    THCGW_REQUIRE(nk <= 64, "thcgw_set_bz: the symmetry-reduced path takes at most 64 k-points");
+    THCGW_REQUIRE(ntr >= 0 && ntr < nk && nki >= 1 && nki <= nk - ntr && nqi >= 1 && nqi <= nk,
+                  "thcgw_set_bz: nk_ibz, nq_ibz or nk_trev_pairs out of range");
+    THCGW_REQUIRE(nsym >= 1 && nsym <= 1024, "thcgw_set_bz: nsym out of range");
+    const int nkn = nk - ntr;
+    auto in = [](int v, int lo, int hi) { return v >= lo && v < hi; };
+    std::vector<char> seen(nk, 0);
+    for (int k = 0; k < nk; ++k) {
+      const int g = bz->k_grid_index[k];
+      THCGW_REQUIRE(in(g, 0, nk) && !seen[g], "thcgw_set_bz: k_grid_index is not a permutation of the grid");
+      seen[g] = 1;
+      THCGW_REQUIRE(bz->kp_trev[k] == 0 || bz->kp_trev[k] == 1, "thcgw_set_bz: kp_trev is 0 or 1");
+      THCGW_REQUIRE((bz->kp_trev[k] != 0) == (k >= nkn),
+                    "thcgw_set_bz: the nk_trev_pairs time-reversal points must be listed last");
+      THCGW_REQUIRE(in(bz->kp_to_ibz[k], 0, nki), "thcgw_set_bz: kp_to_ibz out of range");
+      THCGW_REQUIRE(k >= nki || bz->kp_to_ibz[k] == k, "thcgw_set_bz: the irreducible k-points must be listed first");
+      THCGW_REQUIRE(!bz->kp_trev[k] || in(bz->kp_trev_pair[k], 0, nkn),
+                    "thcgw_set_bz: a time-reversal point needs a partner without time reversal");
+      THCGW_REQUIRE(in(bz->qminus[k], 0, nk) && in(bz->qp_to_ibz[k], 0, nqi) &&
+                        (bz->qp_trev[k] == 0 || bz->qp_trev[k] == 1),
+                    "thcgw_set_bz: qminus, qp_to_ibz or qp_trev out of range");
+      THCGW_REQUIRE(k >= nqi || (bz->qp_to_ibz[k] == k && !bz->qp_trev[k]),
+                    "thcgw_set_bz: the irreducible q-points must be listed first");
+    }
+    for (long e = 0; e < static_cast<long>(nk) * nk; ++e)
+      THCGW_REQUIRE(in(bz->qk_to_k2[e], 0, nk), "thcgw_set_bz: qk_to_k2 out of range");
+    std::vector<char> used(nk, 0);
+    long tot = 0;
+    for (int is = 0; is < nsym; ++is) {
+      const int n = bz->nq_per_s[is];
+      THCGW_REQUIRE(n >= 0 && n <= nk, "thcgw_set_bz: nq_per_s out of range");
+      tot += n;
+      for (int j = 0; j < n; ++j) {
+        const int q = bz->Qs[static_cast<long>(is) * nk + j];
+        THCGW_REQUIRE(in(q, 0, nk) && !used[q], "thcgw_set_bz: every q must appear in exactly one row of Qs");
+        used[q] = 1;
+      }
+      for (int ik = 0; ik < nki; ++ik)
+        THCGW_REQUIRE(in(bz->ks_to_k[static_cast<long>(is) * nki + ik], 0, nkn),
+                      "thcgw_set_bz: k S must be a listed point without time reversal");
+    }
+    THCGW_REQUIRE(tot == nk, "thcgw_set_bz: nq_per_s must sum to N_k");
+    std::vector<double> qw(nqi, 0.0);
+    for (int q = 0; q < nk; ++q) qw[bz->qp_to_ibz[q]] += 1.0;
+    // device int tables: G gather (nkn, nkn), placement (nk x 3), irreducible q (nqi),
+    // X_{kS} (nsym nki), Sigma term offsets (nsym nki + 1) and terms (nki nk)
+    const int nout = nsym * nki;
+    std::vector<int> h;
+    long o[9];
+    o[BZ_G_SRC] = 0;
+    for (int k = 0; k < nkn; ++k) h.push_back(bz->kp_to_ibz[k]);
+    o[BZ_G_TRANS] = static_cast<long>(h.size());
+    for (int k = 0; k < nkn; ++k) h.push_back(bz->kp_trev[k]);
+    o[BZ_P_SRC] = static_cast<long>(h.size());
+    for (int k = 0; k < nk; ++k) h.push_back(bz->kp_trev[k] ? bz->kp_trev_pair[k] : k);
+    o[BZ_P_CONJ] = static_cast<long>(h.size());
+    for (int k = 0; k < nk; ++k) h.push_back(bz->kp_trev[k]);
+    o[BZ_P_DST] = static_cast<long>(h.size());
+    for (int k = 0; k < nk; ++k) h.push_back(bz->k_grid_index[k]);
+    o[BZ_Q_GRID] = static_cast<long>(h.size());
+    for (int q = 0; q < nqi; ++q) h.push_back(bz->k_grid_index[q]);
+    o[BZ_X_SRC] = static_cast<long>(h.size());
+    for (int e = 0; e < nout; ++e) h.push_back(bz->ks_to_k[e]);
+    // Sigma terms of thc_gw.icc:366-395: A = G_PQ rows (points without time reversal),
+    // B = W rows (irreducible q); conj A where k - q needs time reversal (its partner's G),
+    // conj B where q needs it (W(q) = conj W(q_ibz))
+    std::vector<int> off(nout + 1, 0), terms;
+    for (int is = 0; is < nsym; ++is)
+      for (int ik = 0; ik < nki; ++ik) {
+        const int ks = bz->ks_to_k[static_cast<long>(is) * nki + ik];
+        for (int j = 0; j < bz->nq_per_s[is]; ++j) {
+          const int qp = bz->Qs[static_cast<long>(is) * nk + j];
+          const int qs = bz->qp_to_ibz[qp];
+          const bool cb = bz->qp_trev[qp] != 0;
+          const int k2 = cb ? bz->qk_to_k2[static_cast<long>(bz->qminus[qs]) * nk + ks]   // ks + qs
+                            : bz->qk_to_k2[static_cast<long>(qs) * nk + ks];              // ks - qs
+          const bool ca = bz->kp_trev[k2] != 0;
+          const int ia = ca ? bz->kp_trev_pair[k2] : k2;
+          THCGW_REQUIRE(in(ia, 0, nkn) && in(qs, 0, nqi), "thcgw_set_bz: inconsistent tables");
+          terms.push_back(ia | (qs << 8) | (int(ca) << 16) | (int(cb) << 17));
+        }
+        off[is * nki + ik + 1] = static_cast<int>(terms.size());
+      }
+    o[BZ_T_OFF] = static_cast<long>(h.size());
+    h.insert(h.end(), off.begin(), off.end());
+    o[BZ_T_TERMS] = static_cast<long>(h.size());
+    h.insert(h.end(), terms.begin(), terms.end());
+    int* d = nullptr;
+    THCGW_HIP(hipMalloc(&d, sizeof(int) * h.size()));
+    ctx->d_bz = d;
+    THCGW_HIP(hipMemcpy(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    const size_t nD = static_cast<size_t>(nout) * no * no;
+    THCGW_HIP(hipMalloc(&ctx->d_bzD, sizeof(c128) * nD));
+    THCGW_HIP(hipMemcpy(ctx->d_bzD, bz->D, sizeof(c128) * nD, hipMemcpyHostToDevice));
+    std::copy(o, o + 9, ctx->bz_o);
+    ctx->bz_nk_ibz = nki;
+    ctx->bz_nq_ibz = nqi;
+    ctx->bz_nk_nt = nkn;
+    ctx->bz_nsym = nsym;
+    ctx->bz_nterms = static_cast<int>(terms.size());
+    ctx->bz_qw = qw;
+    ctx->bz_on = 1;
+  });
+}
+
 int thcgw_profile_read(thcgw_ctx* ctx, double* ms, double* flops, double* bytes, long* launches) {
   if (!ctx) return THCGW_ERR_ARG;
   return guarded(ctx, [&] {
@@ -1455,6 +1732,43 @@ int thcgw_sigma_kspace(thcgw_ctx* ctx, const double* G, const double* Wqt, const
   return guarded(ctx, [&] {
     THCGW_REQUIRE(G && Wqt && X && Sig, "thcgw_sigma_kspace: null pointer");
     run_sigma_kspace(ctx, C(G), C(Wqt), C(X), Cm(Sig), herm != 0, S(stream));
+  });
+}
+
+int thcgw_pi_ibz(thcgw_ctx* ctx, const double* G, const double* X, double* Pi, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    require_bz(ctx, "thcgw_pi_ibz");
+    THCGW_REQUIRE(G && X && Pi, "thcgw_pi_ibz: null pointer");
+    run_pi_ibz(ctx, C(G), C(X), Cm(Pi), S(stream));
+  });
+}
+
+int thcgw_sigma_ibz(thcgw_ctx* ctx, const double* G, const double* Wqt, const double* X, double* Sig, int herm,
+                    void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    require_bz(ctx, "thcgw_sigma_ibz");
+    THCGW_REQUIRE(G && Wqt && X && Sig, "thcgw_sigma_ibz: null pointer");
+    run_sigma_ibz(ctx, C(G), C(Wqt), C(X), Cm(Sig), herm != 0, S(stream));
+  });
+}
+
+int thcgw_gw_iteration_ibz(thcgw_ctx* ctx, const double* G, const double* X, const double* V, double* Sig,
+                           double* Wqt, double* Ec, int herm, void* stream) {
+  if (!ctx) return THCGW_ERR_ARG;
+  return guarded(ctx, [&] {
+    require_bz(ctx, "thcgw_gw_iteration_ibz");
+    THCGW_REQUIRE(G && X && V && Sig && Wqt, "thcgw_gw_iteration_ibz: null pointer");
+    hipStream_t s = S(stream);
+    const long no = ctx->d.no;
+    check_finite(ctx, "gw_iteration_ibz",
+                 {{G, ctx->d.nt * ctx->bz_nk_ibz * no * no, "G_tskij"},
+                  {X, static_cast<long>(ctx->nk) * ctx->d.nmu * no, "X_skPa"},
+                  {V, ctx->bz_nq_ibz * ctx->M, "V_qPQ"}},
+                 s);
+    run_iteration_ibz(ctx, C(G), C(X), C(V), Cm(Sig), Cm(Wqt), Ec, herm != 0, s);
+    THCGW_HIP(hipStreamSynchronize(s));
   });
 }
 

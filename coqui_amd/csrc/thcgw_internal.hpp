@@ -136,6 +136,14 @@ void kconv_pi(hipStream_t s, const c128* Gp, const c128* Gn, const int* kmq_qk, 
 void kconv_sigma(hipStream_t s, const c128* G, const c128* W, long w_kstride, const int* kmq_kq, int nk, c128* S,
                  long M, double scale);
 
+// symk.hip: Brillouin-zone symmetry (chi0 and Sigma on the irreducible wedge)
+void gather_blocks(hipStream_t s, const c128* src, const int* sidx, const int* trans, int n, int count, c128* dst);
+void gather_rows(hipStream_t s, const c128* src, long sstride, const int* sidx, const int* cj, int count, long len,
+                 c128* dst, long dstride, const int* dmap);
+void kconv_terms(hipStream_t s, const c128* A, int nA, const c128* B, long b_stride, int nB, const int* off,
+                 const int* terms, int nout, c128* out, long M, double scale);
+void sum_groups(hipStream_t s, const c128* src, int ng, long per, c128* dst);
+
 // lu.hip
 void getrf_batched(hipStream_t s, c128* A, long sA, int lda, int n, int* ipiv, int* info, int batch);
 void trsm_right_upper_unit(hipStream_t s, const c128* U, long sU, int ldu, c128* T, long sT, int ldt, int m, int nb,

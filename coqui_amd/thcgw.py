@@ -29,7 +29,7 @@ EXPORTS = (
     "thcgw_mlp_create", "thcgw_mlp_destroy", "thcgw_mlp_last_error", "thcgw_mlp_num_params",
     "thcgw_mlp_get_params", "thcgw_mlp_set_params", "thcgw_mlp_forward", "thcgw_mlp_backward",
     "thcgw_mlp_adam_step", "thcgw_mlp_train_mse", "thcgw_eps_inv_head", "thcgw_sigma_div_correction",
-    "thcgw_set_qsym",
+    "thcgw_set_qsym", "thcgw_set_bz", "thcgw_pi_ibz", "thcgw_sigma_ibz", "thcgw_gw_iteration_ibz",
 )
 
 ERRORS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "singular matrix", 4: "unsupported", 5: "allocation",
@@ -51,6 +51,14 @@ class _Tables(ctypes.Structure):
     _fields_ = [("tau_mesh_f", ctypes.c_void_p), ("wn_mesh_f", ctypes.c_void_p), ("Ttw_ff", ctypes.c_void_p),
                 ("Twt_ff", ctypes.c_void_p), ("Ttw_bb", ctypes.c_void_p), ("Twt_bb", ctypes.c_void_p),
                 ("T_beta_t_ff", ctypes.c_void_p)]
+
+
+class _Bz(ctypes.Structure):
+    """thcgw_bz"""
+    _fields_ = [("nk_ibz", ctypes.c_int), ("nq_ibz", ctypes.c_int), ("nk_trev_pairs", ctypes.c_int),
+                ("nsym", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in (
+                    "k_grid_index", "kp_to_ibz", "kp_trev", "kp_trev_pair", "qk_to_k2", "qminus", "qp_to_ibz",
+                    "qp_trev", "nq_per_s", "Qs", "ks_to_k", "D")]
 
 
 class LriStats(ctypes.Structure):
@@ -87,6 +95,10 @@ def load_library(path=LIB_PATH):
         "thcgw_tau_to_w": (i, [p, p, p, p]),
         "thcgw_pi_kspace": (i, [p, p, p, p, p]),
         "thcgw_sigma_kspace": (i, [p, p, p, p, p, i, p]),
+        "thcgw_set_bz": (i, [p, p]),
+        "thcgw_pi_ibz": (i, [p, p, p, p, p]),
+        "thcgw_sigma_ibz": (i, [p, p, p, p, p, i, p]),
+        "thcgw_gw_iteration_ibz": (i, [p, p, p, p, p, p, p, i, p]),
         "thcgw_dyson_w": (i, [p, p, p, p, p, p]),
         "thcgw_w_to_tau": (i, [p, p, p, p]),
         "thcgw_sigma": (i, [p, p, p, p, p, i, p]),
@@ -288,6 +300,75 @@ class ThcGW:
                       None if phase is None else np.ascontiguousarray(phase, np.complex128),
                       None if conj is None else np.ascontiguousarray(conj, np.int32)]
         self._check(self.lib.thcgw_set_qsym(self.h, *[None if a is None else _np_ptr(a) for a in self._qsym]))
+
+    def set_bz(self, bz=None):
+        """Brillouin-zone symmetry (thcgw_set_bz) from a dict of the tables of
+        coqui_amd.system.make_system_bz ("order" is the k_grid_index); None clears it."""
+        if bz is None:
+            self._check(self.lib.thcgw_set_bz(self.h, None))
+            self._bz = None
+            return
+        ints = {n: np.ascontiguousarray(bz[n], np.int32) for n in (
+            "kp_to_ibz", "kp_trev", "kp_trev_pair", "qk_to_k2", "qminus", "qp_to_ibz", "qp_trev", "nq_per_s", "Qs",
+            "ks_to_k")}
+        ints["k_grid_index"] = np.ascontiguousarray(bz["order"], np.int32)
+        D = np.ascontiguousarray(bz["D"], np.complex128)
+        nsym = int(len(ints["nq_per_s"]))
+        nk_ibz, nq_ibz = int(bz["nk_ibz"]), int(bz["nq_ibz"])
+        for name, shape in (("k_grid_index", (self.nk,)), ("kp_to_ibz", (self.nk,)), ("kp_trev", (self.nk,)),
+                            ("kp_trev_pair", (self.nk,)), ("qk_to_k2", (self.nk, self.nk)), ("qminus", (self.nk,)),
+                            ("qp_to_ibz", (self.nk,)), ("qp_trev", (self.nk,)), ("Qs", (nsym, self.nk)),
+                            ("ks_to_k", (nsym, nk_ibz))):
+            if ints[name].shape != shape:
+                raise ValueError("bz[%s]: expected shape %s, got %s" % (name, shape, ints[name].shape))
+        if D.shape != (nsym, nk_ibz, self.no, self.no):
+            raise ValueError("bz[D]: expected shape %s, got %s" % ((nsym, nk_ibz, self.no, self.no), D.shape))
+        keep = dict(ints, D=D)
+        st = _Bz(nk_ibz, nq_ibz, int(bz["nk_trev_pairs"]), nsym,
+                 *[_np_ptr(keep[n]) for n in ("k_grid_index", "kp_to_ibz", "kp_trev", "kp_trev_pair", "qk_to_k2",
+                                              "qminus", "qp_to_ibz", "qp_trev", "nq_per_s", "Qs", "ks_to_k", "D")])
+        self._check(self.lib.thcgw_set_bz(self.h, ctypes.byref(st)))
+        self._bz = (nk_ibz, nq_ibz)
+
+    def _bz_dims(self):
+        if getattr(self, "_bz", None) is None:
+            raise ValueError("no Brillouin-zone symmetry set (set_bz)")
+        return self._bz
+
+    def pi_ibz(self, G, X, out=None, stream=None):
+        """chi0 on the irreducible q (rpa_pi.icc:22-180 with symmetry): G on the irreducible k."""
+        nki, nqi = self._bz_dims()
+        G = self._dev(G, (self.nt, 1, nki, self.no, self.no), "G")
+        X = self._dev(X, self._x(), "X")
+        P = self._out(out, (self.nt_half, nqi, self.nmu, self.nmu), "out")
+        self._check(self.lib.thcgw_pi_ibz(self.h, _ptr(G), _ptr(X), _ptr(P), self._stream(stream)))
+        return P
+
+    def sigma_ibz(self, G, W_qt, X, hermitize=True, out=None, stream=None):
+        """Sigma on the irreducible k (thc_gw.icc:216-443)."""
+        nki, nqi = self._bz_dims()
+        g = (self.nt, 1, nki, self.no, self.no)
+        G, X = self._dev(G, g, "G"), self._dev(X, self._x(), "X")
+        W_qt = self._dev(W_qt, (nqi, self.nt_half, self.nmu, self.nmu), "W_qt")
+        S = self._out(out, g, "out")
+        self._check(self.lib.thcgw_sigma_ibz(self.h, _ptr(G), _ptr(W_qt), _ptr(X), _ptr(S), int(bool(hermitize)),
+                                             self._stream(stream)))
+        return S
+
+    def gw_iteration_ibz(self, G, X, V, with_ec=True, hermitize=True, Sigma=None, W_qt=None, stream=None):
+        """The iteration on the irreducible wedge: returns (Sigma (irreducible k), W_qtPQ
+        (irreducible q), Ec or None)."""
+        nki, nqi = self._bz_dims()
+        g = (self.nt, 1, nki, self.no, self.no)
+        G, X = self._dev(G, g, "G"), self._dev(X, self._x(), "X")
+        V = self._dev(V, (nqi, self.nmu, self.nmu), "V")
+        S = self._out(Sigma, g, "Sigma")
+        W = self._out(W_qt, (nqi, self.nt_half, self.nmu, self.nmu), "W_qt")
+        ec = ctypes.c_double()
+        self._check(self.lib.thcgw_gw_iteration_ibz(self.h, _ptr(G), _ptr(X), _ptr(V), _ptr(S), _ptr(W),
+                                                    ctypes.byref(ec) if with_ec else None, int(bool(hermitize)),
+                                                    self._stream(stream)))
+        return S, W, (ec.value if with_ec else None)
 
     def profile_read(self):
         """{class: (ms, flops, launches, bytes)} of the GEMM launches recorded since the last

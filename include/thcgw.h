@@ -126,6 +126,60 @@ int thcgw_pi_kspace(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa,
 int thcgw_sigma_kspace(thcgw_ctx* ctx, const double* G_tskij, const double* W_qtPQ, const double* X_skPa,
                        double* Sigma_tskij, int hermitize, void* stream);
 
+/* f4: Brillouin-zone symmetry of the mean field, in the layout of the reference's
+ * bz_symm (src/mean_field/symmetry/bz_symmetry.hpp:497-576).  The N_k k-points are listed
+ * with the nk_ibz irreducible ones first and the nk_trev_pairs points that need time
+ * reversal last; the q-points are listed the same way.  All tables are host int arrays:
+ *   k_grid_index  (N_k)        C-order position (i n2 + j) n3 + l of each listed point on
+ *                              the Gamma-centred k-grid of thcgw_dims;
+ *   kp_to_ibz, kp_trev, kp_trev_pair (N_k)    as in bz_symm (kp_trev_pair < 0: no pair);
+ *   qk_to_k2      (N_k x N_k)  [q][a] = b with k_b = k_a - q;
+ *   qminus, qp_to_ibz, qp_trev (N_k);
+ *   nq_per_s      (nsym), Qs (nsym x N_k, the first nq_per_s[s] of a row used),
+ *   ks_to_k       (nsym x nk_ibz): the operations qsymms of bz_symm, identity first;
+ *   D             (nsym x nk_ibz x N_orb x N_orb complex, host doubles): the orbital rotation
+ *                 of symmetry_rotation(s, k) (MF.hpp), applied as Sigma(k) += D^H Sigma(k S) D
+ *                 (ignored for s = 0).
+ * thcgw_set_bz checks every index (N_k <= 64, nq_ibz <= 64, irreducible points first, time-
+ * reversal points last with partners among the others, every q in exactly one Qs row,
+ * k S never a time-reversal point); bz = NULL clears it. */
+typedef struct thcgw_bz {
+  int nk_ibz, nq_ibz, nk_trev_pairs, nsym;
+  const int* k_grid_index;
+  const int* kp_to_ibz;
+  const int* kp_trev;
+  const int* kp_trev_pair;
+  const int* qk_to_k2;
+  const int* qminus;
+  const int* qp_to_ibz;
+  const int* qp_trev;
+  const int* nq_per_s;
+  const int* Qs;
+  const int* ks_to_k;
+  const double* D;
+} thcgw_bz;
+int thcgw_set_bz(thcgw_ctx* ctx, const thcgw_bz* bz);
+
+/* With a symmetry set (FP64, flags 7 and 9 and the q map off): G_tskij holds the nk_ibz
+ * irreducible k-points (nt x nk_ibz x N_orb^2), X_skPa all N_k listed points.
+ *   thcgw_pi_ibz: Pi_tqPQ (nt_half x nq_ibz x N_mu^2) on the irreducible q by the R-space
+ *     algorithm: G_PQ on the points without time reversal, the conjugate of the partner's
+ *     on the others, the k-grid transform, the Hadamard product and the transform back to
+ *     the irreducible q only (rpa_pi.icc:22-180 with nqpts_ibz < nqpts).
+ *   thcgw_sigma_ibz: Sigma_tskij (nt x nk_ibz x N_orb^2) from W_qtPQ (nq_ibz x nt_half x
+ *     N_mu^2) by the k-space algorithm: per operation s, the Hadamard sums over the q of
+ *     Qs[s] at k S with the time-reversal conjugates, back-projected with X_{kS}, rotated by
+ *     D and summed, in both particle-hole passes (thc_gw.icc:152-178, 216-443).
+ *   thcgw_gw_iteration_ibz: the iteration of thcgw_gw_iteration on the wedge: Pi, tau ->
+ *     iW, Dyson for the irreducible q with V_qPQ (nq_ibz x N_mu^2), iW -> tau, Sigma; Ec
+ *     weighs each irreducible q by its multiplicity (thc_rpa.icc:130-160).
+ * Enqueue only, except the iteration, which synchronises for the Dyson flags and Ec. */
+int thcgw_pi_ibz(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa, double* Pi_tqPQ, void* stream);
+int thcgw_sigma_ibz(thcgw_ctx* ctx, const double* G_tskij, const double* W_qtPQ, const double* X_skPa,
+                    double* Sigma_tskij, int hermitize, void* stream);
+int thcgw_gw_iteration_ibz(thcgw_ctx* ctx, const double* G_tskij, const double* X_skPa, const double* V_qPQ,
+                           double* Sigma_tskij, double* W_qtPQ, double* Ec, int hermitize, void* stream);
+
 /* a5: Pi_tqPQ -> Pi_wqPQ with the particle-hole folded bosonic transform.  Replaces
  * scr_coulomb_t::tau_to_w (scr_coulomb_t.cpp:349-376) -> IAFT::tau_to_w_PHsym (IAFT.icc:49-73). */
 int thcgw_tau_to_w(thcgw_ctx* ctx, const double* Pi_tqPQ, double* Pi_wqPQ, void* stream);
