@@ -299,7 +299,6 @@ def main():
         else:
             ctx.set_qsym()
         if FORMS[form][2] == 3:
-            ctx.set_flag(10, 0)
             ctx.set_bz(bz)
 
     def iterate(form):

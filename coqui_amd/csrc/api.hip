@@ -87,7 +87,7 @@ struct thcgw_ctx {
   int bz_on = 0, bz_nk_ibz = 0, bz_nq_ibz = 0, bz_nk_nt = 0, bz_nsym = 0, bz_nterms = 0;
   std::vector<double> bz_qw;  // multiplicity of each irreducible q
   int* d_bz = nullptr;        // int tables at the offsets bz_o[BZ_*]
-  long bz_o[9] = {};
+  long bz_o[10] = {};
   c128* d_bzD = nullptr;      // orbital rotations (nsym x nk_ibz x no x no)
   int vq() const { return dy_nq > 0 ? dy_nq : nk; }
   std::map<std::string, DevBuf> ws;
@@ -253,8 +253,8 @@ size_t free_device_bytes() {
 // O_out[b,k] = X_k O_in[b,k] X_k^H for nbk = nb*nk matrices (thc_solver_comm.hpp:422,427).
 // herm: O_in is Hermitian, so is O_out; its lower tiles are formed and mirrored.
 void project_p2a(thcgw_ctx* ctx, hipStream_t s, const c128* X, const c128* Oin, c128* Oout, long nbk, c128* Abuf,
-                 bool herm = false) {
-  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
+                 bool herm = false, int xmod = 0) {
+  const int nk = xmod > 0 ? xmod : ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
   thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(X, sX, nk), no,
@@ -346,11 +346,11 @@ std::string sci(double v);
 
 // Flag 10 refuses G(tau, k) that is not Hermitian: max |G - G^H| over all (tau, k) against
 // 1e-12 of max |G| (G built from a Hermitian H(k) is Hermitian to rounding).
-void check_herm_g(thcgw_ctx* ctx, const c128* G, hipStream_t s) {
+void check_herm_g(thcgw_ctx* ctx, const c128* G, hipStream_t s, int nk_g = 0) {
   if (!ctx->herm_g || ctx->deferred) return;
   const int nbx = 256;
   double* d = ctx->buf<double>("hg_chk", static_cast<size_t>(2) * nbx);
-  thcgw::herm_diff(s, G, ctx->d.no, static_cast<long>(ctx->d.nt) * ctx->nk, nbx, d);
+  thcgw::herm_diff(s, G, ctx->d.no, static_cast<long>(ctx->d.nt) * (nk_g > 0 ? nk_g : ctx->nk), nbx, d);
   std::vector<double> h(static_cast<size_t>(2) * nbx);
   THCGW_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
   THCGW_HIP(hipStreamSynchronize(s));
@@ -1039,7 +1039,11 @@ void run_sigma_kspace(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128
 
 // ---- Brillouin-zone symmetry: chi0 and Sigma on the irreducible wedge -------------------
 
-enum BzTab { BZ_G_SRC, BZ_G_TRANS, BZ_P_SRC, BZ_P_CONJ, BZ_P_DST, BZ_Q_GRID, BZ_X_SRC, BZ_T_OFF, BZ_T_TERMS };
+enum BzTab {
+  BZ_G_SRC, BZ_G_TRANS,                       // Sigma: G of the listed points without time reversal
+  BZ_PG_SRC, BZ_PG_TRANS, BZ_PG_CONJ, BZ_PG_X, // chi0: G and X of every grid point (partners conjugated)
+  BZ_Q_GRID, BZ_X_SRC, BZ_T_OFF, BZ_T_TERMS
+};
 
 const int* bz_tab(const thcgw_ctx* ctx, BzTab t) { return ctx->d_bz + ctx->bz_o[t]; }
 
@@ -1049,38 +1053,44 @@ void require_bz(const thcgw_ctx* ctx, const std::string& entry) {
   THCGW_REQUIRE(!ctx->trev && !ctx->sym, entry + ": flag 9 and the q map do not combine with thcgw_set_bz");
 }
 
-// G_PQ(tau_it) on the listed points without time reversal, from the irreducible G
-// (thc_solver_comm.hpp:395-426): B (nk_nt x M).
-void bz_project(thcgw_ctx* ctx, const c128* G, const c128* X, int it, c128* B, hipStream_t s) {
-  const int no = ctx->d.no, nmu = ctx->d.nmu, nkn = ctx->bz_nk_nt;
-  const long gs = static_cast<long>(ctx->bz_nk_ibz) * no * no;
-  c128* Gg = ctx->buf<c128>("bz_gg", static_cast<size_t>(nkn) * no * no);
-  c128* Ab = ctx->buf<c128>("bz_ab", static_cast<size_t>(std::max(nkn, ctx->bz_nsym * ctx->bz_nk_ibz)) * nmu * no);
-  thcgw::gather_blocks(s, G + it * gs, bz_tab(ctx, BZ_G_SRC), bz_tab(ctx, BZ_G_TRANS), no, nkn, Gg);
-  project_p2a(ctx, s, X, Gg, B, nkn, Ab);
-}
-
 // Pi(tau_i, q) on the irreducible q by the R-space algorithm (rpa_pi.icc:22-180 with
-// nqpts_ibz < nqpts): per tau pair, G_PQ on the points without time reversal, placed on the
-// C-order grid with the partner's conjugate on the others (rpa_pi.icc:106-113), the fused
-// k-grid kernel of the full grid, and the irreducible q gathered from its result
-// (rpa_pi.icc:160-169).
+// nqpts_ibz < nqpts).  The reference projects G on the points without time reversal and
+// takes each time-reversal point's G_PQ as the conjugate of its partner's
+// (rpa_pi.icc:106-113).  Here every grid point is projected directly in C-order grid
+// position, a time-reversal point from the conjugated X and G of its partner:
+// conj(X) conj(G) conj(X)^H is that conjugate exactly.  Then, per tau pair, the fused
+// k-grid kernel of the full grid and the irreducible q gathered from its result
+// (rpa_pi.icc:160-169).  Projections are batched over tau pairs.
 void run_pi_ibz(thcgw_ctx* ctx, const c128* G, const c128* X, c128* Pi, hipStream_t s) {
-  const int nk = ctx->nk, nt = ctx->d.nt, nth = ctx->nt_half, nqi = ctx->bz_nq_ibz, nkn = ctx->bz_nk_nt;
-  const long M = ctx->M;
-  c128* B = ctx->buf<c128>("bz_buf", static_cast<size_t>(nkn) * M);
-  c128* GP = ctx->buf<c128>("pi_gp", static_cast<size_t>(nk) * M);
-  c128* GN = ctx->buf<c128>("pi_gn", static_cast<size_t>(nk) * M);
+  check_herm_g(ctx, G, s, ctx->bz_nk_ibz);
+  const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
+  const int nqi = ctx->bz_nq_ibz;
+  const long M = ctx->M, gs = static_cast<long>(ctx->bz_nk_ibz) * no * no, sX = static_cast<long>(nmu) * no;
+  const long per = static_cast<long>(nk) * no * no;
+  const int P = choose_pair_chunk(ctx, 2);
+  c128* Xg = ctx->buf<c128>("bz_xgrid", static_cast<size_t>(nk) * sX);
+  c128* Gg = ctx->buf<c128>("bz_ggrid", static_cast<size_t>(P) * per);
+  c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(P) * nk * nmu * no);
+  c128* GP = ctx->buf<c128>("pi_gp", static_cast<size_t>(P) * nk * M);
+  c128* GN = ctx->buf<c128>("pi_gn", static_cast<size_t>(P) * nk * M);
   c128* PF = ctx->buf<c128>("bz_pf", static_cast<size_t>(nk) * M);
+  thcgw::gather_rows(s, X, sX, bz_tab(ctx, BZ_PG_X), bz_tab(ctx, BZ_PG_CONJ), nk, sX, Xg, sX, nullptr);
   const double scale = -2.0 / (double(nk) * nk);  // sp = -2 for ns = 1 (rpa_pi.icc:46)
-  for (int i = 0; i < nth; ++i) {
-    bz_project(ctx, G, X, i, B, s);
-    thcgw::gather_rows(s, B, M, bz_tab(ctx, BZ_P_SRC), bz_tab(ctx, BZ_P_CONJ), nk, M, GP, M, bz_tab(ctx, BZ_P_DST));
-    bz_project(ctx, G, X, nt - 1 - i, B, s);
-    thcgw::gather_rows(s, B, M, bz_tab(ctx, BZ_P_SRC), bz_tab(ctx, BZ_P_CONJ), nk, M, GN, M, bz_tab(ctx, BZ_P_DST));
-    thcgw::pi_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP, GN, PF, M, scale);
-    thcgw::gather_rows(s, PF, M, bz_tab(ctx, BZ_Q_GRID), nullptr, nqi, M, Pi + static_cast<long>(i) * nqi * M, M,
-                       nullptr);
+  auto gather = [&](int it, c128* dst) {
+    thcgw::gather_blocks(s, G + it * gs, bz_tab(ctx, BZ_PG_SRC), bz_tab(ctx, BZ_PG_TRANS), bz_tab(ctx, BZ_PG_CONJ),
+                         no, nk, dst);
+  };
+  for (int i0 = 0; i0 < nth; i0 += P) {
+    const int c = std::min(P, nth - i0);
+    for (int j = 0; j < c; ++j) gather(i0 + j, Gg + j * per);
+    project_p2a(ctx, s, Xg, Gg, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    for (int j = 0; j < c; ++j) gather(nt - 1 - (i0 + j), Gg + j * per);
+    project_p2a(ctx, s, Xg, Gg, GN, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    for (int j = 0; j < c; ++j) {
+      thcgw::pi_kfft(s, ctx->d.kgrid, ctx->force_fallback, GP + j * nk * M, GN + j * nk * M, PF, M, scale);
+      thcgw::gather_rows(s, PF, M, bz_tab(ctx, BZ_Q_GRID), nullptr, nqi, M, Pi + static_cast<long>(i0 + j) * nqi * M,
+                         M, nullptr);
+    }
   }
 }
 
@@ -1104,30 +1114,46 @@ const c128* bz_xd(thcgw_ctx* ctx, const c128* X, hipStream_t s) {
 // per tau of both particle-hole passes, G_PQ on the points without time reversal, the
 // Hadamard sums of every operation S over the q of Qs[S] at k S (time-reversal conjugates
 // in the term flags, thc_gw.icc:366-395), Sigma_ab(k S) = X_{kS}^H Sigma_PQ X_{kS} rotated
-// by D (one product with X_{kS} D), and the sum over S.
+// by D (one product with X_{kS} D), and the sum over S.  Projections and back-projections
+// are batched over tau.
 void run_sigma_ibz(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X, c128* Sig, bool herm,
                    hipStream_t s) {
+  check_herm_g(ctx, G, s, ctx->bz_nk_ibz);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu, nt = ctx->d.nt, nth = ctx->nt_half;
   const int nki = ctx->bz_nk_ibz, nqi = ctx->bz_nq_ibz, nkn = ctx->bz_nk_nt, nsym = ctx->bz_nsym;
   const int nout = nsym * nki;
   const long M = ctx->M, gs = static_cast<long>(nki) * no * no, sX = static_cast<long>(nmu) * no;
+  const long per = static_cast<long>(nkn) * no * no, so = static_cast<long>(no) * no;
+  const int P = choose_pair_chunk(ctx, std::max(1, (nkn + nout + nk - 1) / nk));
   const c128* XD = bz_xd(ctx, X, s);
-  c128* B = ctx->buf<c128>("bz_buf", static_cast<size_t>(nkn) * M);
-  c128* SP = ctx->buf<c128>("bz_sp", static_cast<size_t>(nout) * M);
-  c128* SA = ctx->buf<c128>("bz_sa", static_cast<size_t>(nout) * no * no);
-  c128* Ab = ctx->buf<c128>("bz_ab", static_cast<size_t>(std::max(nkn, nout)) * nmu * no);
+  c128* Gg = ctx->buf<c128>("bz_gg", static_cast<size_t>(P) * per);
+  c128* B = ctx->buf<c128>("bz_buf", static_cast<size_t>(P) * nkn * M);
+  c128* SP = ctx->buf<c128>("bz_sp", static_cast<size_t>(P) * nout * M);
+  c128* SA = ctx->buf<c128>("bz_sa", static_cast<size_t>(P) * nout * so);
+  c128* Ab = ctx->buf<c128>("proj_ab", static_cast<size_t>(P) * std::max(nkn, nout) * nmu * no);
   for (int pass = 0; pass < 2; ++pass) {
     const int ntp = pass == 0 ? nth : nt / 2;  // the middle tau of an odd grid is the first pass's
-    for (int it = 0; it < ntp; ++it) {
-      const int itg = pass == 0 ? it : nt - it - 1;
-      bz_project(ctx, G, X, itg, B, s);
-      thcgw::kconv_terms(s, B, nkn, Wqt + static_cast<long>(it) * M, static_cast<long>(nth) * M, nqi,
-                         bz_tab(ctx, BZ_T_OFF), bz_tab(ctx, BZ_T_TERMS), nout, SP, M, -1.0 / nk);
-      thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(XD, sX), no, thcgw::bp(SP, M), nmu,
-                   kZero, Ab, sX, nmu, nout);
-      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Ab, sX), nmu, thcgw::bp(XD, sX), no,
-                   kZero, SA, static_cast<long>(no) * no, no, nout);
-      thcgw::sum_groups(s, SA, nsym, gs, Sig + itg * gs);
+    for (int i0 = 0; i0 < ntp; i0 += P) {
+      const int c = std::min(P, ntp - i0);
+      auto itg = [&](int j) { return pass == 0 ? i0 + j : nt - 1 - (i0 + j); };
+      for (int j = 0; j < c; ++j)
+        thcgw::gather_blocks(s, G + itg(j) * gs, bz_tab(ctx, BZ_G_SRC), bz_tab(ctx, BZ_G_TRANS), nullptr, no, nkn,
+                             Gg + j * per);
+      project_p2a(ctx, s, X, Gg, B, static_cast<long>(c) * nkn, Ab, ctx->herm_g, nkn);
+      // with flag 10 every Sigma_PQ of an operation is Hermitian (Hadamard sums of Hermitian G
+      // and W): its lower half L goes back as X^H L X + (X^H L X)^H, a triangular first product
+      const int hn = ctx->herm_g ? nmu : 0;
+      for (int j = 0; j < c; ++j)
+        thcgw::kconv_terms(s, B + j * nkn * M, nkn, Wqt + static_cast<long>(i0 + j) * M, static_cast<long>(nth) * M,
+                           nqi, bz_tab(ctx, BZ_T_OFF), bz_tab(ctx, BZ_T_TERMS), nout, SP + j * nout * M, M, -1.0 / nk,
+                           hn);
+      const int nb = c * nout;
+      thcgw::zgemm(s, thcgw::OP_C, thcgw::OP_N, no, nmu, nmu, kOne, thcgw::bp(XD, sX, nout), no, thcgw::bp(SP, M), nmu,
+                   kZero, Ab, sX, nmu, nb, kZero, hn ? thcgw::GEMM_TRI_KFROM_COL : 0);
+      thcgw::zgemm(s, thcgw::OP_N, thcgw::OP_N, no, no, nmu, kOne, thcgw::bp(Ab, sX), nmu, thcgw::bp(XD, sX, nout), no,
+                   kZero, SA, so, no, nb);
+      if (hn) thcgw::hermitize(s, SA, no, nb, 2.0);
+      for (int j = 0; j < c; ++j) thcgw::sum_groups(s, SA + j * nout * so, nsym, gs, Sig + itg(j) * gs);
     }
   }
   if (herm) thcgw::hermitize(s, Sig, no, static_cast<long>(nt) * nki);
@@ -1574,17 +1600,24 @@ int thcgw_set_bz(thcgw_ctx* ctx, const thcgw_bz* bz) {
     // X_{kS} (nsym nki), Sigma term offsets (nsym nki + 1) and terms (nki nk)
     const int nout = nsym * nki;
     std::vector<int> h;
-    long o[9];
+    long o[10];
     o[BZ_G_SRC] = 0;
     for (int k = 0; k < nkn; ++k) h.push_back(bz->kp_to_ibz[k]);
     o[BZ_G_TRANS] = static_cast<long>(h.size());
     for (int k = 0; k < nkn; ++k) h.push_back(bz->kp_trev[k]);
-    o[BZ_P_SRC] = static_cast<long>(h.size());
-    for (int k = 0; k < nk; ++k) h.push_back(bz->kp_trev[k] ? bz->kp_trev_pair[k] : k);
-    o[BZ_P_CONJ] = static_cast<long>(h.size());
-    for (int k = 0; k < nk; ++k) h.push_back(bz->kp_trev[k]);
-    o[BZ_P_DST] = static_cast<long>(h.size());
-    for (int k = 0; k < nk; ++k) h.push_back(bz->k_grid_index[k]);
+    // chi0 in grid order: grid point g holds listed point k; a time-reversal point is
+    // formed from its partner p (rpa_pi.icc:106-113)
+    std::vector<int> listed(nk);
+    for (int k = 0; k < nk; ++k) listed[bz->k_grid_index[k]] = k;
+    auto src_pt = [&](int g) { const int k = listed[g]; return bz->kp_trev[k] ? bz->kp_trev_pair[k] : k; };
+    o[BZ_PG_SRC] = static_cast<long>(h.size());
+    for (int g = 0; g < nk; ++g) h.push_back(bz->kp_to_ibz[src_pt(g)]);
+    o[BZ_PG_TRANS] = static_cast<long>(h.size());
+    for (int g = 0; g < nk; ++g) h.push_back(bz->kp_trev[src_pt(g)]);
+    o[BZ_PG_CONJ] = static_cast<long>(h.size());
+    for (int g = 0; g < nk; ++g) h.push_back(bz->kp_trev[listed[g]]);
+    o[BZ_PG_X] = static_cast<long>(h.size());
+    for (int g = 0; g < nk; ++g) h.push_back(src_pt(g));
     o[BZ_Q_GRID] = static_cast<long>(h.size());
     for (int q = 0; q < nqi; ++q) h.push_back(bz->k_grid_index[q]);
     o[BZ_X_SRC] = static_cast<long>(h.size());
@@ -1620,7 +1653,7 @@ int thcgw_set_bz(thcgw_ctx* ctx, const thcgw_bz* bz) {
     const size_t nD = static_cast<size_t>(nout) * no * no;
     THCGW_HIP(hipMalloc(&ctx->d_bzD, sizeof(c128) * nD));
     THCGW_HIP(hipMemcpy(ctx->d_bzD, bz->D, sizeof(c128) * nD, hipMemcpyHostToDevice));
-    std::copy(o, o + 9, ctx->bz_o);
+    std::copy(o, o + 10, ctx->bz_o);
     ctx->bz_nk_ibz = nki;
     ctx->bz_nq_ibz = nqi;
     ctx->bz_nk_nt = nkn;

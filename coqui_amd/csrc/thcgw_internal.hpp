@@ -137,11 +137,12 @@ void kconv_sigma(hipStream_t s, const c128* G, const c128* W, long w_kstride, co
                  long M, double scale);
 
 // symk.hip: Brillouin-zone symmetry (chi0 and Sigma on the irreducible wedge)
-void gather_blocks(hipStream_t s, const c128* src, const int* sidx, const int* trans, int n, int count, c128* dst);
+void gather_blocks(hipStream_t s, const c128* src, const int* sidx, const int* trans, const int* cj, int n, int count,
+                   c128* dst);
 void gather_rows(hipStream_t s, const c128* src, long sstride, const int* sidx, const int* cj, int count, long len,
                  c128* dst, long dstride, const int* dmap);
 void kconv_terms(hipStream_t s, const c128* A, int nA, const c128* B, long b_stride, int nB, const int* off,
-                 const int* terms, int nout, c128* out, long M, double scale);
+                 const int* terms, int nout, c128* out, long M, double scale, int herm_n = 0);
 void sum_groups(hipStream_t s, const c128* src, int ng, long per, c128* dst);
 
 // lu.hip

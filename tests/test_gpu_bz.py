@@ -55,27 +55,34 @@ def test_pi_sigma_ibz_vs_oracle(torch, kind, kgrid, no, nmu):
     s, bz, ia, ctx, G, Gi, Xl = setup(torch, kind, kgrid, no, nmu)
     try:
         Gi_h, Xl_h = Gi.cpu().numpy(), Xl.cpu().numpy()
-        P = ctx.pi_ibz(Gi, Xl)
-        assert relmax(P.cpu().numpy(), bz_oracle.pi_ibz(Gi_h, Xl_h, bz, kgrid)) < 1e-10
-        # against the full-grid GPU path at the irreducible q
-        Pf = ctx.pi(G, torch.from_numpy(s.X).cuda())
-        ref = Pf[:, torch.from_numpy(bz["order"][:bz["nq_ibz"]]).cuda()]
-        assert relmax(P.cpu().numpy(), ref.cpu().numpy()) < 1e-10
+        Pi_ref = bz_oracle.pi_ibz(Gi_h, Xl_h, bz, kgrid)
         rng = np.random.default_rng(5)
         ct = rng.uniform(0.5, 1.5, ia.nt_half)
         Wi = np.ascontiguousarray(s.V[bz["order"][:bz["nq_ibz"]]][:, None] * ct[None, :, None, None])
-        S = ctx.sigma_ibz(Gi, torch.from_numpy(Wi).cuda(), Xl, hermitize=False)
-        assert relmax(S.cpu().numpy(), bz_oracle.sigma_ibz(Gi_h, Xl_h, Wi, bz)) < 1e-10
+        S_ref = bz_oracle.sigma_ibz(Gi_h, Xl_h, Wi, bz)
+        for herm in (0, 1):  # flag 10: Hermitian G projections and Hermitian Sigma_PQ halves
+            ctx.set_flag(10, herm)
+            P = ctx.pi_ibz(Gi, Xl)
+            assert relmax(P.cpu().numpy(), Pi_ref) < 1e-10
+            S = ctx.sigma_ibz(Gi, torch.from_numpy(Wi).cuda(), Xl, hermitize=False)
+            assert relmax(S.cpu().numpy(), S_ref) < 1e-10
+        # against the full-grid GPU path at the irreducible q
+        ctx.set_flag(10, 0)
+        Pf = ctx.pi(G, torch.from_numpy(s.X).cuda())
+        ref = Pf[:, torch.from_numpy(bz["order"][:bz["nq_ibz"]]).cuda()]
+        assert relmax(P.cpu().numpy(), ref.cpu().numpy()) < 1e-10
     finally:
         ctx.close()
 
 
 @pytest.mark.parametrize("kind,kgrid,no,nmu,dyson", [c + (0,) for c in CASES] + [CASES[0] + (2,), CASES[3] + (2,)])
 def test_iteration_ibz_vs_full_grid(torch, kind, kgrid, no, nmu, dyson):
+    """dyson = 2 runs the resolvent form with flag 10 (Hermitian G), as the benchmark does."""
     print("case: bz iteration %s kgrid=%s no=%d nmu=%d dyson=%d" % (kind, kgrid, no, nmu, dyson), flush=True)
     s, bz, ia, ctx, G, Gi, Xl = setup(torch, kind, kgrid, no, nmu)
     try:
         ctx.set_flag(6, dyson)
+        ctx.set_flag(10, int(dyson == 2))
         V = torch.from_numpy(s.V).cuda()
         Sf, Wf, ecf = ctx.gw_iteration(G, torch.from_numpy(s.X).cuda(), V)
         order = torch.from_numpy(bz["order"]).cuda()
