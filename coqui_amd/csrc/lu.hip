@@ -47,10 +47,12 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(c128* A, long sA, int l
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
 
-  __shared__ double red_val[PT / 64];
-  __shared__ int red_idx[PT / 64];
+  // Per-wave pivot candidates, double buffered by column parity.  A column's candidates are
+  // written before its first barrier and read between its two barriers; the next write to
+  // the same buffer comes two columns later, after a barrier that every reader has passed.
+  __shared__ double red_val[2][PT / 64];
+  __shared__ int red_idx[2][PT / 64];
   __shared__ c128 pcol[W], jcol[W];
-  __shared__ int s_p;
 
   c128 v[R][W];
 #pragma unroll
@@ -93,24 +95,25 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(c128* A, long sA, int l
           bidx = oi;
         }
       }
+      const int buf = c & 1;
       if (lane == 0) {
-        red_val[wave] = best;
-        red_idx[wave] = bidx;
+        red_val[buf][wave] = best;
+        red_idx[buf][wave] = bidx;
       }
       __syncthreads();
-      if (t == 0) {
-        double bv = red_val[0];
-        int bi = red_idx[0];
-        for (int q = 1; q < PT / 64; ++q)
-          if (red_val[q] > bv || (red_val[q] == bv && red_idx[q] < bi)) {
-            bv = red_val[q];
-            bi = red_idx[q];
-          }
-        s_p = bi;
-        piv[jc] = bi;
+      // every thread reduces the per-wave candidates itself (broadcast LDS reads, the same
+      // order and tie rule everywhere), so no second barrier hands the pivot around
+      double bv = red_val[buf][0];
+      int p = red_idx[buf][0];
+      for (int q = 1; q < PT / 64; ++q) {
+        const double qv = red_val[buf][q];
+        const int qi = red_idx[buf][q];
+        if (qv > bv || (qv == bv && qi < p)) {
+          bv = qv;
+          p = qi;
+        }
       }
-      __syncthreads();
-      const int p = s_p;
+      if (t == 0) piv[jc] = p;
       // 2. exchange the panel entries of columns p and jc
       const int op = (p - j0) % PT, sp_ = (p - j0) / PT;
       const int oj = (jc - j0) % PT, sj = (jc - j0) / PT;
@@ -155,7 +158,8 @@ __global__ __launch_bounds__(PT) void getf2_panel_kernel(c128* A, long sA, int l
       } else if (first_zero == 0) {
         first_zero = jc + 1;
       }
-      __syncthreads();
+      // No barrier closes the column: the next one writes pcol/jcol only after its first
+      // barrier, which every thread reaches after finishing this column's reads.
     }
   }
 
