@@ -20,6 +20,7 @@
 #include "lri.hpp"
 #include "thcgw_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace thcgw {
@@ -395,7 +396,9 @@ void potrf_rec(hipStream_t s, c128* A, long sA, int lda, int j0, int n, int* inf
 // for a factorisation plus a triangular inverse, without the triangular solves' narrow
 // updates; the four products are N,N / N,C / C,N / N,N with triangular-K flags.  On exit A
 // holds L on and below the diagonal except L21 (the diagonal blocks are exact; only their
-// diagonal is read afterwards) and Z holds L^-H with zeros below the diagonal.
+// diagonal is read afterwards) and Z holds L^-H with zeros in the band of 128 below the
+// diagonal (the only part of its lower triangle a K-bounded product with tiles up to 128
+// wide reads, whatever the tiles' origin); further below it holds scratch.
 void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int ldz, int j0, int n, int* info,
                  int batch) {
   if (n <= CB) {
@@ -423,17 +426,24 @@ void cholinv_rec(hipStream_t s, c128* A, long sA, int lda, c128* Z, long sZ, int
         GEMM_TRI_KTO_COL);
   zgemm(s, OP_N, OP_N, n1, n2, n1, kMinusOne, bp(Z11, sZ), ldz, bp(A12, sA), lda, zero, Z12, sZ, ldz, batch, zero,
         GEMM_TRI_KFROM_ROW);
-  const long nb = static_cast<long>(n2) * n1;
-  hipLaunchKernelGGL(zero_block_kernel, dim3(static_cast<unsigned>((nb + 255) / 256), batch), dim3(256), 0, s, Z21, sZ,
-                     ldz, n2, n1);
+  // Z21 held a temporary and lies below the diagonal of Z.  The products that read Z later
+  // (the K-bounded ones of this recursion's ancestors, each tiled from its own sub-block
+  // origin, and Z Z^H with K from the row) read its lower part only inside a diagonal tile
+  // of theirs, at most 128 wide: element (r, c) is read only if r - c < 128.  So the zeros
+  // they need are the corner of Z21 within 128 rows and columns of the split point.
+  const int rows = std::min(128, n2), cols = std::min(128, n1), c0 = n1 - cols;
+  const long nb = static_cast<long>(rows) * cols;
+  hipLaunchKernelGGL(zero_block_kernel, dim3(static_cast<unsigned>((nb + 255) / 256), batch), dim3(256), 0, s, Z21 + c0,
+                     sZ, ldz, rows, cols);
   THCGW_HIP(hipGetLastError());
 }
 
 }  // namespace
 
 // A = L L^H in place (lower; the strict upper triangle is overwritten with scratch) and
-// Z = L^-H (upper, zeros below the diagonal) for `batch` Hermitian n x n matrices, both
-// packed (leading dimension n).  info as for potrf_batched.
+// Z = L^-H (upper; zeros in the band of 128 below the diagonal, scratch further below,
+// which no K-bounded product reads) for `batch` Hermitian n x n matrices,
+// both packed (leading dimension n).  info as for potrf_batched.
 void cholinv_batched(hipStream_t s, c128* A, long sA, c128* Z, long sZ, int n, int* info, int batch) {
   THCGW_REQUIRE(n >= 1, "cholinv: order must be positive");
   if (batch <= 0) return;

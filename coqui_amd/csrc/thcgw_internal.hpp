@@ -165,7 +165,8 @@ void trsm_right_lower_conj_f32(hipStream_t s, const c64* L, long sL, int ldl, c6
 void zero_upper(hipStream_t s, c128* A, long sA, int n, int batch);
 void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int batch, const double* add = nullptr,
                  int amod = 1, int aoff = 0);
-// A = L L^H (lower) and Z = L^-H (upper) by GEMM-only recursion (A and Z packed)
+// A = L L^H (lower) and Z = L^-H (upper, valid for products with triangular-K flags: zeros
+// only in the band of 128 below the diagonal) by GEMM-only recursion
 void cholinv_batched(hipStream_t s, c128* A, long sA, c128* Z, long sZ, int n, int* info, int batch);
 // Z = L^-H for lower non-unit L (Z packed, ldz = n)
 void tri_inverse_conj(hipStream_t s, const c128* L, long sL, int ldl, c128* Z, long sZ, int ldz, int n, int batch);
