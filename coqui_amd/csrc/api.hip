@@ -96,6 +96,9 @@ struct thcgw_ctx {
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
   int mem_share = 1;    // scratch pools side by side (batch lanes): automatic chunks take 1/mem_share
+  int dyson_lanes = 2;  // flag 11: resolvent Dyson chunks pipelined over two streams (2, default) or not (1)
+  hipStream_t dy_s[2] = {nullptr, nullptr};
+  hipEvent_t dy_ev[3] = {nullptr, nullptr, nullptr};
   unsigned long* scratch_gen = nullptr;  // bumped on every scratch (re)allocation (a batch's graph)
   int herm_g = 0;       // flag 10: G(tau, k) Hermitian, its projections formed on the lower triangle
   int dyson_algo = 0;   // 0 = LU with partial pivoting (reference algorithm), 1 = Hermitian (Cholesky),
@@ -161,6 +164,17 @@ struct thcgw_ctx {
     d_symph = nullptr;
     if (d_bz) (void)hipFree(d_bz);
     d_bz = nullptr;
+    for (hipStream_t& x : dy_s) {
+      if (x) {
+        (void)hipStreamSynchronize(x);
+        (void)hipStreamDestroy(x);
+      }
+      x = nullptr;
+    }
+    for (hipEvent_t& e : dy_ev) {
+      if (e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
     if (d_bzD) (void)hipFree(d_bzD);
     d_bzD = nullptr;
     d_tau = nullptr;
@@ -444,7 +458,8 @@ int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
   // matrices per chunk and 1.259 s with 400; at N_mu = 800 the plain iteration takes 0.645 s
   // in chunks of 1677 + 691 matrices and 0.635 s in one chunk of 2368.
   double held = 0.0;
-  for (const char* name : {"dy_lu", "dy_z", "dy_z64", "dy_m64"}) held += double(ctx->held_bytes(name));
+  for (const char* name : {"dy_lu", "dy_z", "dy_z64", "dy_m64", "dy_lu1", "dy_z1"})
+    held += double(ctx->held_bytes(name));
   const double budget =
       std::min(0.45 * (double(free_device_bytes()) + held), 64.0 * double(1L << 30)) / ctx->mem_share;
   const int c = std::max(1, std::min(static_cast<int>(budget / per), total));
@@ -570,15 +585,25 @@ bool dyson_chunk_hpd_f32(thcgw_ctx* ctx, c128* PiW, const c128* V, const c64* Lv
 // factorisation together with the inverse of its factor (cholinv_batched) and n^3/6 for
 // the triangular product, all in FP64.  Returns false, leaving the chunk's Pi slots
 // untouched, when some A is not positive definite.
-bool dyson_chunk_inv(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Vi, const double* ldv, int b0, int cb,
-                     c128* d_trace, double* d_ldet, bool solve, c128* Ab, c128* Z, int* info, hipStream_t s) {
+// The chunk's front half: A and its Cholesky factor with Z (reads the chunk's Pi slots,
+// writes only Ab, Z, the trace terms and the flags); `trp` names the trace-parts buffer.
+void resolvent_front(thcgw_ctx* ctx, const c128* PiW, const c128* V, const c128* Vi, int b0, int cb, c128* d_trace,
+                     c128* Ab, c128* Z, int* info, hipStream_t s, const char* trp) {
+  const int n = ctx->d.nmu, nq = ctx->vq();
+  const long M = ctx->M;
+  const c128* P = PiW + static_cast<long>(b0) * M;
+  c128* part = d_trace ? ctx->buf<c128>(trp, static_cast<size_t>(cb) * thcgw::resolvent_prep_parts(n)) : nullptr;
+  thcgw::resolvent_prep(s, V, Vi, M, nq, b0, P, Ab, M, n, d_trace ? d_trace + b0 : nullptr, part, cb);
+  thcgw::cholinv_batched(s, Ab, M, Z, M, n, info, cb);
+}
+
+// The back half, once the factorisations are known to have succeeded: ln det and
+// W = Z Z^H - V into the chunk's Pi slots.
+void resolvent_back(thcgw_ctx* ctx, c128* PiW, const c128* V, const double* ldv, int b0, int cb, double* d_ldet,
+                    bool solve, const c128* Ab, const c128* Z, hipStream_t s) {
   const int n = ctx->d.nmu, nq = ctx->vq();
   const long M = ctx->M;
   c128* P = PiW + static_cast<long>(b0) * M;
-  c128* part = d_trace ? ctx->buf<c128>("dy_trp", static_cast<size_t>(cb) * thcgw::resolvent_prep_parts(n)) : nullptr;
-  thcgw::resolvent_prep(s, V, Vi, M, nq, b0, P, Ab, M, n, d_trace ? d_trace + b0 : nullptr, part, cb);
-  thcgw::cholinv_batched(s, Ab, M, Z, M, n, info, cb);
-  if (!ctx->deferred && any_nonzero(info, cb, s)) return false;
   if (d_ldet) thcgw::chol_logdet(s, Ab, M, n, d_ldet + b0, cb, ldv, nq, b0);
   if (solve) {
     // W = Z Z^H - V from the lower tiles; Z is upper triangular, so the sum starts at the row
@@ -586,6 +611,13 @@ bool dyson_chunk_inv(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Vi, c
                  cb, kZero, thcgw::GEMM_TRI_LOWER | thcgw::GEMM_TRI_KFROM_ROW | thcgw::GEMM_HERM_SUB,
                  thcgw::bp(V, M, nq, 1, b0));
   }
+}
+
+bool dyson_chunk_inv(thcgw_ctx* ctx, c128* PiW, const c128* V, const c128* Vi, const double* ldv, int b0, int cb,
+                     c128* d_trace, double* d_ldet, bool solve, c128* Ab, c128* Z, int* info, hipStream_t s) {
+  resolvent_front(ctx, PiW, V, Vi, b0, cb, d_trace, Ab, Z, info, s, "dy_trp");
+  if (!ctx->deferred && any_nonzero(info, cb, s)) return false;
+  resolvent_back(ctx, PiW, V, ldv, b0, cb, d_ldet, solve, Ab, Z, s);
   return true;
 }
 
@@ -627,6 +659,52 @@ VFactors factor_dyson_v(thcgw_ctx* ctx, const c128* V, hipStream_t s) {
   return f;
 }
 
+// Flag 11: the resolvent chunks pipelined over two streams, each with its own buffers (the
+// chunk size halved, so that both hold the memory of one).  Chunk c's front half runs on
+// lane c % 2; while the host reads its factorisation flags, chunk c + 1's front half runs on
+// the other lane; then chunk c's back half (or, if a factorisation failed, the LU form on
+// its untouched Pi slots) follows on its own lane.  The small products of one chunk's
+// recursion so overlap the other chunk's large ones.
+void dyson_resolvent_lanes(thcgw_ctx* ctx, c128* PiW, const c128* V, const VFactors& f, int total, int CH,
+                           c128* d_trace, double* d_ldet, bool solve, bool pi_first, int* info_all, hipStream_t s) {
+  const int n = ctx->d.nmu;
+  const long M = ctx->M;
+  for (int l = 0; l < 2; ++l)
+    if (!ctx->dy_s[l]) THCGW_HIP(hipStreamCreateWithFlags(&ctx->dy_s[l], hipStreamNonBlocking));
+  for (hipEvent_t& e : ctx->dy_ev)
+    if (!e) THCGW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  c128* LU[2] = {ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M), ctx->buf<c128>("dy_lu1", static_cast<size_t>(CH) * M)};
+  c128* Z[2] = {ctx->buf<c128>("dy_z", static_cast<size_t>(CH) * M), ctx->buf<c128>("dy_z1", static_cast<size_t>(CH) * M)};
+  // the lanes start after the caller's earlier work on s (Pi, the factors of V)
+  THCGW_HIP(hipEventRecord(ctx->dy_ev[2], s));
+  for (hipStream_t x : ctx->dy_s) THCGW_HIP(hipStreamWaitEvent(x, ctx->dy_ev[2], 0));
+  const int nch = (total + CH - 1) / CH;
+  auto b0 = [&](int c) { return c * CH; };
+  auto cbn = [&](int c) { return std::min(CH, total - c * CH); };
+  auto front = [&](int c) {
+    const int l = c & 1;
+    resolvent_front(ctx, PiW, V, f.Vi, b0(c), cbn(c), d_trace, LU[l], Z[l], info_all + b0(c), ctx->dy_s[l],
+                    l ? "dy_trp1" : "dy_trp");
+  };
+  front(0);
+  for (int c = 0; c < nch; ++c) {
+    if (c + 1 < nch) front(c + 1);
+    const int l = c & 1;
+    hipStream_t ls = ctx->dy_s[l];
+    if (!any_nonzero(info_all + b0(c), cbn(c), ls)) {
+      resolvent_back(ctx, PiW, V, f.ldv, b0(c), cbn(c), d_ldet, solve, LU[l], Z[l], ls);
+    } else {
+      int* ipiv = ctx->buf<int>(l ? "dy_ipiv1" : "dy_ipiv", static_cast<size_t>(CH) * n);
+      dyson_chunk_lu(ctx, PiW, V, b0(c), cbn(c), d_trace, d_ldet, solve, pi_first, LU[l], Z[l], ipiv, info_all + b0(c),
+                     ls);
+    }
+  }
+  for (int l = 0; l < 2; ++l) {
+    THCGW_HIP(hipEventRecord(ctx->dy_ev[l], ctx->dy_s[l]));
+    THCGW_HIP(hipStreamWaitEvent(s, ctx->dy_ev[l], 0));
+  }
+}
+
 // In place Pi -> W - V for `total` matrices at PiW (matrix b uses V member b % vq()), in
 // chunks; Tr, ln det and the factorisation flags at d_trace + b, d_ldet + b, info + b.
 void dyson_matrices(thcgw_ctx* ctx, c128* PiW, const c128* V, const VFactors& f, int total, c128* d_trace,
@@ -635,6 +713,9 @@ void dyson_matrices(thcgw_ctx* ctx, c128* PiW, const c128* V, const VFactors& f,
   const long M = ctx->M;
   const int CH = choose_dyson_chunk(ctx, total);
   const bool mixed = ctx->precision != 0;
+  if (f.inv && ctx->dyson_lanes > 1 && !ctx->deferred && total > 1)
+    return dyson_resolvent_lanes(ctx, PiW, V, f, total, std::max(1, (CH + 1) / 2), d_trace, d_ldet, solve, pi_first,
+                                 info_all, s);
   c128* LU = ctx->buf<c128>("dy_lu", static_cast<size_t>(CH) * M);
   c128* Z = nullptr;
   c64* Z64 = nullptr;
@@ -1469,6 +1550,9 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     } else if (flag == 7) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 7 (precision) takes 0 (FP64), 1 (mixed) or 2 (mixed, split BF16)");
       ctx->precision = static_cast<int>(value);
+    } else if (flag == 11) {
+      THCGW_REQUIRE(value >= 0 && value <= 2, "flag 11 (resolvent Dyson lanes) takes 0, 1 or 2");
+      ctx->dyson_lanes = value == 2 ? 2 : 1;
     } else if (flag == 10) {
       THCGW_REQUIRE(value == 0 || value == 1, "flag 10 (Hermitian G) takes 0 or 1");
       ctx->herm_g = static_cast<int>(value);

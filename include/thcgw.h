@@ -20,9 +20,12 @@
  *     thcgw_batch_gw_iteration, thcgw_dyson_g, thcgw_nelec, thcgw_update_mu,
  *     thcgw_lri_dyson) first check their input arrays and return THCGW_ERR_NONFINITE,
  *     before any factorisation, when one holds a NaN or an infinity (one read pass);
- *   - ns = 1 (no spin polarisation), npol = 1, no k-point symmetry: k-points form
- *     a Gamma-centred kgrid[0] x kgrid[1] x kgrid[2] mesh in C order with
- *     lattice vectors = identity, and q-points = k-points.
+ *   - ns = 1 (no spin polarisation), npol = 1: k-points form a Gamma-centred
+ *     kgrid[0] x kgrid[1] x kgrid[2] mesh in C order with lattice vectors = identity,
+ *     and q-points = k-points; symmetry only where flag 9, thcgw_set_qsym or
+ *     thcgw_set_bz says so;
+ *   - work enqueued on `stream` may fork to the context's own streams and joins back to
+ *     `stream` before the call returns or its later work runs (flag 11).
  *
  * Layouts (names as in the reference):
  *   G_tskij     (nt, 1, nk, no, no)      Green's function on the fermionic IR tau mesh
@@ -423,6 +426,11 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           of V (W to ~1e-9 relative for cond(V) ~ 1e7).  With flag 7 = 1 or 2 the Dyson
  *           step stays in this FP64 form and only the projections and k-grid transforms run
  *           in FP32;
+ *   flag 11: 2 (default) = the resolvent form's (flag 6 = 2) matrix chunks pipelined over two
+ *           streams of the context, each with half the automatic chunk size: one chunk's
+ *           factorisation runs while the host reads the previous chunk's flags and that
+ *           chunk's Z Z^H runs; 1 = one stream.  Results are bit-identical.  A batch's captured
+ *           graph keeps one stream per system;
  *   flag 10: G(tau, k) is Hermitian (as for any G built from a Hermitian H(k) and Fermi
  *           occupations): the projections X G X^H for chi0 and Sigma form their lower tiles
  *           and mirror them, half the work of that product, and Sigma's back-projection

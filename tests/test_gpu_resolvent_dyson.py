@@ -68,16 +68,19 @@ def test_resolvent_chunking_bit_exact(nk8_nmu200, torch):
     """Each matrix's arithmetic does not depend on how the matrices are chunked: the
     automatic chunk (one chunk here), ragged chunks of 7 (fewer than the 8 q, so each V_q
     serves one matrix per chunk) and chunks of 20 (several matrices per V_q) give
-    bit-identical W, Tr(V Pi) and ln det."""
+    bit-identical W, Tr(V Pi) and ln det, also with the chunks pipelined over two streams
+    (flag 11)."""
     c = nk8_nmu200
     outs = []
-    for chunk in (0, 7, 20):
+    for lanes, chunk in ((1, 0), (1, 7), (1, 20), (2, 0), (2, 7), (2, 20)):
         c["ctx"].set_flag(3, chunk)
+        c["ctx"].set_flag(11, lanes)
         try:
             PiW = torch.from_numpy(c["ref"]["Pi_w"].copy()).cuda()
             tr, ld = c["ctx"].dyson_w(PiW, c["s"].V, with_terms=True)
         finally:
             c["ctx"].set_flag(3, 0)
+            c["ctx"].set_flag(11, 1)
         outs.append((to_np(PiW), np.asarray(tr), np.asarray(ld)))
     for W, tr, ld in outs[1:]:
         assert np.array_equal(W, outs[0][0])
@@ -179,14 +182,17 @@ def test_resolvent_large_order(torch):
     assert np.max(np.abs(W[2:])) < bound * np.max(np.abs(V))
 
 
-def test_resolvent_falls_back_to_lu(torch):
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_resolvent_falls_back_to_lu(torch, lanes):
     """Pi = 2 I with V = I makes A = V^-1 - Pi = -I indefinite: those chunks run LU
-    (W = (I - 2I)^-1 - I = -2I); the chunks with Pi = 0 stay in the resolvent form (W = 0)."""
+    (W = (I - 2I)^-1 - I = -2I); the chunks with Pi = 0 stay in the resolvent form (W = 0).
+    With two lanes (flag 11) the failing chunks fall on both streams."""
     ia = IAFT.from_table(1000.0, "1e2.1e-15")
     nmu = 40
     ctx = ThcGW((1, 1, 1), 2, nmu, ia)
     ctx.set_flag(6, 2)
     ctx.set_flag(3, 1)
+    ctx.set_flag(11, lanes)
     V = np.eye(nmu, dtype=complex)[None]
     Pi = np.zeros((ia.nw_half, 1, nmu, nmu), complex)
     Pi[0] = 2.0 * np.eye(nmu)
