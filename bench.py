@@ -192,7 +192,7 @@ def main():
     ap.add_argument("--gemm-tile32", type=int, default=0, help="FP32 GEMM tile shape: 0 64x64, 1 128x64, 2 128x128")
     ap.add_argument("--dyson-chunk", type=int, default=0, help="matrices per Dyson batch (0 = sized from free memory)")
     ap.add_argument("--pair-chunk", type=int, default=0, help="tau pairs per projection batch (0 = sized from free memory)")
-    ap.add_argument("--dyson-lanes", type=int, default=2, choices=(1, 2),
+    ap.add_argument("--dyson-lanes", type=int, default=1, choices=(1, 2),
                     help="resolvent Dyson chunks pipelined over this many streams (flag 11)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (HBM counter calibration)")

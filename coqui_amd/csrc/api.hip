@@ -96,7 +96,7 @@ struct thcgw_ctx {
   int pair_chunk = 0;   // tau pairs per projection batch (0 = automatic)
   int dyson_chunk = 0;  // matrices per Dyson batch (0 = automatic)
   int mem_share = 1;    // scratch pools side by side (batch lanes): automatic chunks take 1/mem_share
-  int dyson_lanes = 2;  // flag 11: resolvent Dyson chunks pipelined over two streams (2, default) or not (1)
+  int dyson_lanes = 1;  // flag 11: resolvent Dyson chunks pipelined over two streams (2) or not (1, default)
   hipStream_t dy_s[2] = {nullptr, nullptr};
   hipEvent_t dy_ev[3] = {nullptr, nullptr, nullptr};
   unsigned long* scratch_gen = nullptr;  // bumped on every scratch (re)allocation (a batch's graph)

@@ -426,11 +426,11 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           of V (W to ~1e-9 relative for cond(V) ~ 1e7).  With flag 7 = 1 or 2 the Dyson
  *           step stays in this FP64 form and only the projections and k-grid transforms run
  *           in FP32;
- *   flag 11: 2 (default) = the resolvent form's (flag 6 = 2) matrix chunks pipelined over two
- *           streams of the context, each with half the automatic chunk size: one chunk's
+ *   flag 11: 2 = the resolvent form's (flag 6 = 2) matrix chunks pipelined over two streams
+ *           of the context, each with half the automatic chunk size: one chunk's
  *           factorisation runs while the host reads the previous chunk's flags and that
- *           chunk's Z Z^H runs; 1 = one stream.  Results are bit-identical.  A batch's captured
- *           graph keeps one stream per system;
+ *           chunk's Z Z^H runs; 1 (default) = one stream.  Results are bit-identical.  A
+ *           batch's captured graph keeps one stream per system;
  *   flag 10: G(tau, k) is Hermitian (as for any G built from a Hermitian H(k) and Fermi
  *           occupations): the projections X G X^H for chi0 and Sigma form their lower tiles
  *           and mirror them, half the work of that product, and Sigma's back-projection
