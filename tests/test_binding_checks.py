@@ -107,3 +107,31 @@ def test_mlp_checks():
         m.set_params(np.zeros(9, np.float32))
     with pytest.raises(ValueError):
         m.train_mse(torch.zeros((5, 4), dtype=torch.float64), torch.zeros((5, 3), dtype=torch.float32), 1e-3)
+
+
+def test_bz_checks():
+    """The Brillouin-zone symmetry binding: mis-shaped tables are refused before
+    thcgw_set_bz, the wedge entry points need a symmetry set, and their arrays have the
+    wedge's shapes (irreducible k for G and Sigma, irreducible q for Pi, W and V)."""
+    from coqui_amd.system import make_system_bz
+
+    _, bz = make_system_bz((3, 2, 2), 2, 4, system_id=1, kind="trev")
+    c = fake_ctx(kgrid=(3, 2, 2), no=2, nmu=4)
+    with pytest.raises(ValueError, match="no Brillouin-zone symmetry"):
+        c.pi_ibz(cplx(c.nt, 1, bz["nk_ibz"], c.no, c.no), cplx(1, c.nk, c.nmu, c.no))
+    with pytest.raises(ValueError, match="qk_to_k2"):
+        c.set_bz(dict(bz, qk_to_k2=bz["qk_to_k2"][:-1]))
+    with pytest.raises(ValueError, match=r"bz\[D\]"):
+        c.set_bz(dict(bz, D=bz["D"][:, :-1]))
+    c._bz = (bz["nk_ibz"], bz["nq_ibz"])  # as after a successful thcgw_set_bz
+    nki, nqi = c._bz
+    full = cplx(c.nt, 1, c.nk, c.no, c.no)  # G on every k, not on the irreducible k
+    with pytest.raises(ValueError, match="G: expected shape"):
+        c.pi_ibz(full, cplx(1, c.nk, c.nmu, c.no))
+    with pytest.raises(ValueError, match="G: expected shape"):
+        c.sigma_ibz(full, cplx(nqi, c.nt_half, c.nmu, c.nmu), cplx(1, c.nk, c.nmu, c.no))
+    with pytest.raises(ValueError, match="G: expected shape"):
+        c.gw_iteration_ibz(full, cplx(1, c.nk, c.nmu, c.no), cplx(nqi, c.nmu, c.nmu))
+    # the wedge shapes get past the shape checks (then stop at the host-resident test)
+    with pytest.raises(ValueError, match="GPU"):
+        c.pi_ibz(cplx(c.nt, 1, nki, c.no, c.no), cplx(1, c.nk, c.nmu, c.no))
