@@ -8,7 +8,10 @@ resident in HBM.
 
 Forms (--forms; --headline names the one reported as `value`; errors are taken against
 the plain form, which runs first):
-  plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm);
+  plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm),
+             complex products by four real MFMA products;
+  plain_g3   the plain form with complex-FP64 products by three real MFMA products (flag 12,
+             Gauss's form), as every form below uses;
   hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
              transforms, Cholesky factorisations and ln det;
@@ -71,25 +74,27 @@ def batch_systems(n=24):
 
 
 # form: (flag 6 Dyson algorithm, flag 7 precision, symmetry: 0 none, 1 time reversal (flag 9),
-# 2 the C4 q-point map (thcgw_set_qsym), 3 the C4 irreducible wedge (thcgw_set_bz))
-FORMS = {"plain": (0, 0, 0), "hermitian": (1, 0, 0), "mixed": (1, 1, 0), "mixed_bf16": (1, 2, 0),
-         "resolvent": (2, 0, 0), "mixed_resolvent": (2, 1, 0),
-         "plain_trev": (0, 0, 1), "hermitian_trev": (1, 0, 1), "mixed_trev": (1, 1, 1),
-         "resolvent_trev": (2, 0, 1), "mixed_resolvent_trev": (2, 1, 1),
-         "plain_c4": (0, 0, 2), "hermitian_c4": (1, 0, 2), "mixed_c4": (1, 1, 2),
-         "resolvent_c4": (2, 0, 2), "mixed_resolvent_c4": (2, 1, 2),
-         "plain_ibz": (0, 0, 3), "resolvent_ibz": (2, 0, 3)}
+# 2 the C4 q-point map (thcgw_set_qsym), 3 the C4 irreducible wedge (thcgw_set_bz),
+# flag 12: complex-FP64 products by 3 real MFMA products instead of 4)
+FORMS = {"plain": (0, 0, 0, 0), "plain_g3": (0, 0, 0, 1), "hermitian": (1, 0, 0, 1), "mixed": (1, 1, 0, 1),
+         "mixed_bf16": (1, 2, 0, 1), "resolvent": (2, 0, 0, 1), "mixed_resolvent": (2, 1, 0, 1),
+         "plain_trev": (0, 0, 1, 0), "hermitian_trev": (1, 0, 1, 1), "mixed_trev": (1, 1, 1, 1),
+         "resolvent_trev": (2, 0, 1, 1), "mixed_resolvent_trev": (2, 1, 1, 1),
+         "plain_c4": (0, 0, 2, 0), "hermitian_c4": (1, 0, 2, 1), "mixed_c4": (1, 1, 2, 1),
+         "resolvent_c4": (2, 0, 2, 1), "mixed_resolvent_c4": (2, 1, 2, 1),
+         "plain_ibz": (0, 0, 3, 0), "resolvent_ibz": (2, 0, 3, 1)}
 HA_PER_MEV = 1.0 / 27211.386245988
 # GEMM profile class -> kernel template as rocprofv3 names it (64x64 tiles)
 OPS = {"N": 0, "T": 1, "C": 2}
 
 
-def gemm_kernel_name(cls):
+def gemm_kernel_name(cls, g3=False):
     if cls.endswith("32"):
         return "gemm_kernel<float, c64, c64, c64, %d, %d, 64, 64, false>" % (OPS[cls[0]], OPS[cls[1]])
     if cls.endswith("bf"):
         return "gemm_bf16x6_kernel<c64, c64, c64, %d, %d, false>" % (OPS[cls[0]], OPS[cls[1]])
-    return "gemm_kernel<double, c128, c128, c128, %d, %d, 64, 64, false>" % (OPS[cls[0]], OPS[cls[1]])
+    return "gemm_kernel<double, c128, c128, c128, %d, %d, 64, 64, false%s>" % (OPS[cls[0]], OPS[cls[1]],
+                                                                             ", true" if g3 else "")
 
 
 def log(*a):
@@ -206,8 +211,10 @@ def main():
                     help="batch24: streams the systems are spread over (parallel branches of the graph)")
     ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
                     help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
-    ap.add_argument("--forms", default="plain,mixed_resolvent",
-                    help="comma-separated forms to time (plain, hermitian, mixed, mixed_bf16, resolvent, "
+    ap.add_argument("--batch-g3", type=int, default=1, choices=(0, 1),
+                    help="batch24: complex-FP64 products of the batched pass by three real products (flag 12)")
+    ap.add_argument("--forms", default="plain,plain_g3,mixed_resolvent",
+                    help="comma-separated forms to time (plain, plain_g3, hermitian, mixed, mixed_bf16, resolvent, "
                          "mixed_resolvent, and their _trev / _c4 variants); plain runs first")
     ap.add_argument("--headline", default="mixed_resolvent", help="the form reported as value")
     args = ap.parse_args()
@@ -297,6 +304,7 @@ def main():
         ctx.set_flag(7, FORMS[form][1])
         ctx.set_flag(10, int(FORMS[form][0] == 2))  # the resolvent forms also project G as Hermitian
         ctx.set_flag(9, int(FORMS[form][2] == 1))
+        ctx.set_flag(12, FORMS[form][3])  # process-wide
         if FORMS[form][2] == 2:
             ctx.set_qsym(*s.qsym)
         else:
@@ -459,10 +467,13 @@ def main():
         "Ec_Ha": hd["Ec_Ha"],
         "plain_s_per_iteration": round(plain["ms_per_step"] / 1000.0, 6) if plain else None,
         "speedup_vs_plain": (1.0 if args.headline == "plain" else hd.get("speedup_vs_plain")) if plain else None,
+        "speedup_vs_plain_g3": (round(results["plain_g3"]["ms_per_step"] / ms_step, 3)
+                                if "plain_g3" in results else None),
         "qp_error_meV": (0.0 if args.headline == "plain" else hd.get("err_qp_meV")) if plain else None,
         "ec_error_Ha": (0.0 if args.headline == "plain" else hd.get("err_Ec_Ha")) if plain else None,
         "forms": {k: {kk: vv for kk, vv in v.items() if kk != "prof"} for k, v in results.items()},
-        "roofline": {"kernel": "%s (GEMM class %s of the %s form)" % (gemm_kernel_name(cls), cls, args.headline),
+        "roofline": {"kernel": "%s (GEMM class %s of the %s form)" % (gemm_kernel_name(cls, FORMS[args.headline][3]),
+                                                                       cls, args.headline),
                      "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": round(g_by / max(g_n, 1)),
@@ -518,6 +529,7 @@ def run_batch(args):
 
     # plain FP64 reference pass, one system after another; then the same pass timed
     ref = []
+    sysd[0]["ctx"].set_flag(12, 0)  # process-wide: four real products per complex product
     for d in sysd:
         d["ctx"].set_flag(6, 0)
         d["ctx"].set_flag(7, 0)
@@ -545,6 +557,7 @@ def run_batch(args):
         d["ctx"].set_flag(6, 2 if args.batch_dyson == "resolvent" else 1)
         d["ctx"].set_flag(10, int(args.batch_dyson == "resolvent"))  # Hermitian G projections
         d["ctx"].set_flag(7, 1)  # mixed precision
+    sysd[0]["ctx"].set_flag(12, int(args.batch_g3))  # process-wide: FP64 products by three real products
 
     def timed(steps, graph):
         b_set = b.lib.thcgw_batch_set(b.h, int(graph), 1)
@@ -602,6 +615,7 @@ def run_batch(args):
                    "systems": [{"kgrid": list(k), "n_orb": no, "n_mu": nmu} for k, no, nmu, _ in specs],
                    "ir_table": BATCH_TABLE, "n_tau": ia.nt, "beta": beta, "parallelism": "single GPU"},
         "batch_lanes": args.batch_lanes,
+        "fp64_three_real_products": bool(args.batch_g3),
         "ms_per_step_one_by_one": round(ms_eager, 3),
         "graph_speedup": round(ms_eager / ms_graph, 3),
         "ms_plain_fp64_one_by_one": round(ms_plain, 3),

@@ -1544,6 +1544,9 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
     else if (flag == 8) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 8 (FP32 tile) takes 0, 1 or 2");
       thcgw::gemm_set_tile_mode32(static_cast<int>(value));
+    } else if (flag == 12) {
+      THCGW_REQUIRE(value == 0 || value == 1, "flag 12 (FP64 complex product by three real products) takes 0 or 1");
+      thcgw::gemm_set_complex3(static_cast<int>(value));
     } else if (flag == 6) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 6 (Dyson algorithm) takes 0 (LU), 1 (Hermitian) or 2 (resolvent)");
       ctx->dyson_algo = static_cast<int>(value);
@@ -2307,6 +2310,8 @@ int thcgw_batch_gw_iteration(thcgw_batch* b, const double* const* G, const doubl
     }
     sig.push_back(stream);
     sig.push_back(reinterpret_cast<const void*>(static_cast<long>(want_ec)));
+    // the process-wide GEMM settings (flags 5, 8, 12) pick the kernels a graph holds
+    sig.push_back(reinterpret_cast<const void*>(static_cast<long>(thcgw::gemm_mode_key())));
     // A scratch buffer grown since the capture (a context run alone with other settings)
     // leaves the graph pointing at freed memory: the allocation count is part of the key.
     sig.push_back(reinterpret_cast<const void*>(b->gen));

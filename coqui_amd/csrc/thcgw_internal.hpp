@@ -113,6 +113,8 @@ void gemm_stats_reset();
 void gemm_profile_enable(bool on);
 void gemm_set_tile_mode(int mode);    // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
 void gemm_set_tile_mode32(int mode);  // FP32 products without FP64 folding: 0 = 64x64, 1 = 128x64, 2 = 128x128
+void gemm_set_complex3(int on);       // FP64 64x64 products: 1 = three real MFMA products per complex product
+int gemm_mode_key();                  // the process-wide settings above, packed (a captured graph's key)
 constexpr int GEMM_PROFILE_CLASSES = 17;
 void gemm_profile_read(double* ms, double* flops, double* bytes, long* launches);
 

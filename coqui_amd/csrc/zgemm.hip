@@ -146,7 +146,8 @@ struct TileIO {
 // FOLD (FP32 arithmetic only): after every K chunk the FP32 accumulators are added into
 // FP64 accumulators and cleared, so rounding grows with the chunk length (16) instead of
 // with K.
-template <typename R, typename TA, typename TB, typename TC, int OPA, int OPB, int BM, int BN, bool FOLD = false>
+template <typename R, typename TA, typename TB, typename TC, int OPA, int OPB, int BM, int BN, bool FOLD = false,
+          bool G3 = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
     BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : (BN == 64 || sizeof(R) == 4 ? 2 : 1))))
 void gemm_kernel(Args p) {
@@ -191,7 +192,12 @@ void gemm_kernel(Args p) {
   const int wm = wave >> 1, wn = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
 
-  typename AR::acc cr[TM][TN], ci[TM][TN];
+  // G3: three real products per complex product (Gauss): cr accumulates Re a Re b, ci
+  // Im a Im b and cs (Re a + Im a)(Re b + Im b); after the K loop Re c = cr - ci and
+  // Im c = cs - cr - ci.
+  static_assert(!(G3 && FOLD), "three-product form without FP64 folding only");
+  constexpr int GT = G3 ? TM : 1, GN = G3 ? TN : 1;
+  typename AR::acc cr[TM][TN], ci[TM][TN], cs[GT][GN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -199,6 +205,10 @@ void gemm_kernel(Args p) {
       cr[i][j] = typename AR::acc{0, 0, 0, 0};
       ci[i][j] = typename AR::acc{0, 0, 0, 0};
     }
+#pragma unroll
+  for (int i = 0; i < GT; ++i)
+#pragma unroll
+    for (int j = 0; j < GN; ++j) cs[i][j] = typename AR::acc{0, 0, 0, 0};
   constexpr int FT = FOLD ? TM : 1, FN = FOLD ? TN : 1;
   f64x4 fr[FT][FN], fi[FT][FN];
   if constexpr (FOLD) {
@@ -230,6 +240,33 @@ void gemm_kernel(Args p) {
     }
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
+      if constexpr (G3) {
+        R a_r[TM], a_i[TM], a_s[TM], b_r[TN], b_i[TN], b_s[TN];
+        const int kk = ks * 4 + lk;
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const RC v = As[cur][kk][(wm * (BM / 2) + i * 16 + lr) ^ swz<RC>(kk)];
+          a_r[i] = v.x;
+          a_i[i] = v.y;
+          a_s[i] = v.x + v.y;
+        }
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          const RC v = Bs[cur][kk][(wn * (BN / 2) + j * 16 + lr) ^ swz<RC>(kk)];
+          b_r[j] = v.x;
+          b_i[j] = v.y;
+          b_s[j] = v.x + v.y;
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            cr[i][j] = AR::mfma(a_r[i], b_r[j], cr[i][j]);
+            ci[i][j] = AR::mfma(a_i[i], b_i[j], ci[i][j]);
+            cs[i][j] = AR::mfma(a_s[i], b_s[j], cs[i][j]);
+          }
+        continue;
+      }
       R a_r[TM], a_i[TM], a_n[TM], b_r[TN], b_i[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) {
@@ -279,6 +316,19 @@ void gemm_kernel(Args p) {
       IOB::store(Bs[cur ^ 1], rb);
     }
     __syncthreads();
+  }
+
+  if constexpr (G3) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const R t1 = cr[i][j][v], t2 = ci[i][j][v];
+          cr[i][j][v] = t1 - t2;
+          ci[i][j][v] = cs[i][j][v] - t1 - t2;
+        }
   }
 
   // Epilogue (in double whatever the arithmetic).
@@ -676,6 +726,8 @@ hipEvent_t pool_event() {
 }
 
 // Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128 (complex<double> only).
+std::atomic<int> g_complex3{0};  // FP64 64 x 64 tiles: 1 = three real products per complex product
+
 template <int OPA, int OPB>
 void launch(hipStream_t s, Args a, int shape) {
   const int bm = shape == 0 ? 64 : 128, bn = shape == 2 ? 128 : 64;
@@ -684,7 +736,9 @@ void launch(hipStream_t s, Args a, int shape) {
   const long nblocks = static_cast<long>(a.batch) * a.tiles_m * a.tiles_n;
   THCGW_REQUIRE(nblocks < (1L << 31), "zgemm: grid too large");
   const dim3 g(static_cast<unsigned>(nblocks)), t(NT);
-  if (shape == 0)
+  if (shape == 0 && g_complex3.load())
+    hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 64, 64, false, true>), g, t, 0, s, a);
+  else if (shape == 0)
     hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 64, 64>), g, t, 0, s, a);
   else if (shape == 1)
     hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 128, 64>), g, t, 0, s, a);
@@ -878,6 +932,8 @@ void gemm_stats_reset() {
 
 void gemm_set_tile_mode(int mode) { g_tile_mode = mode; }
 void gemm_set_tile_mode32(int mode) { g_tile_mode32 = mode; }
+void gemm_set_complex3(int on) { g_complex3 = on; }
+int gemm_mode_key() { return g_tile_mode.load() | (g_tile_mode32.load() << 4) | (g_complex3.load() << 8); }
 
 void gemm_profile_enable(bool on) {
   std::lock_guard<std::mutex> lk(g_prof_mu);

@@ -263,3 +263,52 @@ def test_fp64_tile_shapes(orc, tile):
             assert abs(ec - ref["Ec"]) < 1e-10 * abs(ref["Ec"])
     finally:
         ctx.set_flag(5, 0)
+
+
+def test_fp64_three_products(orc):
+    """Complex-FP64 GEMMs by three real MFMA products (flag 12) give the oracle's iteration
+    in every Dyson form (1e-10; the resolvent form 1e-10 + cond(V) eps).  In the mixed
+    forms, whose FP64 products are the Dyson step's, the result stays within the mixed
+    tolerances (W, Sigma 5e-5 relative, Ec 1e-6 Ha) and differs from the four-product
+    result by less than 10% of the mixed form's own error."""
+    import torch
+
+    kgrid, no, nmu, table = (2, 2, 2), 10, 150, "1e2.1e-15"
+    print("case: three-product FP64 GEMMs", flush=True)
+    s = make_system(kgrid, no, nmu, system_id=15)
+    tab = orc.load_ir(1000.0, None, table=table)
+    ctx = ThcGW(kgrid, no, nmu, IAFT.from_table(1000.0, table))
+    G = ctx.update_G(s.C, s.E, 0.0)
+    ref = orc.gw_iteration(tab, orc.update_G(tab, s.C, s.E), s.X, s.V, kgrid, with_ec=True)
+    try:
+        ctx.set_flag(12, 1)
+        for f6 in (0, 1, 2):
+            ctx.set_flag(6, f6)
+            S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+            torch.cuda.synchronize()
+            to = 1e-10 if f6 < 2 else 1e-10 + max(np.linalg.cond(v) for v in s.V) * np.finfo(float).eps
+            print("FP64 Dyson form %d: W %.2e Sigma %.2e" % (f6, relmax(W.cpu().numpy(), ref["W_qt"]),
+                                                              relmax(S.cpu().numpy(), ref["Sigma"])))
+            assert np.max(np.abs(S.cpu().numpy() - ref["Sigma"])) < to * np.max(np.abs(ref["Sigma"]))
+            assert np.max(np.abs(W.cpu().numpy() - ref["W_qt"])) < to * np.max(np.abs(ref["W_qt"]))
+            assert abs(ec - ref["Ec"]) < 1e-10 * abs(ref["Ec"])
+        ctx.set_flag(7, 1)
+        for f6 in (1, 2):
+            ctx.set_flag(6, f6)
+            ctx.set_flag(10, int(f6 == 2))
+            out = {}
+            for g3 in (0, 1):
+                ctx.set_flag(12, g3)
+                S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True, hermitize=True)
+                out[g3] = (S.cpu().numpy(), W.cpu().numpy(), ec)
+            e_w, e_s = relmax(out[1][1], ref["W_qt"]), relmax(out[1][0], ref["Sigma"])
+            d_w, d_s = relmax(out[1][1], out[0][1]), relmax(out[1][0], out[0][0])
+            print("mixed, Dyson form %d: W %.2e Sigma %.2e; three vs four products: W %.2e Sigma %.2e"
+                  % (f6, e_w, e_s, d_w, d_s))
+            assert e_w < 5e-5 and e_s < 5e-5 and abs(out[1][2] - ref["Ec"]) < 1e-6
+            # Sigma takes W(tau) through FP32 k-grid transforms: where the two W differ by a
+            # few FP64 ulps, an FP32 rounding of W can flip by one FP32 ulp
+            assert d_w < 0.1 * relmax(out[0][1], ref["W_qt"]) and d_s < 0.1 * relmax(out[0][0], ref["Sigma"])
+    finally:
+        ctx.set_flag(12, 0)
+        ctx.close()
