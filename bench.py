@@ -11,7 +11,8 @@ the plain form, which runs first):
   plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm),
              complex products by four real MFMA products;
   plain_g3   the plain form with complex-FP64 products by three real MFMA products (flag 12,
-             Gauss's form), as every form below uses;
+             Gauss's form), as every form below uses (--g3-mask, default 5: also the FP32
+             products with FP64 folding, i.e. the back-projection of Sigma);
   hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
              transforms, Cholesky factorisations and ln det;
@@ -211,6 +212,9 @@ def main():
                     help="batch24: streams the systems are spread over (parallel branches of the graph)")
     ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
                     help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
+    ap.add_argument("--g3-mask", type=int, default=5,
+                    help="flag 12 of the forms with three-product GEMMs: 1 FP64, | 2 FP32, | 4 FP32 with FP64 folding "
+                         "(default 5: not the FP32 chi0 projections, where three products triple the Ec error)")
     ap.add_argument("--batch-g3", type=int, default=1, choices=(0, 1),
                     help="batch24: complex-FP64 products of the batched pass by three real products (flag 12)")
     ap.add_argument("--forms", default="plain,plain_g3,mixed_resolvent",
@@ -304,7 +308,7 @@ def main():
         ctx.set_flag(7, FORMS[form][1])
         ctx.set_flag(10, int(FORMS[form][0] == 2))  # the resolvent forms also project G as Hermitian
         ctx.set_flag(9, int(FORMS[form][2] == 1))
-        ctx.set_flag(12, FORMS[form][3])  # process-wide
+        ctx.set_flag(12, FORMS[form][3] * args.g3_mask if FORMS[form][3] else 0)  # process-wide
         if FORMS[form][2] == 2:
             ctx.set_qsym(*s.qsym)
         else:

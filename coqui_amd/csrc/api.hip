@@ -1545,7 +1545,7 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 8 (FP32 tile) takes 0, 1 or 2");
       thcgw::gemm_set_tile_mode32(static_cast<int>(value));
     } else if (flag == 12) {
-      THCGW_REQUIRE(value == 0 || value == 1, "flag 12 (FP64 complex product by three real products) takes 0 or 1");
+      THCGW_REQUIRE(value >= 0 && value <= 7, "flag 12 (complex products by three real products) takes a mask 0..7");
       thcgw::gemm_set_complex3(static_cast<int>(value));
     } else if (flag == 6) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 6 (Dyson algorithm) takes 0 (LU), 1 (Hermitian) or 2 (resolvent)");

@@ -149,7 +149,7 @@ struct TileIO {
 template <typename R, typename TA, typename TB, typename TC, int OPA, int OPB, int BM, int BN, bool FOLD = false,
           bool G3 = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
-    BM == 64 ? (FOLD ? 3 : Arith<R>::WAVES) : (BN == 64 || sizeof(R) == 4 ? 2 : 1))))
+    BM == 64 ? (FOLD ? (G3 ? 2 : 3) : Arith<R>::WAVES) : (BN == 64 || sizeof(R) == 4 ? 2 : 1))))
 void gemm_kernel(Args p) {
   using AR = Arith<R>;
   using RC = typename AR::cplx;
@@ -193,9 +193,9 @@ void gemm_kernel(Args p) {
   const int lr = lane & 15, lk = lane >> 4;
 
   // G3: three real products per complex product (Gauss): cr accumulates Re a Re b, ci
-  // Im a Im b and cs (Re a + Im a)(Re b + Im b); after the K loop Re c = cr - ci and
+  // Im a Im b and cs (Re a + Im a)(Re b + Im b); after the K loop (with FOLD: after every K
+  // chunk, before the chunk is added into the FP64 accumulators) Re c = cr - ci and
   // Im c = cs - cr - ci.
-  static_assert(!(G3 && FOLD), "three-product form without FP64 folding only");
   constexpr int GT = G3 ? TM : 1, GN = G3 ? TN : 1;
   typename AR::acc cr[TM][TN], ci[TM][TN], cs[GT][GN];
 #pragma unroll
@@ -305,8 +305,15 @@ void gemm_kernel(Args p) {
         for (int j = 0; j < TN; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            fr[i][j][v] += static_cast<double>(cr[i][j][v]);
-            fi[i][j][v] += static_cast<double>(ci[i][j][v]);
+            if constexpr (G3) {
+              const R t1 = cr[i][j][v], t2 = ci[i][j][v];
+              fr[i][j][v] += static_cast<double>(t1 - t2);
+              fi[i][j][v] += static_cast<double>(cs[i][j][v] - t1 - t2);
+              cs[i][j][v] = 0;
+            } else {
+              fr[i][j][v] += static_cast<double>(cr[i][j][v]);
+              fi[i][j][v] += static_cast<double>(ci[i][j][v]);
+            }
             cr[i][j][v] = 0;
             ci[i][j][v] = 0;
           }
@@ -318,7 +325,7 @@ void gemm_kernel(Args p) {
     __syncthreads();
   }
 
-  if constexpr (G3) {
+  if constexpr (G3 && !FOLD) {
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -726,7 +733,9 @@ hipEvent_t pool_event() {
 }
 
 // Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128 (complex<double> only).
-std::atomic<int> g_complex3{0};  // FP64 64 x 64 tiles: 1 = three real products per complex product
+// three real products per complex product on 64 x 64 tiles: bit 0 FP64 arithmetic, bit 1 FP32
+// without FP64 folding, bit 2 FP32 with FP64 folding (not the split-BF16 products)
+std::atomic<int> g_complex3{0};
 
 template <int OPA, int OPB>
 void launch(hipStream_t s, Args a, int shape) {
@@ -736,7 +745,7 @@ void launch(hipStream_t s, Args a, int shape) {
   const long nblocks = static_cast<long>(a.batch) * a.tiles_m * a.tiles_n;
   THCGW_REQUIRE(nblocks < (1L << 31), "zgemm: grid too large");
   const dim3 g(static_cast<unsigned>(nblocks)), t(NT);
-  if (shape == 0 && g_complex3.load())
+  if (shape == 0 && (g_complex3.load() & 1))
     hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 64, 64, false, true>), g, t, 0, s, a);
   else if (shape == 0)
     hipLaunchKernelGGL((gemm_kernel<double, c128, c128, c128, OPA, OPB, 64, 64>), g, t, 0, s, a);
@@ -770,9 +779,15 @@ void launch_f32(hipStream_t s, Args a) {
     else
       hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, OPA, OPB, false>), dim3(static_cast<unsigned>(nblocks)),
                          dim3(NT), 0, s, a);
-  } else if (a.tri & GEMM_FOLD64)
+  } else if ((a.tri & GEMM_FOLD64) && (g_complex3.load() & 4))
+    hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, true, true>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(NT), 0, s, a);
+  else if (a.tri & GEMM_FOLD64)
     hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, true>), dim3(static_cast<unsigned>(nblocks)),
                        dim3(NT), 0, s, a);
+  else if (g_complex3.load() & 2)
+    hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, false, true>),
+                       dim3(static_cast<unsigned>(nblocks)), dim3(NT), 0, s, a);
   else
     hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64>), dim3(static_cast<unsigned>(nblocks)),
                        dim3(NT), 0, s, a);

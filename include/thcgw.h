@@ -439,12 +439,16 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           thcgw_gw_iteration refuse G with max |G - G^H| > 1e-12 max |G| (THCGW_ERR_ARG);
  *   flag 8: FP32 GEMM tile shape (process-wide; products without FP64 chunk accumulation):
  *           0 = 64x64, 1 = 128x64, 2 = 128x128;
- *   flag 12: complex-FP64 GEMMs on 64x64 tiles (process-wide): 0 = four real MFMA products
- *           per complex product (default), 1 = three (Gauss: Re = ArBr - AiBi,
- *           Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi), a quarter fewer MFMA cycles.  Its real part
- *           rounds as before; the imaginary part's error is bounded by eps (|Ar| + |Ai|)
- *           (|Br| + |Bi|) per term rather than by eps |A| |B| (normwise stable, Higham 1992);
- *           products on the 128-wide tiles (flag 5 = 2, 3) keep four;
+ *   flag 12: complex GEMMs on 64x64 tiles by three real MFMA products per complex product
+ *           instead of four (process-wide mask; 0 = four everywhere, the default): bit 0 the
+ *           FP64 products, bit 1 the FP32 products without FP64 folding, bit 2 the FP32
+ *           products with FP64 folding (the mixed forms' back-projection of Sigma and the
+ *           mixed Hermitian Dyson products).  Three products (Gauss: Re = ArBr - AiBi,
+ *           Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi) take a quarter fewer MFMA cycles; the real
+ *           part rounds as before, the imaginary part's error is bounded by
+ *           eps (|Ar| + |Ai|)(|Br| + |Bi|) per term rather than eps |A||B| (normwise stable,
+ *           Higham 1992).  Products on the 128-wide tiles (flags 5, 8) and split-BF16
+ *           products keep four;
  *   flag 7: precision: 0 = FP64 throughout (default); 1 = mixed: the projections X G X^H and
  *           X^H Sigma X, the k-grid transforms with the Hadamard products, and the products
  *           of the Hermitian Dyson form run in FP32 (complex<float> intermediates), while
