@@ -454,14 +454,16 @@ int choose_dyson_chunk(thcgw_ctx* ctx, int total) {
   const double per = 2.0 * double(ctx->M) * 16.0;
   // The chunk buffers already held are reused, so they count as free: otherwise the chunk
   // shrinks after the first call wherever memory is short.  Larger chunks run faster per
-  // matrix: at N_k = 64, N_mu = 1600 the mixed resolvent iteration takes 1.310 s with 181
-  // matrices per chunk and 1.259 s with 400; at N_mu = 800 the plain iteration takes 0.645 s
-  // in chunks of 1677 + 691 matrices and 0.635 s in one chunk of 2368.
+  // matrix (the recursion's small levels take one launch per chunk): at N_k = 64,
+  // N_mu = 1600 the mixed resolvent iteration takes 1.225 s in chunks sized to 45% of the free
+  // memory, 1.208 s with 437 matrices and 1.204 s with 525, while 656 (54 GB of chunk
+  // buffers) left the runtime without memory for its own queues.  60% keeps 40% of the
+  // free memory for the runtime and later workspaces.
   double held = 0.0;
   for (const char* name : {"dy_lu", "dy_z", "dy_z64", "dy_m64", "dy_lu1", "dy_z1"})
     held += double(ctx->held_bytes(name));
   const double budget =
-      std::min(0.45 * (double(free_device_bytes()) + held), 64.0 * double(1L << 30)) / ctx->mem_share;
+      std::min(0.6 * (double(free_device_bytes()) + held), 64.0 * double(1L << 30)) / ctx->mem_share;
   const int c = std::max(1, std::min(static_cast<int>(budget / per), total));
   // equal chunks rather than full ones and a short last one
   const int nchunks = (total + c - 1) / c;

@@ -410,7 +410,7 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  * control flags and GEMM launch timing.
  *   flag 1: use the generic (LDS) k-grid transform kernel instead of the specialised ones;
  *   flag 2: tau pairs per projection batch (0 = automatic);
- *   flag 3: matrices per Dyson batch (0 = automatic: equal batches within 45% of free device
+ *   flag 3: matrices per Dyson batch (0 = automatic: equal batches within 60% of free device
  *           memory, at most 64 GiB, counting the batch buffers ctx already holds);
  *   flag 4: record HIP events around every complex-FP64 GEMM launch (process-wide);
  *   flag 5: GEMM tile shape (process-wide): 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128;
