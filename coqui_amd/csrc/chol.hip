@@ -84,9 +84,11 @@ __global__ __launch_bounds__(CB* CB) void potrf_leaf_kernel(c128* A, long sA, in
 // (L[k][k] = sqrt(d_k)) and L^-1[r][c] = T[r][c] / sqrt(d_r).  info as for potrf_leaf_kernel
 // (a non-positive or NaN pivot is recorded and replaced by 1).  Order 32: 20.7 us per call on
 // 200 matrices, against 46.1 us for a leaf that scales column j and forms L^-1 by one
-// thread's forward substitution per column (three barriers per column); an order-64 leaf
-// (four elements per thread, 66.5 KB of LDS) takes 89 us, which only matches the order-32
-// leaf together with the products of the level it replaces.
+// thread's forward substitution per column (three barriers per column).  Order-64 leaves
+// took 89 us (this scheme, four elements per thread, 66.5 KB of LDS) and 78 us (the block
+// in registers, one published column per step): VALU issue, not LDS, bounds them, and
+// either only matched the order-32 leaf together with the products of the level it
+// replaces.
 template <int NB>
 __global__ __launch_bounds__(1024) void cholinv_leaf_kernel(c128* A, long sA, int lda, c128* Z, long sZ, int ldz,
                                                             int j0, int nb, int* info) {
