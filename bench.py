@@ -11,8 +11,9 @@ the plain form, which runs first):
   plain      FP64 throughout, Dyson by LU with partial pivoting (the reference algorithm),
              complex products by four real MFMA products;
   plain_g3   the plain form with complex-FP64 products by three real MFMA products (flag 12,
-             Gauss's form), as every form below uses (--g3-mask, default 5: also the FP32
-             products with FP64 folding, i.e. the back-projection of Sigma);
+             Gauss's form), as every form below uses (--g3-mask, default 13: also the FP32
+             products with FP64 folding, i.e. the back-projection of Sigma, and Sigma's FP32
+             projections of G);
   hermitian  FP64 throughout, Dyson by the two-Cholesky Hermitian form;
   mixed      FP32 products (projections, k-grid transforms, Dyson products), FP64 tau<->iW
              transforms, Cholesky factorisations and ln det;
@@ -212,9 +213,10 @@ def main():
                     help="batch24: streams the systems are spread over (parallel branches of the graph)")
     ap.add_argument("--batch-dyson", default="resolvent", choices=("hermitian", "resolvent"),
                     help="batch24: Dyson form of the mixed-precision batch (FP32 Hermitian or FP64 resolvent)")
-    ap.add_argument("--g3-mask", type=int, default=5,
+    ap.add_argument("--g3-mask", type=int, default=13,
                     help="flag 12 of the forms with three-product GEMMs: 1 FP64, | 2 FP32, | 4 FP32 with FP64 "
-                         "folding (default 5: not the FP32 chi0 projections, where three products triple the Ec error)")
+                         "folding, | 8 Sigma's FP32 projections (default 13: not the FP32 chi0 projections, where "
+                         "three products triple the Ec error)")
     ap.add_argument("--batch-g3-mask", type=int, default=1,
                     help="batch24: flag 12 of the batched pass (default 1, FP64 only: with 5 the batch is no faster "
                          "and its largest QP error grows from 0.63 to 0.93 meV)")

@@ -317,16 +317,17 @@ const c64* x_c64(thcgw_ctx* ctx, const c128* X, hipStream_t s) {
 // O_out[b,k] = X_k O_in[b,k] X_k^H with O_in complex<double>, result complex<float>; herm as
 // for project_p2a.
 void project_p2a_f32(thcgw_ctx* ctx, hipStream_t s, const c64* X64, const c128* Oin, c64* Oout, long nbk, c64* Abuf,
-                     bool herm = false) {
+                     bool herm = false, bool g3 = false) {
+  const int f = ctx->pf() | (g3 ? thcgw::GEMM_G3 : 0);
   const int nk = ctx->nk, no = ctx->d.no, nmu = ctx->d.nmu;
   const long sX = static_cast<long>(nmu) * no;
   THCGW_REQUIRE(nbk < (1L << 30), "primary_to_aux: batch too large");
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_N, nmu, no, no, kOne, thcgw::bp(X64, sX, nk), DT_C64, no,
                thcgw::bp(Oin, static_cast<long>(no) * no), DT_C128, no, kZero, Abuf, DT_C64, sX, no,
-               static_cast<int>(nbk), kZero, ctx->pf());
+               static_cast<int>(nbk), kZero, f);
   thcgw::cgemm(s, thcgw::OP_N, thcgw::OP_C, nmu, nmu, no, kOne, thcgw::bp(Abuf, sX), DT_C64, no,
                thcgw::bp(X64, sX, nk), DT_C64, no, kZero, Oout, DT_C64, ctx->M, nmu, static_cast<int>(nbk), kZero,
-               ctx->pf() | (herm ? (thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB) : 0));
+               f | (herm ? (thcgw::GEMM_TRI_LOWER | thcgw::GEMM_HERM_SUB) : 0));
 }
 
 // O_out[b,k] = X_k^H O_in[b,k] X_k with O_in complex<float>, result complex<double>; herm: O_in
@@ -999,12 +1000,14 @@ void run_sigma_f32(thcgw_ctx* ctx, const c128* G, const c128* Wqt, const c128* X
   c64* SN = ctx->buf<c64>("mx_sn", static_cast<size_t>(P) * nk * M);
   c64* Ab = ctx->buf<c64>("mx_ab", static_cast<size_t>(P) * nk * nmu * no);
   const double scale = -1.0 / (double(nk) * nk);
+  // flag 12 bit 3: these projections (Sigma's only, not chi0's) by three real products
+  const bool g3 = (thcgw::gemm_complex3() & 8) != 0;
   for (int i0 = 0; i0 < nth; i0 += P) {
     const int c = std::min(P, nth - i0);
     const int c2 = std::max(0, std::min(c, nt / 2 - i0));
-    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g);
+    project_p2a_f32(ctx, s, X64, G + i0 * gs, GP, static_cast<long>(c) * nk, Ab, ctx->herm_g, g3);
     if (c2 > 0)
-      project_p2a_f32(ctx, s, X64, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab, ctx->herm_g);
+      project_p2a_f32(ctx, s, X64, G + (nt - i0 - c2) * gs, GN, static_cast<long>(c2) * nk, Ab, ctx->herm_g, g3);
     for (int j = 0; j < c; ++j) {
       const int i = i0 + j;
       const bool second = j < c2;
@@ -1547,7 +1550,7 @@ int thcgw_set_flag(thcgw_ctx* ctx, int flag, long value) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 8 (FP32 tile) takes 0, 1 or 2");
       thcgw::gemm_set_tile_mode32(static_cast<int>(value));
     } else if (flag == 12) {
-      THCGW_REQUIRE(value >= 0 && value <= 7, "flag 12 (complex products by three real products) takes a mask 0..7");
+      THCGW_REQUIRE(value >= 0 && value <= 15, "flag 12 (complex products by three real products) takes a mask 0..15");
       thcgw::gemm_set_complex3(static_cast<int>(value));
     } else if (flag == 6) {
       THCGW_REQUIRE(value >= 0 && value <= 2, "flag 6 (Dyson algorithm) takes 0 (LU), 1 (Hermitian) or 2 (resolvent)");

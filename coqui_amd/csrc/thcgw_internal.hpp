@@ -90,6 +90,9 @@ constexpr int GEMM_TILE32_SHIFT = 6;
 // C = X - H on both triangles from the lower tiles (H complex<double>, same layout as C, or
 // none for C = X; no beta, gamma or split BF16).
 constexpr int GEMM_HERM_SUB = 256;
+// cgemm only: three real MFMA products per complex product for this product, whatever the
+// process-wide setting (gemm_set_complex3) says for FP32 products.
+constexpr int GEMM_G3 = 2048;
 
 // C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] + gamma * I   (row-major, M x N)
 void zgemm(hipStream_t s, int opA, int opB, int M, int N, int K, Cplx alpha, BatchPtr A, long lda,
@@ -113,7 +116,8 @@ void gemm_stats_reset();
 void gemm_profile_enable(bool on);
 void gemm_set_tile_mode(int mode);    // 0 automatic, 1 = 64x64, 2 = 128x64, 3 = 128x128
 void gemm_set_tile_mode32(int mode);  // FP32 products without FP64 folding: 0 = 64x64, 1 = 128x64, 2 = 128x128
-void gemm_set_complex3(int on);       // FP64 64x64 products: 1 = three real MFMA products per complex product
+void gemm_set_complex3(int on);       // 64x64 products by three real MFMA products: mask, see flag 12
+int gemm_complex3();                  // the mask set last
 int gemm_mode_key();                  // the process-wide settings above, packed (a captured graph's key)
 constexpr int GEMM_PROFILE_CLASSES = 17;
 void gemm_profile_read(double* ms, double* flops, double* bytes, long* launches);

@@ -734,7 +734,8 @@ hipEvent_t pool_event() {
 
 // Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128 (complex<double> only).
 // three real products per complex product on 64 x 64 tiles: bit 0 FP64 arithmetic, bit 1 FP32
-// without FP64 folding, bit 2 FP32 with FP64 folding (not the split-BF16 products)
+// without FP64 folding, bit 2 FP32 with FP64 folding (not the split-BF16 products); bit 3 is
+// read by the callers (the FP32 projections of G for Sigma, through GEMM_G3)
 std::atomic<int> g_complex3{0};
 
 template <int OPA, int OPB>
@@ -779,13 +780,13 @@ void launch_f32(hipStream_t s, Args a) {
     else
       hipLaunchKernelGGL((gemm_bf16x6_kernel<TA, TB, TC, OPA, OPB, false>), dim3(static_cast<unsigned>(nblocks)),
                          dim3(NT), 0, s, a);
-  } else if ((a.tri & GEMM_FOLD64) && (g_complex3.load() & 4))
+  } else if ((a.tri & GEMM_FOLD64) && ((g_complex3.load() & 4) || (a.tri & GEMM_G3)))
     hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, true, true>),
                        dim3(static_cast<unsigned>(nblocks)), dim3(NT), 0, s, a);
   else if (a.tri & GEMM_FOLD64)
     hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, true>), dim3(static_cast<unsigned>(nblocks)),
                        dim3(NT), 0, s, a);
-  else if (g_complex3.load() & 2)
+  else if ((g_complex3.load() & 2) || (a.tri & GEMM_G3))
     hipLaunchKernelGGL((gemm_kernel<float, TA, TB, TC, OPA, OPB, 64, 64, false, true>),
                        dim3(static_cast<unsigned>(nblocks)), dim3(NT), 0, s, a);
   else
@@ -948,6 +949,7 @@ void gemm_stats_reset() {
 void gemm_set_tile_mode(int mode) { g_tile_mode = mode; }
 void gemm_set_tile_mode32(int mode) { g_tile_mode32 = mode; }
 void gemm_set_complex3(int on) { g_complex3 = on; }
+int gemm_complex3() { return g_complex3.load(); }
 int gemm_mode_key() { return g_tile_mode.load() | (g_tile_mode32.load() << 4) | (g_complex3.load() << 8); }
 
 void gemm_profile_enable(bool on) {

@@ -443,7 +443,8 @@ int thcgw_set_qsym(thcgw_ctx* ctx, const int* src, const int* perm, const double
  *           instead of four (process-wide mask; 0 = four everywhere, the default): bit 0 the
  *           FP64 products, bit 1 the FP32 products without FP64 folding, bit 2 the FP32
  *           products with FP64 folding (the mixed forms' back-projection of Sigma and the
- *           mixed Hermitian Dyson products).  Three products (Gauss: Re = ArBr - AiBi,
+ *           mixed Hermitian Dyson products), bit 3 the FP32 projections of G for Sigma
+ *           only (bit 1 also covers those for chi0, whose rounding feeds Ec).  Three products (Gauss: Re = ArBr - AiBi,
  *           Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi) take a quarter fewer MFMA cycles; the real
  *           part rounds as before, the imaginary part's error is bounded by
  *           eps (|Ar| + |Ai|)(|Br| + |Bi|) per term rather than eps |A||B| (normwise stable,

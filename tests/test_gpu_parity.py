@@ -269,7 +269,7 @@ def test_fp64_three_products(orc):
     """Complex GEMMs by three real MFMA products (flag 12): in FP64 (bit 0) they give the
     oracle's iteration in every Dyson form (1e-10; the resolvent form 1e-10 + cond(V) eps).
     In the mixed forms the result stays within the mixed tolerances (W, Sigma 5e-5
-    relative, Ec 1e-6 Ha) also with the FP32 products on three (bits 1, 2), and with the
+    relative, Ec 1e-6 Ha) also with the FP32 products on three (bits 1, 2, 3), and with the
     FP64 ones only it differs from the four-product result by less than 10% of the mixed
     form's own error."""
     import torch
@@ -298,7 +298,7 @@ def test_fp64_three_products(orc):
             ctx.set_flag(6, f6)
             ctx.set_flag(10, int(f6 == 2))
             out = {}
-            for g3 in (0, 1, 5, 7):
+            for g3 in (0, 1, 5, 7, 13):
                 ctx.set_flag(12, g3)
                 S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True, hermitize=True)
                 out[g3] = (S.cpu().numpy(), W.cpu().numpy(), ec)

@@ -36,8 +36,9 @@ TABLE = "1e3.5e-18"  # 80 tau points
 HA_PER_MEV = 1.0 / 27211.386245988
 SHAPES = {"k64-mu800": ((4, 4, 4), 52, 800), "k64-mu1600": ((4, 4, 4), 104, 1600)}
 # flag 12 (process-wide, reset when a form's context closes): the headline form's complex
-# products by three real MFMA products, FP64 and the folded FP32 back-projection of Sigma
-FORMS = {"plain": {6: 0, 7: 0, 10: 0, 12: 0}, "mixed_resolvent": {6: 2, 7: 1, 10: 1, 12: 5}}
+# products by three real MFMA products: FP64, the folded FP32 back-projection of Sigma and
+# Sigma's FP32 projections of G
+FORMS = {"plain": {6: 0, 7: 0, 10: 0, 12: 0}, "mixed_resolvent": {6: 2, 7: 1, 10: 1, 12: 13}}
 TOLS = {"plain": dict(pi=1e-10, w=1e-10, s=1e-10, herm=1e-12),
         "mixed_resolvent": dict(pi=1e-5, w=1e-5, s=1e-4, herm=1e-5)}
 
