@@ -264,11 +264,9 @@ __global__ __launch_bounds__(256) void chol_logdet_kernel(const c128* L, long sL
 // lower triangle of A[b] = Vi[m] - P[b], and, when part is given, per workgroup the partial
 // sum of Tr(V P) for Hermitian V and P, sum_i V_ii P_ii + 2 sum_{i>j} Re(conj(V_ij) P_ij)
 // (real), at part[b * gridDim.x + blockIdx.x].  Each workgroup covers RP_ELEMS consecutive
-// elements of the lower triangle in packed row order (row r holds r + 1 of them) of the
-// matrices b = j, j + vmod, j + 2 vmod, ... (j = blockIdx.y) that share V[m] and Vi[m],
-// RP_GROUP of them per pass: the V and Vi elements are read once per pass instead of once
-// per matrix.  The form reads only these lower triangles.  (Walking the full matrices and
-// skipping the upper elements left half the lanes idle: 0.65 of the HBM peak.)
+// elements of the matrices b = j, j + vmod, j + 2 vmod, ... (j = blockIdx.y) that share
+// V[m] and Vi[m], RP_GROUP of them per pass: the V and Vi elements are read once per pass
+// instead of once per matrix.  The form reads only these lower triangles.
 constexpr int RP_ELEMS = 4096;
 constexpr int RP_GROUP = 8;
 __global__ __launch_bounds__(256) void resolvent_prep_kernel(const c128* V, const c128* Vi, long sV, int vmod,
@@ -276,40 +274,36 @@ __global__ __launch_bounds__(256) void resolvent_prep_kernel(const c128* V, cons
                                                              c128* part, int batch) {
   __shared__ double red[RP_GROUP][256];
   const int j = blockIdx.y;
-  const long nl = static_cast<long>(n) * (n + 1) / 2;  // packed lower-triangle elements
+  const long nn = static_cast<long>(n) * n;
   const long m = static_cast<long>((j + voff) % vmod) * sV;
   const long e0 = static_cast<long>(blockIdx.x) * RP_ELEMS;
-  const long e1 = e0 + RP_ELEMS < nl ? e0 + RP_ELEMS : nl;
-  // row and column of the thread's first packed element: r (r + 1) / 2 <= p < (r + 1) (r + 2) / 2
-  const long p0 = e0 + threadIdx.x;
-  long r0 = static_cast<long>((sqrt(8.0 * static_cast<double>(p0) + 1.0) - 1.0) * 0.5);
-  while ((r0 + 1) * (r0 + 2) / 2 <= p0) ++r0;
-  while (r0 * (r0 + 1) / 2 > p0) --r0;
-  const int rfirst = static_cast<int>(r0), cfirst = static_cast<int>(p0 - r0 * (r0 + 1) / 2);
+  const long e1 = e0 + RP_ELEMS < nn ? e0 + RP_ELEMS : nn;
   for (int bg = j; bg < batch; bg += RP_GROUP * vmod) {
     int nb = 0;  // matrices in this pass: bg + k vmod for k < nb
     while (nb < RP_GROUP && bg + nb * vmod < batch) ++nb;
     double acc[RP_GROUP];
 #pragma unroll
     for (int k = 0; k < RP_GROUP; ++k) acc[k] = 0.0;
-    int r = rfirst, c = cfirst;
-    for (long pe = p0; pe < e1; pe += 256) {
-      const long e = static_cast<long>(r) * n + c;
-      const c128 w = Vi[m + e];
-      const c128 v = part ? V[m + e] : c128{0.0, 0.0};
-      const double f = (c == r) ? 1.0 : 2.0;
+    const long first = e0 + threadIdx.x;
+    int r = static_cast<int>(first / n), c = static_cast<int>(first % n);
+    for (long e = first; e < e1; e += 256) {
+      if (c <= r) {
+        const c128 w = Vi[m + e];
+        const c128 v = part ? V[m + e] : c128{0.0, 0.0};
+        const double f = (c == r) ? 1.0 : 2.0;
 #pragma unroll
-      for (int k = 0; k < RP_GROUP; ++k) {
-        if (k < nb) {
-          const long off = static_cast<long>(bg + k * vmod) * sA + e;
-          const c128 pv = P[off];
-          A[off] = c128{w.x - pv.x, w.y - pv.y};
-          acc[k] += f * (v.x * pv.x + v.y * pv.y);  // Re(conj(v) p)
+        for (int k = 0; k < RP_GROUP; ++k) {
+          if (k < nb) {
+            const long off = static_cast<long>(bg + k * vmod) * sA + e;
+            const c128 pv = P[off];
+            A[off] = c128{w.x - pv.x, w.y - pv.y};
+            acc[k] += f * (v.x * pv.x + v.y * pv.y);  // Re(conj(v) p)
+          }
         }
       }
-      c += 256;  // next element of this thread: 256 further in packed order
-      while (c > r) {
-        c -= r + 1;
+      c += 256;  // next element of this thread: 256 further in row-major order
+      while (c >= n) {
+        c -= n;
         ++r;
       }
     }
@@ -604,7 +598,7 @@ void chol_logdet(hipStream_t s, const c128* L, long sL, int n, double* ldet, int
 }
 
 int resolvent_prep_parts(int n) {
-  return static_cast<int>((static_cast<long>(n) * (n + 1) / 2 + RP_ELEMS - 1) / RP_ELEMS);
+  return static_cast<int>((static_cast<long>(n) * n + RP_ELEMS - 1) / RP_ELEMS);
 }
 
 void resolvent_prep(hipStream_t s, const c128* V, const c128* Vi, long sV, int vmod, int voff, const c128* P, c128* A,
