@@ -91,3 +91,28 @@ def test_batch_graph_replays_match_eager(orc, torch, precision, trev, dyson, lan
             assert relmax(x["W"].cpu().numpy(), ref["W_qt"]) < tol
     finally:
         b.close()
+
+
+def test_batch_graph_follows_gemm_mode(orc, torch):
+    """The process-wide GEMM settings are part of the captured graph's key: after flag 12
+    (complex products by three real products) changes, the next batch call captures again
+    and matches the systems run one by one with the new setting (1e-13), not the graph of
+    the old one."""
+    print("case: batch graph and flag 12 systems=%s" % (SYSTEMS,), flush=True)
+    systems = build(orc, torch, 1, False, 2)
+    b = ThcBatch([x["ctx"] for x in systems], use_graph=True, tune=False)
+    args = [[x[k] for x in systems] for k in ("G", "X", "V", "Sig", "W")]
+    try:
+        for mode in (0, 13, 0):
+            systems[0]["ctx"].set_flag(12, mode)
+            b.gw_iteration(*args)  # eager pass (first call) or capture
+            refs = [eager(x) for x in systems]
+            for _ in range(2):
+                b.gw_iteration(*args)
+                torch.cuda.synchronize()
+                for x, (S, W, e) in zip(systems, refs):
+                    assert relmax(x["Sig"].cpu().numpy(), S) < 1e-13, mode
+                    assert relmax(x["W"].cpu().numpy(), W) < 1e-13, mode
+    finally:
+        systems[0]["ctx"].set_flag(12, 0)
+        b.close()

@@ -157,11 +157,16 @@ def test_qp_and_gm_nk8_nmu200(nk8_nmu200, orc, torch):
     ((2, 2, 2), "1e3.5e-18", 6, 24),      # 80 tau points: iW -> tau with 41 frequencies (11 K blocks)
     ((1, 1, 1), "1e4.1e-15", 5, 20),      # 104 tau points: tau <-> iW by the GEMM (T has 53 rows)
 ])
-def test_iteration_shapes(orc, torch, kgrid, table, no, nmu):
+@pytest.mark.parametrize("g3", [0, 1])  # flag 12: FP64 complex products by four or three real products
+def test_iteration_shapes(orc, torch, kgrid, table, no, nmu, g3):
     s, ia, tab, G = setup(orc, kgrid, no, nmu, table, sid=3)
     ref = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=True)
     ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
-    S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+    try:
+        ctx.set_flag(12, g3)
+        S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+    finally:
+        ctx.set_flag(12, 0)
     assert relmax(to_np(W), ref["W_qt"]) < TOL
     assert relmax(to_np(S), ref["Sigma"]) < TOL
     assert abs(ec - ref["Ec"]) < TOL * abs(ref["Ec"])

@@ -137,12 +137,17 @@ def test_resolvent_mixed_nk8_nmu200(nk8_nmu200, torch):
     ((1, 1, 1), "1e2.1e-15", 5, 70),
     ((2, 2, 2), "1e2.1e-15", 12, 129),    # one past a tile boundary
 ])
-def test_resolvent_iteration_shapes(orc, torch, kgrid, table, no, nmu):
+@pytest.mark.parametrize("g3", [0, 1])  # flag 12: FP64 complex products by four or three real products
+def test_resolvent_iteration_shapes(orc, torch, kgrid, table, no, nmu, g3):
     s, ia, tab, G = system(orc, kgrid, no, nmu, table, sid=5)
     ref = orc.gw_iteration(tab, G, s.X, s.V, s.kgrid, with_ec=True)
     ctx = ThcGW(s.kgrid, s.no, s.nmu, ia)
     ctx.set_flag(6, 2)
-    S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+    try:
+        ctx.set_flag(12, g3)
+        S, W, ec = ctx.gw_iteration(G, s.X, s.V, with_ec=True)
+    finally:
+        ctx.set_flag(12, 0)
     assert relmax(to_np(W), ref["W_qt"]) < TOL
     assert relmax(to_np(S), ref["Sigma"]) < TOL
     assert abs(ec - ref["Ec"]) < TOL * abs(ref["Ec"])
