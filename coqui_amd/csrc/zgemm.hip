@@ -8,7 +8,8 @@
 // All matrices are row-major with explicit leading dimensions.  One 256-thread
 // workgroup (four waves, 2 x 2) computes a BM x BN tile of C; each wave owns a
 // (BM/2) x (BN/2) sub-tile made of v_mfma_f64_16x16x4_f64 blocks with separate real
-// and imaginary accumulators (a complex multiply-add is four real MFMAs).  Two tile
+// and imaginary accumulators (a complex multiply-add is four real MFMAs, or three in the
+// G3 variant: Ar Br, Ai Bi and (Ar + Ai)(Br + Bi), combined after the K loop).  Two tile
 // shapes are built: 128 x 128 (4 x 4 MFMA blocks per wave, accumulators in 256 AGPRs,
 // one wave per SIMD) for large products and 64 x 64 (2 x 2 blocks) for small ones.
 // K is staged through LDS in chunks of 8 complex values, double buffered, so the next
@@ -732,12 +733,12 @@ hipEvent_t pool_event() {
   return e;
 }
 
-// Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128 (complex<double> only).
-// three real products per complex product on 64 x 64 tiles: bit 0 FP64 arithmetic, bit 1 FP32
+// Three real products per complex product on 64 x 64 tiles: bit 0 FP64 arithmetic, bit 1 FP32
 // without FP64 folding, bit 2 FP32 with FP64 folding (not the split-BF16 products); bit 3 is
-// read by the callers (the FP32 projections of G for Sigma, through GEMM_G3)
+// read by the callers (the FP32 projections of G for Sigma, through GEMM_G3).
 std::atomic<int> g_complex3{0};
 
+// Tile shapes: 0 = 64 x 64, 1 = 128 x 64, 2 = 128 x 128 (complex<double> only).
 template <int OPA, int OPB>
 void launch(hipStream_t s, Args a, int shape) {
   const int bm = shape == 0 ? 64 : 128, bn = shape == 2 ? 128 : 64;
