@@ -690,7 +690,9 @@ def run_scgw(args):
         a, w = sc._buffers()
         ctx.pi(sc.G, sc.X, out=a)
         ctx.tau_to_w(a, out=w)
-        Pi0 = w.clone()
+        # the copy of Pi(iW), as large as w, is kept in host memory: the device holds w, the
+        # library's Dyson chunk buffers (up to 60% of the free memory) and the training pairs
+        Pi0 = w.cpu()
         for mode in ("direct", "none", "woodbury", "previous", "learned"):
             w.copy_(Pi0)
             torch.cuda.synchronize()
